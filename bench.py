@@ -1,0 +1,165 @@
+"""Benchmark of the einsum path on one MI355X.
+
+Workload (one "step"): the CCD W intermediates at o=24, v=96 in FP64 with inputs resident
+in HBM -- Wmnij += 0.25 t_ijef g_mnef (GEMM), Wabef += 0.25 g_mnef t_mnab (GEMM),
+Wmbej += -0.5 t_jnfb g_mnef (generic, non-GEMM layout), E = 0.25 t.g (DOT) and the
+permute ijab -> iajb. `value` is the step's algorithmic FP64 FLOP / step time (GFLOP/s).
+
+Prints one JSON line with the roofline of the dominant kernel (Wabef, FP64 MFMA bound,
+timed with HIP events on the stream the kernels run on) and the CPU baseline (the
+oracle's restatement of the reference CPU einsum timed on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+FP64_MFMA_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--o", type=int, default=24)
+    p.add_argument("--v", type=int, default=96)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--traffic", type=str, default=None, help="JSON file with measured HBM bytes per Wabef launch")
+    return p.parse_args()
+
+
+def cpu_baseline(I, o, v, flops):
+    """Times the oracle (reference CPU einsum restated: dispatch + host BLAS / generic loop
+    nest) on one full step of the same workload."""
+    import numpy as np
+
+    from oracle import pyoracle
+
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", str(cores))
+    pyoracle.set_threads(cores)
+    W = {"Wmnij": I.g_oooo.copy(), "Wabef": I.g_vvvv.copy(), "Wmbej": I.g_ovvo.copy()}
+    E = np.zeros(())
+    P = np.empty((o, v, o, v))
+    t0 = time.perf_counter()
+    pyoracle.einsum(1.0, "mnij", W["Wmnij"], 0.25, "ijef", I.t_oovv, "mnef", I.g_oovv)
+    pyoracle.einsum(1.0, "abef", W["Wabef"], 0.25, "mnef", I.g_oovv, "mnab", I.t_oovv)
+    pyoracle.einsum(1.0, "mbej", W["Wmbej"], -0.5, "jnfb", I.t_oovv, "mnef", I.g_oovv)
+    pyoracle.einsum(0.0, "", E, 0.25, "ijab", I.t_oovv, "ijab", I.g_oovv)
+    pyoracle.permute(0.0, "iajb", P, 1.0, "ijab", I.t_oovv)
+    dt = time.perf_counter() - t0
+    total = sum(flops.values())
+    return {"value": round(total / dt / 1e9, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+            "sample": f"one full step (Wmnij, Wabef, Wmbej, E, permute) at o={o} v={v}, {dt:.2f} s; "
+                      f"host BLAS: {os.path.basename(pyoracle.blas_name())}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import einsums_amd.core as core
+    from einsums_amd import workloads
+
+    o, v = args.o, args.v
+    I = workloads.ccd_inputs(o, v, seed=7)
+    flops = workloads.ccd_flops(o, v)
+    nbytes = workloads.ccd_bytes(o, v)
+    dev = lambda a: torch.from_numpy(a).to("cuda")
+    t, g = dev(I.t_oovv), dev(I.g_oovv)
+    W = {"Wmnij": dev(I.g_oooo), "Wabef": dev(I.g_vvvv), "Wmbej": dev(I.g_ovvo)}
+    E = torch.zeros((), dtype=torch.float64, device="cuda")
+    P = torch.empty((o, v, o, v), dtype=torch.float64, device="cuda")
+    plans = {
+        "Wmnij": (core.compile_plan("mnij", "ijef", "mnef"), t, g, 0.25, 1.0),
+        "Wabef": (core.compile_plan("abef", "mnef", "mnab"), g, t, 0.25, 1.0),
+        "Wmbej": (core.compile_plan("mbej", "jnfb", "mnef"), t, g, -0.5, 1.0),
+        "E": (core.compile_plan("", "ijab", "ijab"), t, g, 0.25, 0.0),
+    }
+    names = ["Wmnij", "Wabef", "Wmbej", "E", "permute"]
+    ev = {n: [] for n in names}
+
+    def step(record):
+        for n in names:
+            if record:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+            if n == "permute":
+                core.permute(0.0, "iajb", P, 1.0, "ijab", t)
+            else:
+                plan, A, B, abp, cp = plans[n]
+                C = E if n == "E" else W[n]
+                plan.execute(cp, C, abp, A, B)
+            if record:
+                e1.record()
+                ev[n].append((e0, e1))
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    # timed region: K steps bracketed by device synchronisation
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = {n: float(np.mean([a.elapsed_time(b) for a, b in ev[n]])) for n in names}  # ms
+    ms_per_step = wall * 1e3 / args.steps
+    total_flop = sum(flops.values())
+    algs = {n: core.algorithm_name(plans[n][0].last_algorithm) for n in names if n in plans}
+
+    dom = "Wabef"
+    achieved = flops[dom] / (per[dom] * 1e-3) / 1e12
+    traffic = None
+    if args.traffic and Path(args.traffic).exists():
+        traffic = json.loads(Path(args.traffic).read_text()).get("Wabef_bytes")
+    kernels = {}
+    for n in names:
+        gf = flops[n] / (per[n] * 1e-3) / 1e9 if flops[n] else 0.0
+        gbs = nbytes[n] / (per[n] * 1e-3) / 1e9
+        bound = "mfma" if n in ("Wmnij", "Wabef", "Wmbej") else "hbm"
+        frac = gf / 1e3 / FP64_MFMA_PEAK_TFLOPS if bound == "mfma" else gbs / HBM_PEAK_GBS
+        kernels[n] = {"ms": round(per[n], 4), "gflops": round(gf, 1), "gbs": round(gbs, 1), "bound": bound,
+                      "frac": round(frac, 4), "algorithm": algs.get(n, "PERMUTE")}
+    out = {
+        "metric": "einsum GFLOP/s & % of MFMA/HBM peak; split-prec vs plain FP64 at <=1e-11 rel err",
+        "value": round(total_flop / (ms_per_step * 1e-3) / 1e9, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"CCD W intermediates o={o} v={v} FP64: Wmnij, Wabef, Wmbej, E, permute ijab->iajb",
+                   "o": o, "v": v, "step_gflop": round(total_flop / 1e9, 3)},
+        "roofline": {"bound": "mfma", "kernel": "Wabef (gemm_dense_kernel, FP64 MFMA)", "achieved": round(achieved, 3),
+                     "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
+                     "traffic": traffic},
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(I, o, v, flops)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,255 @@
+"""Python surface of the einsum path, shaped like ``einsums.core`` of EinsumsPy.
+
+``compile_plan(C, A, B)`` analyses three index strings and returns a plan whose class
+names the algorithm, exactly like the reference
+(libs/EinsumsPy/TensorAlgebra/src/PyTensorAlgebra.cpp:169-300, classes exported at
+libs/EinsumsPy/TensorAlgebra/src/Export.cpp:14-146). ``plan.execute(C_prefactor, C,
+AB_prefactor, A, B)`` runs it on the GPU through the C ABI. Operands are torch tensors
+on the GPU (any strides), ``GPUView`` wrappers of host arrays, host numpy arrays (staged
+through device memory), or equally long lists of those (one call per entry).
+
+There is no CPU path: every execute runs HIP kernels from libeinsums_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Any, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (DOT, DIRECT, GEMM, GEMV, GENERIC, GER, INDETERMINATE, ALGORITHM_NAMES, DimensionError,
+                   EinsumsError, RankError, check, lib, make_desc)
+
+__all__ = [
+    "compile_plan", "EinsumGenericPlan", "EinsumDotPlan", "EinsumDirectProductPlan", "EinsumGerPlan",
+    "EinsumGemvPlan", "EinsumGemmPlan", "GPUView", "COPY", "MAP", "DEVICE_TENSOR", "gpu_enabled", "einsum",
+    "permute", "stream_handle", "DimensionError", "RankError", "EinsumsError",
+]
+
+# GPUView modes (libs/EinsumsPy/GPU/include/EinsumsPy/GPU/PyGPUView.hpp:27-46)
+COPY, MAP, DEVICE_TENSOR = 0, 1, 2
+
+_SUPPORTED = (torch.float64, torch.float32)
+
+
+def gpu_enabled() -> bool:
+    """True when a HIP device is usable (the reference's ein.core.gpu_enabled)."""
+    return torch.cuda.is_available()
+
+
+def stream_handle(stream: torch.cuda.Stream | None = None) -> ctypes.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class GPUView:
+    """Device mirror of a host array (reference PyGPUView, COPY and MAP modes).
+
+    COPY: the device copy is refreshed only by ``update_H2D``/``update_D2H``.
+    MAP: host memory is kept in step automatically after every execute that writes it.
+    (The reference maps the host buffer with hipHostRegister; this backend copies,
+    because registering existing host memory is not available on every system.)
+    """
+
+    def __init__(self, array, mode: int = COPY):
+        if isinstance(array, torch.Tensor):
+            host = array
+        else:
+            host = torch.from_numpy(np.asarray(array))
+        if host.dtype not in _SUPPORTED:
+            raise TypeError(f"GPUView: unsupported dtype {host.dtype}")
+        self._host = host
+        self._array = array
+        self.mode = mode
+        self.tensor = host.to("cuda")
+
+    def update_H2D(self) -> None:
+        self.tensor.copy_(self._host)
+
+    def update_D2H(self) -> None:
+        self._host.copy_(self.tensor.cpu())
+
+    @property
+    def shape(self):
+        return tuple(self.tensor.shape)
+
+    def dim(self, i: int) -> int:
+        return self.tensor.shape[i]
+
+    def rank(self) -> int:
+        return self.tensor.dim()
+
+
+def _as_device(x, write: bool):
+    """Returns (device tensor, finaliser)."""
+    if isinstance(x, GPUView):
+        fin = (lambda: x.update_D2H()) if (write and x.mode == MAP) else None
+        return x.tensor, fin
+    if isinstance(x, torch.Tensor):
+        if x.is_cuda:
+            return x, None
+        dev = x.to("cuda")
+        return dev, ((lambda: x.copy_(dev.cpu())) if write else None)
+    if isinstance(x, np.ndarray):
+        host = torch.from_numpy(x)
+        dev = host.to("cuda")
+        return dev, ((lambda: host.copy_(dev.cpu())) if write else None)
+    raise TypeError(f"unsupported operand type {type(x).__name__}")
+
+
+def _desc(t: torch.Tensor, idx: str):
+    if t.dim() != len(idx):
+        raise RankError(f"rank of tensor ({t.dim()}) does not match index string {idx!r}")
+    return make_desc(idx, t.shape, t.stride())
+
+
+def _fn(name: str, dtype: torch.dtype):
+    return getattr(lib, f"{name}_{'f64' if dtype == torch.float64 else 'f32'}")
+
+
+def _labels_only_desc(idx: str):
+    return make_desc(idx, [1] * len(idx), [1] * len(idx))
+
+
+def classify(C_indices: str, A_indices: str, B_indices: str) -> int:
+    out = ctypes.c_int32(-1)
+    check(lib.ein_einsum_classify(ctypes.byref(_labels_only_desc(C_indices)), ctypes.byref(_labels_only_desc(A_indices)),
+                                  ctypes.byref(_labels_only_desc(B_indices)), ctypes.byref(out)), "classify")
+    return out.value
+
+
+class EinsumGenericPlan:
+    """Plan for an arbitrary contraction (reference PyEinsumGenericPlan)."""
+
+    algorithm = GENERIC
+
+    def __init__(self, C_indices: str, A_indices: str, B_indices: str):
+        self.C_indices = C_indices
+        self.A_indices = A_indices
+        self.B_indices = B_indices
+        self.last_algorithm = INDETERMINATE
+
+    def __repr__(self) -> str:
+        return f"{type(self).__name__}({self.C_indices!r} <- {self.A_indices!r} * {self.B_indices!r})"
+
+    def execute(self, C_prefactor, C, AB_prefactor, A, B, stream: torch.cuda.Stream | None = None) -> None:
+        if isinstance(C, (list, tuple)):
+            if not (isinstance(A, (list, tuple)) and isinstance(B, (list, tuple))) or not (len(C) == len(A) == len(B)):
+                raise ValueError("The lengths of the collections need to be the same!")
+            for c, a, b in zip(C, A, B):
+                self.execute(C_prefactor, c, AB_prefactor, a, b, stream)
+            return
+        dC, finC = _as_device(C, write=True)
+        dA, _ = _as_device(A, write=False)
+        dB, _ = _as_device(B, write=False)
+        dtype = dC.dtype
+        if dtype not in _SUPPORTED or dA.dtype != dtype or dB.dtype != dtype:
+            raise TypeError(f"execute: operands must share float64 or float32, got {dC.dtype}, {dA.dtype}, {dB.dtype}")
+        cd, ad, bd = _desc(dC, self.C_indices), _desc(dA, self.A_indices), _desc(dB, self.B_indices)
+        used = ctypes.c_int32(-1)
+        fn = _fn("ein_einsum_execute", dtype)
+        status = fn(self.algorithm, float(C_prefactor), ctypes.byref(cd), _ptr(dC), float(AB_prefactor),
+                    ctypes.byref(ad), _ptr(dA), ctypes.byref(bd), _ptr(dB), ctypes.byref(used), stream_handle(stream))
+        check(status, f"einsum {self.C_indices} <- {self.A_indices},{self.B_indices}")
+        self.last_algorithm = used.value
+        if finC is not None:
+            torch.cuda.current_stream().synchronize()
+            finC()
+
+
+class EinsumDotPlan(EinsumGenericPlan):
+    algorithm = DOT
+
+
+class EinsumDirectProductPlan(EinsumGenericPlan):
+    algorithm = DIRECT
+
+
+class EinsumGerPlan(EinsumGenericPlan):
+    algorithm = GER
+
+
+class EinsumGemvPlan(EinsumGenericPlan):
+    algorithm = GEMV
+
+
+class EinsumGemmPlan(EinsumGenericPlan):
+    algorithm = GEMM
+
+
+_PLAN_CLASSES = {GENERIC: EinsumGenericPlan, DOT: EinsumDotPlan, DIRECT: EinsumDirectProductPlan, GER: EinsumGerPlan,
+                 GEMV: EinsumGemvPlan, GEMM: EinsumGemmPlan}
+
+
+def compile_plan(C_indices: str, A_indices: str, B_indices: str) -> EinsumGenericPlan:
+    """Choose the algorithm for ``C[C_indices] <- A[A_indices] * B[B_indices]``."""
+    alg = classify(C_indices, A_indices, B_indices)
+    return _PLAN_CLASSES[alg](C_indices, A_indices, B_indices)
+
+
+def einsum(C_indices: str, C, A_indices: str, A, B_indices: str, B, C_prefactor: float = 0.0,
+           AB_prefactor: float = 1.0, stream=None) -> int:
+    """C = C_prefactor*C + AB_prefactor * contraction; returns the algorithm that ran."""
+    plan = compile_plan(C_indices, A_indices, B_indices)
+    plan.execute(C_prefactor, C, AB_prefactor, A, B, stream)
+    return plan.last_algorithm
+
+
+def contract(C_indices: str, C, A_indices: str, A, B_indices: str, B, C_prefactor: float = 0.0,
+             AB_prefactor: float = 1.0, stream=None) -> None:
+    """Always the generic contraction kernel (the reference's OnlyUseGenericAlgorithm)."""
+    plan = EinsumGenericPlan(C_indices, A_indices, B_indices)
+    plan.execute(C_prefactor, C, AB_prefactor, A, B, stream)
+
+
+def permute(C_prefactor: float, C_indices: str, C: torch.Tensor, A_prefactor: float, A_indices: str, A: torch.Tensor,
+            stream=None) -> None:
+    """C = C_prefactor*C + A_prefactor*A with axes relabelled (reference
+    ``permute(beta, Indices{...}, &C, alpha, Indices{...}, A)``, Permute.hpp:330-407)."""
+    if sorted(C_indices) != sorted(A_indices) or len(set(A_indices)) != len(A_indices):
+        raise ValueError(f"permute: {C_indices!r} is not a permutation of {A_indices!r}")
+    dC, finC = _as_device(C, write=True)
+    dA, _ = _as_device(A, write=False)
+    if dC.dtype != dA.dtype or dC.dtype not in _SUPPORTED:
+        raise TypeError("permute: operands must share float64 or float32")
+    r = dA.dim()
+    if dC.dim() != r or len(A_indices) != r:
+        raise RankError("permute: rank mismatch")
+    perm = [A_indices.index(ch) for ch in C_indices]
+    for i in range(r):
+        if dC.shape[i] != dA.shape[perm[i]]:
+            raise DimensionError("permute: extents do not match")
+    I64 = ctypes.c_int64 * max(r, 1)
+    I32 = ctypes.c_int32 * max(r, 1)
+    status = _fn("ein_permute", dC.dtype)(r, I64(*dA.shape), I32(*perm), float(A_prefactor), _ptr(dA), I64(*dA.stride()),
+                                          float(C_prefactor), _ptr(dC), I64(*dC.stride()), stream_handle(stream))
+    check(status, "permute")
+    if finC is not None:
+        torch.cuda.current_stream().synchronize()
+        finC()
+
+
+def gemm(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch.Tensor, beta: float, C: torch.Tensor,
+         stream=None) -> None:
+    """Row-major C = alpha*op(A) op(B) + beta*C on 2-D tensors (linear_algebra::gemm)."""
+    for t in (A, B, C):
+        if t.dim() != 2 or t.stride(1) != 1:
+            raise ValueError("gemm: 2-D row-major tensors with unit column stride required")
+    ta, tb = transA.upper() != "N", transB.upper() != "N"
+    m = A.shape[1] if ta else A.shape[0]
+    k = A.shape[0] if ta else A.shape[1]
+    n = B.shape[0] if tb else B.shape[1]
+    # row-major C = op(A) op(B)  <=>  column-major C^T = op(B)^T op(A)^T
+    status = _fn("ein_gemm", C.dtype)(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0),
+                                      _ptr(A), A.stride(0), float(beta), _ptr(C), C.stride(0), stream_handle(stream))
+    check(status, "gemm")
+
+
+def algorithm_name(alg: int) -> str:
+    return ALGORITHM_NAMES.get(alg, "?")

@@ -1,0 +1,422 @@
+// Bandwidth-bound leaves for gfx950: gemv, dot, ger, direct product, scale, axpby and
+// whole-view tensor scaling. All reductions are deterministic (fixed-shape trees, no
+// atomics) so repeated calls give bit-identical results.
+//
+// Reference leaves replaced (column-major BLAS semantics kept):
+//   blas::gpu::gemv  -> hipblasDgemv_64   (hipBLASVendor/src/gemv.hip:65-113)
+//   blas::gpu::dot   -> hipblasDdot_64    (hipBLASVendor/src/dot.hip:29-108) and the
+//                       atomic dot_kernel  (LinearAlgebra/GPULinearAlgebra.hpp:36-100)
+//   blas::gpu::ger   -> hipblasDger_64    (hipBLASVendor/src/ger.hip:46-119)
+//   blas::gpu::dirprod -> ddirprod_kernel (hipBLASVendor/src/direct_prod.hip:20-271)
+//   blas::gpu::scal / axpby               (hipBLASVendor/src/scal.hip:25-68)
+#include <algorithm>
+
+#include "blas1.hpp"
+
+namespace ein {
+
+// ------------------------------------------------------------------- gemv
+// y[i] (+)= alpha * sum_j A[i + j*lda] x[j]; block = 64 rows x 4 waves over j.
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_n_kernel(int64_t m, int64_t n, T alpha, T const *__restrict__ A, int64_t lda,
+                                                     T const *__restrict__ x, int64_t incx, T beta, T *__restrict__ y,
+                                                     int64_t incy, int64_t jchunk, T *__restrict__ partial) {
+    __shared__ T red[4][64];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int64_t const i = int64_t(blockIdx.x) * 64 + lane;
+    int64_t const jb = int64_t(blockIdx.y) * jchunk;
+    int64_t const je = std::min<int64_t>(n, jb + jchunk);
+    T acc = T(0);
+    if (i < m) {
+        for (int64_t j = jb + wave; j < je; j += 4) acc += A[i + j * lda] * x[j * incx];
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && i < m) {
+        T s = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        if (partial != nullptr) {
+            partial[int64_t(blockIdx.y) * m + i] = s;
+        } else {
+            T v = alpha * s;
+            if (beta != T(0)) v += beta * y[i * incy];
+            y[i * incy] = v;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_n_finish(int64_t m, int splits, T alpha, T const *__restrict__ partial,
+                                                     T beta, T *__restrict__ y, int64_t incy) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    T s = T(0);
+    for (int k = 0; k < splits; ++k) s += partial[int64_t(k) * m + i];
+    T v = alpha * s;
+    if (beta != T(0)) v += beta * y[i * incy];
+    y[i * incy] = v;
+}
+
+// y[j] (+)= alpha * sum_i A[i + j*lda] x[i]; one wave per column, lanes along i.
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, T alpha, T const *__restrict__ A, int64_t lda,
+                                                     T const *__restrict__ x, int64_t incx, T beta, T *__restrict__ y,
+                                                     int64_t incy) {
+    int const lane = threadIdx.x & 63;
+    int64_t const j = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (j >= n) return;
+    T const *col = A + j * lda;
+    T acc0 = T(0), acc1 = T(0);
+    int64_t i = lane;
+    for (; i + 64 < m; i += 128) {
+        acc0 += col[i] * x[i * incx];
+        acc1 += col[i + 64] * x[(i + 64) * incx];
+    }
+    if (i < m) acc0 += col[i] * x[i * incx];
+    T s = wave_reduce_sum(acc0 + acc1);
+    if (lane == 0) {
+        T v = alpha * s;
+        if (beta != T(0)) v += beta * y[j * incy];
+        y[j * incy] = v;
+    }
+}
+
+template <typename T>
+int gemv_impl(char trans, int64_t m, int64_t n, T alpha, T const *A, int64_t lda, T const *x, int64_t incx, T beta, T *y,
+              int64_t incy, hipStream_t stream) {
+    bool const notrans = trans == 'N' || trans == 'n';
+    if (!notrans && !(trans == 'T' || trans == 't' || trans == 'C' || trans == 'c')) return fail(-1, "gemv: invalid trans");
+    if (m < 0) return fail(-2, "gemv: m < 0");
+    if (n < 0) return fail(-3, "gemv: n < 0");
+    if (lda < std::max<int64_t>(1, m)) return fail(-6, "gemv: lda too small");
+    if (incx == 0) return fail(-8, "gemv: incx == 0");
+    if (incy == 0) return fail(-11, "gemv: incy == 0");
+    int64_t const leny = notrans ? m : n;
+    int64_t const lenx = notrans ? n : m;
+    if (leny == 0) return EIN_SUCCESS;
+    // negative increments address from the far end, as in BLAS
+    if (incx < 0) x = x - (lenx - 1) * incx;
+    if (incy < 0) y = y - (leny - 1) * incy;
+    if (lenx == 0 || alpha == T(0)) return scale_strided<T>(leny, beta, y, incy, stream);
+    if (notrans) {
+        int64_t rb     = ceil_div(m, 64);
+        int64_t target = int64_t(num_cus()) * 4;
+        int splits     = int(std::max<int64_t>(1, std::min<int64_t>(ceil_div(target, rb), ceil_div(n, 64))));
+        splits         = std::min(splits, 1024);
+        int64_t jchunk = ceil_div(n, splits);
+        splits         = int(ceil_div(n, jchunk));
+        T *partial     = nullptr;
+        if (splits > 1) {
+            void *ws = nullptr;
+            EIN_TRY(workspace(stream, size_t(splits) * size_t(m) * sizeof(T), &ws));
+            partial = static_cast<T *>(ws);
+        }
+        dim3 grid{uint32_t(rb), uint32_t(splits)};
+        gemv_n_kernel<T><<<grid, 256, 0, stream>>>(m, n, alpha, A, lda, x, incx, beta, y, incy, jchunk, partial);
+        EIN_LAUNCH_CHECK();
+        if (splits > 1) {
+            gemv_n_finish<T><<<uint32_t(ceil_div(m, 256)), 256, 0, stream>>>(m, splits, alpha, partial, beta, y, incy);
+            EIN_LAUNCH_CHECK();
+        }
+    } else {
+        gemv_t_kernel<T><<<uint32_t(ceil_div(n, 4)), 256, 0, stream>>>(m, n, alpha, A, lda, x, incx, beta, y, incy);
+        EIN_LAUNCH_CHECK();
+    }
+    return EIN_SUCCESS;
+}
+
+// -------------------------------------------------------------------- dot
+template <typename T>
+__global__ __launch_bounds__(256) void dot_partial_kernel(int64_t n, T const *__restrict__ x, int64_t incx,
+                                                          T const *__restrict__ y, int64_t incy, T *__restrict__ partial) {
+    __shared__ T red[4];
+    T acc0 = T(0), acc1 = T(0);
+    int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+    int64_t e            = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (incx == 1 && incy == 1) {
+        for (; e + stride < n; e += 2 * stride) {
+            acc0 += x[e] * y[e];
+            acc1 += x[e + stride] * y[e + stride];
+        }
+        if (e < n) acc0 += x[e] * y[e];
+    } else {
+        for (; e < n; e += stride) acc0 += x[e * incx] * y[e * incy];
+    }
+    T s = wave_reduce_sum(acc0 + acc1);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void reduce_finish_kernel(int parts, T const *__restrict__ partial, T alpha, T beta,
+                                                            T *__restrict__ result) {
+    __shared__ T red[4];
+    T acc = T(0);
+    for (int k = threadIdx.x; k < parts; k += 256) acc += partial[k];
+    T s = wave_reduce_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T tot = (red[0] + red[1]) + (red[2] + red[3]);
+        T v   = alpha * tot;
+        if (beta != T(0)) v += beta * result[0];
+        result[0] = v;
+    }
+}
+
+template <typename T>
+int dot_impl(int64_t n, T alpha, T const *x, int64_t incx, T const *y, int64_t incy, T beta, T *result,
+             hipStream_t stream) {
+    if (n < 0) return fail(-1, "dot: n < 0");
+    if (result == nullptr) return fail(-8, "dot: null result");
+    if (n > 0 && incx < 0) x = x - (n - 1) * incx;
+    if (n > 0 && incy < 0) y = y - (n - 1) * incy;
+    int parts = int(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256 * 8), int64_t(num_cus()) * 4)));
+    void *ws  = nullptr;
+    EIN_TRY(workspace(stream, size_t(parts) * sizeof(T), &ws));
+    T *partial = static_cast<T *>(ws);
+    dot_partial_kernel<T><<<parts, 256, 0, stream>>>(n, x, incx, y, incy, partial);
+    EIN_LAUNCH_CHECK();
+    reduce_finish_kernel<T><<<1, 256, 0, stream>>>(parts, partial, alpha, beta, result);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
+// -------------------------------------------------------------------- ger
+template <typename T>
+__global__ __launch_bounds__(256) void ger_kernel(int64_t m, int64_t n, T alpha, T const *__restrict__ x, int64_t incx,
+                                                  T const *__restrict__ y, int64_t incy, T *__restrict__ A, int64_t lda) {
+    int64_t const i = int64_t(blockIdx.x) * 64 + (threadIdx.x & 63);
+    if (i >= m) return;
+    T const xi = alpha * x[i * incx];
+    for (int64_t j = int64_t(blockIdx.y) * 4 + (threadIdx.x >> 6); j < n; j += int64_t(gridDim.y) * 4)
+        A[i + j * lda] += xi * y[j * incy];
+}
+
+template <typename T>
+int ger_impl(int64_t m, int64_t n, T alpha, T const *x, int64_t incx, T const *y, int64_t incy, T *A, int64_t lda,
+             hipStream_t stream) {
+    if (m < 0) return fail(-1, "ger: m < 0");
+    if (n < 0) return fail(-2, "ger: n < 0");
+    if (incx == 0) return fail(-5, "ger: incx == 0");
+    if (incy == 0) return fail(-7, "ger: incy == 0");
+    if (lda < std::max<int64_t>(1, m)) return fail(-9, "ger: lda too small");
+    if (m == 0 || n == 0 || alpha == T(0)) return EIN_SUCCESS;
+    if (incx < 0) x = x - (m - 1) * incx;
+    if (incy < 0) y = y - (n - 1) * incy;
+    int64_t rb = ceil_div(m, 64);
+    int64_t cb = std::min<int64_t>(ceil_div(n, 4), std::max<int64_t>(1, int64_t(num_cus()) * 8 / rb));
+    cb         = std::min<int64_t>(cb, 65535);
+    ger_kernel<T><<<dim3(uint32_t(rb), uint32_t(cb)), 256, 0, stream>>>(m, n, alpha, x, incx, y, incy, A, lda);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
+// ------------------------------------------------- dirprod / scale / axpby
+template <typename T>
+__global__ __launch_bounds__(256) void dirprod_kernel(int64_t n, T alpha, T const *__restrict__ x, int64_t incx,
+                                                      T const *__restrict__ y, int64_t incy, T beta, T *__restrict__ z,
+                                                      int64_t incz) {
+    int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) {
+        T v = alpha * (x[e * incx] * y[e * incy]);
+        if (beta != T(0)) v += beta * z[e * incz];
+        z[e * incz] = v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scale_kernel(int64_t n, T alpha, T *__restrict__ x, int64_t incx) {
+    int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride)
+        x[e * incx] = alpha == T(0) ? T(0) : alpha * x[e * incx];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void axpby_kernel(int64_t n, T alpha, T const *__restrict__ x, int64_t incx, T beta,
+                                                    T *__restrict__ y, int64_t incy) {
+    int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) {
+        T v = alpha * x[e * incx];
+        if (beta != T(0)) v += beta * y[e * incy];
+        y[e * incy] = v;
+    }
+}
+
+inline uint32_t stream_blocks(int64_t n) {
+    int64_t b = ceil_div(n, 256);
+    return uint32_t(std::max<int64_t>(1, std::min<int64_t>(b, int64_t(num_cus()) * 16)));
+}
+
+template <typename T>
+int scale_strided(int64_t n, T alpha, T *x, int64_t incx, hipStream_t stream) {
+    if (n <= 0 || alpha == T(1)) return EIN_SUCCESS;
+    if (incx < 0) x = x - (n - 1) * incx;
+    scale_kernel<T><<<stream_blocks(n), 256, 0, stream>>>(n, alpha, x, incx);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
+template <typename T>
+int dirprod_impl(int64_t n, T alpha, T const *x, int64_t incx, T const *y, int64_t incy, T beta, T *z, int64_t incz,
+                 hipStream_t stream) {
+    if (n < 0) return fail(-1, "dirprod: n < 0");
+    if (n == 0) return EIN_SUCCESS;
+    if (incx < 0) x = x - (n - 1) * incx;
+    if (incy < 0) y = y - (n - 1) * incy;
+    if (incz < 0) z = z - (n - 1) * incz;
+    dirprod_kernel<T><<<stream_blocks(n), 256, 0, stream>>>(n, alpha, x, incx, y, incy, beta, z, incz);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
+template <typename T>
+int axpby_impl(int64_t n, T alpha, T const *x, int64_t incx, T beta, T *y, int64_t incy, hipStream_t stream) {
+    if (n < 0) return fail(-1, "axpby: n < 0");
+    if (n == 0) return EIN_SUCCESS;
+    if (incx < 0) x = x - (n - 1) * incx;
+    if (incy < 0) y = y - (n - 1) * incy;
+    axpby_kernel<T><<<stream_blocks(n), 256, 0, stream>>>(n, alpha, x, incx, beta, y, incy);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
+// --------------------------------------------------- whole-view tensor scale
+template <typename T>
+__global__ __launch_bounds__(256) void tensor_scale_kernel(TensorShape S, T alpha, T *__restrict__ c) {
+    int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < S.total; e += stride) {
+        int64_t off = 0, rem = e;
+        for (int d = S.rank - 1; d >= 0; --d) {
+            int64_t q = rem / S.dims[d];
+            off += (rem - q * S.dims[d]) * S.strides[d];
+            rem = q;
+        }
+        c[off] = alpha == T(0) ? T(0) : alpha * c[off];
+    }
+}
+
+template <typename T>
+int tensor_scale(ein_tensor_desc const &C, T *c, T alpha, hipStream_t stream) {
+    if (alpha == T(1)) return EIN_SUCCESS;
+    TensorShape S{};
+    EIN_TRY(make_shape(C, S));
+    if (S.total == 0) return EIN_SUCCESS;
+    if (S.rank <= 1) return scale_strided<T>(S.total, alpha, c, S.rank == 0 ? 1 : S.strides[0], stream);
+    tensor_scale_kernel<T><<<stream_blocks(S.total), 256, 0, stream>>>(S, alpha, c);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
+// Collapses a descriptor to its distinct axes, merging axes that are contiguous with
+// the next one (row-major sense) so the decode loop is as short as possible.
+int make_shape(ein_tensor_desc const &C, TensorShape &S) {
+    if (C.rank < 0 || C.rank > EIN_MAX_RANK) return fail(EIN_ERROR_RANK, "tensor rank out of range");
+    S.rank  = 0;
+    S.total = 1;
+    for (int d = 0; d < C.rank; ++d) {
+        if (C.dims[d] < 0) return fail(EIN_ERROR_VALUE, "negative tensor extent");
+        S.total *= C.dims[d];
+        if (C.dims[d] == 1) continue;
+        if (S.rank > 0 && S.strides[S.rank - 1] == C.strides[d] * C.dims[d]) {
+            S.dims[S.rank - 1] *= C.dims[d];
+            S.strides[S.rank - 1] = C.strides[d];
+        } else {
+            S.dims[S.rank]    = C.dims[d];
+            S.strides[S.rank] = C.strides[d];
+            ++S.rank;
+        }
+    }
+    if (S.rank == 0 && S.total == 1) {
+        S.rank       = 1;
+        S.dims[0]    = 1;
+        S.strides[0] = 1;
+    }
+    return EIN_SUCCESS;
+}
+
+template int gemv_impl<double>(char, int64_t, int64_t, double, double const *, int64_t, double const *, int64_t, double,
+                               double *, int64_t, hipStream_t);
+template int gemv_impl<float>(char, int64_t, int64_t, float, float const *, int64_t, float const *, int64_t, float,
+                              float *, int64_t, hipStream_t);
+template int dot_impl<double>(int64_t, double, double const *, int64_t, double const *, int64_t, double, double *,
+                              hipStream_t);
+template int dot_impl<float>(int64_t, float, float const *, int64_t, float const *, int64_t, float, float *, hipStream_t);
+template int ger_impl<double>(int64_t, int64_t, double, double const *, int64_t, double const *, int64_t, double *,
+                              int64_t, hipStream_t);
+template int ger_impl<float>(int64_t, int64_t, float, float const *, int64_t, float const *, int64_t, float *, int64_t,
+                             hipStream_t);
+template int dirprod_impl<double>(int64_t, double, double const *, int64_t, double const *, int64_t, double, double *,
+                                  int64_t, hipStream_t);
+template int dirprod_impl<float>(int64_t, float, float const *, int64_t, float const *, int64_t, float, float *, int64_t,
+                                 hipStream_t);
+template int axpby_impl<double>(int64_t, double, double const *, int64_t, double, double *, int64_t, hipStream_t);
+template int axpby_impl<float>(int64_t, float, float const *, int64_t, float, float *, int64_t, hipStream_t);
+template int scale_strided<double>(int64_t, double, double *, int64_t, hipStream_t);
+template int scale_strided<float>(int64_t, float, float *, int64_t, hipStream_t);
+template int tensor_scale<double>(ein_tensor_desc const &, double *, double, hipStream_t);
+template int tensor_scale<float>(ein_tensor_desc const &, float *, float, hipStream_t);
+
+} // namespace ein
+
+extern "C" {
+
+int ein_gemv_f64(char trans, int64_t m, int64_t n, double alpha, const double *A, int64_t lda, const double *x, int64_t incx,
+                 double beta, double *y, int64_t incy, ein_stream_t stream) {
+    return ein::gemv_impl<double>(trans, m, n, alpha, A, lda, x, incx, beta, y, incy, ein::as_stream(stream));
+}
+int ein_gemv_f32(char trans, int64_t m, int64_t n, float alpha, const float *A, int64_t lda, const float *x, int64_t incx,
+                 float beta, float *y, int64_t incy, ein_stream_t stream) {
+    return ein::gemv_impl<float>(trans, m, n, alpha, A, lda, x, incx, beta, y, incy, ein::as_stream(stream));
+}
+int ein_dot_f64(int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy, double beta,
+                double *result, ein_stream_t stream) {
+    return ein::dot_impl<double>(n, alpha, x, incx, y, incy, beta, result, ein::as_stream(stream));
+}
+int ein_dot_f32(int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float beta, float *result,
+                ein_stream_t stream) {
+    return ein::dot_impl<float>(n, alpha, x, incx, y, incy, beta, result, ein::as_stream(stream));
+}
+int ein_ger_f64(int64_t m, int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy, double *A,
+                int64_t lda, ein_stream_t stream) {
+    return ein::ger_impl<double>(m, n, alpha, x, incx, y, incy, A, lda, ein::as_stream(stream));
+}
+int ein_ger_f32(int64_t m, int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float *A,
+                int64_t lda, ein_stream_t stream) {
+    return ein::ger_impl<float>(m, n, alpha, x, incx, y, incy, A, lda, ein::as_stream(stream));
+}
+int ein_dirprod_f64(int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy, double beta,
+                    double *z, int64_t incz, ein_stream_t stream) {
+    return ein::dirprod_impl<double>(n, alpha, x, incx, y, incy, beta, z, incz, ein::as_stream(stream));
+}
+int ein_dirprod_f32(int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float beta, float *z,
+                    int64_t incz, ein_stream_t stream) {
+    return ein::dirprod_impl<float>(n, alpha, x, incx, y, incy, beta, z, incz, ein::as_stream(stream));
+}
+int ein_scale_f64(int64_t n, double alpha, double *x, int64_t incx, ein_stream_t stream) {
+    if (n < 0) return ein::fail(-1, "scale: n < 0");
+    return ein::scale_strided<double>(n, alpha, x, incx, ein::as_stream(stream));
+}
+int ein_scale_f32(int64_t n, float alpha, float *x, int64_t incx, ein_stream_t stream) {
+    if (n < 0) return ein::fail(-1, "scale: n < 0");
+    return ein::scale_strided<float>(n, alpha, x, incx, ein::as_stream(stream));
+}
+int ein_axpby_f64(int64_t n, double alpha, const double *x, int64_t incx, double beta, double *y, int64_t incy,
+                  ein_stream_t stream) {
+    return ein::axpby_impl<double>(n, alpha, x, incx, beta, y, incy, ein::as_stream(stream));
+}
+int ein_axpby_f32(int64_t n, float alpha, const float *x, int64_t incx, float beta, float *y, int64_t incy,
+                  ein_stream_t stream) {
+    return ein::axpby_impl<float>(n, alpha, x, incx, beta, y, incy, ein::as_stream(stream));
+}
+int ein_tensor_scale_f64(const ein_tensor_desc *C, double *c, double alpha, ein_stream_t stream) {
+    if (C == nullptr) return -1;
+    return ein::tensor_scale<double>(*C, c, alpha, ein::as_stream(stream));
+}
+int ein_tensor_scale_f32(const ein_tensor_desc *C, float *c, float alpha, ein_stream_t stream) {
+    if (C == nullptr) return -1;
+    return ein::tensor_scale<float>(*C, c, alpha, ein::as_stream(stream));
+}
+
+} // extern "C"

@@ -1,0 +1,663 @@
+// Generic binary contraction on gfx950 (any label pattern) without atomics.
+//
+// Reference behaviour restated: einsum_generic_algorithm (TensorAlgebra/Backends/
+// BaseAlgebra.hpp:85-139, device launcher GPUTensorAlgebra.hpp:200-342): C is scaled by
+// C_prefactor (zeroed when it is 0), then every target point receives
+// AB_prefactor * sum over the link labels of A*B. Repeated labels on one tensor address a
+// diagonal (their strides add, as get_stride_for does in Detail/Utilities.hpp:631-652).
+//
+// Instead of one thread per point of the full index space with an atomicAdd per point,
+// the labels are split into four groups -- batch (in A, B, C), M (in C and A only, or C
+// only), N (in C and B only) and K (not in C) -- and the contraction becomes
+//     C[b, i, j] = alpha * sum_p A[b, i, p] * B[b, p, j] + beta * C[b, i, j]
+// over multi-indices decoded with fast 32-bit division. Large M and N go to an MFMA
+// strided-gather GEMM; otherwise each target point is reduced by one thread or one
+// wave (optionally split over K with a deterministic second pass).
+#include <algorithm>
+#include <cstdlib>
+#include <numeric>
+#include <vector>
+
+#include "contract.hpp"
+
+namespace ein {
+
+// ============================================================ host analysis
+
+int analyse_contraction(ein_tensor_desc const &C, ein_tensor_desc const &A, ein_tensor_desc const &B,
+                        ContractionAnalysis &out) {
+    ein_tensor_desc const *T[3] = {&C, &A, &B};
+    char const *names[3]        = {"C", "A", "B"};
+    int labels[3 * EIN_MAX_RANK];
+    out.nlab = 0;
+    for (int t = 0; t < 3; ++t) {
+        if (T[t]->rank < 0 || T[t]->rank > EIN_MAX_RANK)
+            return fail(EIN_ERROR_RANK, std::string("rank of ") + names[t] + " out of range");
+    }
+    for (int i = 0; i < 3 * EIN_MAX_RANK; ++i) {
+        out.ext[i] = -1;
+        out.sA[i] = out.sB[i] = out.sC[i] = 0;
+        out.inA[i] = out.inB[i] = out.inC[i] = false;
+    }
+    out.c_repeats = false;
+    out.c_size    = 1;
+    for (int t = 0; t < 3; ++t) {
+        for (int d = 0; d < T[t]->rank; ++d) {
+            int lab = T[t]->labels[d];
+            int64_t e = T[t]->dims[d];
+            if (e < 0) return fail(EIN_ERROR_VALUE, std::string("negative extent in ") + names[t]);
+            int u = -1;
+            for (int q = 0; q < out.nlab; ++q)
+                if (labels[q] == lab) u = q;
+            if (u < 0) {
+                u           = out.nlab++;
+                labels[u]   = lab;
+                out.ext[u]  = e;
+            } else if (out.ext[u] != e) {
+                return fail(EIN_ERROR_DIMENSION, "Inconsistent dimensions found!");
+            }
+            int64_t s = T[t]->strides[d];
+            if (t == 0) {
+                if (out.inC[u]) out.c_repeats = true;
+                out.inC[u] = true;
+                out.sC[u] += s;
+            } else if (t == 1) {
+                out.inA[u] = true;
+                out.sA[u] += s;
+            } else {
+                out.inB[u] = true;
+                out.sB[u] += s;
+            }
+        }
+    }
+    for (int d = 0; d < C.rank; ++d) out.c_size *= C.dims[d];
+    return EIN_SUCCESS;
+}
+
+namespace {
+
+struct Digit {
+    int64_t e;
+    int64_t s[3];
+};
+
+// Sort digits outer -> inner by descending key stride, then fold neighbours that are
+// contiguous in every operand (outer stride == inner stride * inner extent).
+std::vector<Digit> order_and_fold(std::vector<Digit> v, int nops, int key0, int key1) {
+    std::stable_sort(v.begin(), v.end(), [&](Digit const &x, Digit const &y) {
+        int64_t ax = std::llabs(x.s[key0]), ay = std::llabs(y.s[key0]);
+        if (ax != ay) return ax > ay;
+        return std::llabs(x.s[key1]) > std::llabs(y.s[key1]);
+    });
+    std::vector<Digit> out;
+    for (auto const &d : v) {
+        if (!out.empty()) {
+            bool ok = true;
+            for (int o = 0; o < nops; ++o) ok = ok && out.back().s[o] == d.s[o] * d.e;
+            if (ok) {
+                out.back().e *= d.e;
+                for (int o = 0; o < nops; ++o) out.back().s[o] = d.s[o];
+                continue;
+            }
+        }
+        out.push_back(d);
+    }
+    return out;
+}
+
+int64_t total_of(std::vector<Digit> const &v) {
+    int64_t t = 1;
+    for (auto const &d : v) t *= d.e;
+    return t;
+}
+
+int fill_group(IndexGroup &G, std::vector<Digit> const &v, int nops) {
+    if (int(v.size()) > GMAX) return fail(EIN_ERROR_UNSUPPORTED, "contraction: too many index digits in one group");
+    G.n     = int32_t(v.size());
+    int64_t t = total_of(v);
+    G.total = uint32_t(t < (int64_t{1} << 31) ? t : 0);
+    for (int d = 0; d < GMAX; ++d) {
+        G.ext[d] = 1;
+        G.fd[d]  = make_fastdiv(1);
+        for (int o = 0; o < 3; ++o) G.s[o][d] = 0;
+    }
+    for (int d = 0; d < G.n; ++d) {
+        if (v[d].e >= (int64_t{1} << 31)) return fail(EIN_ERROR_UNSUPPORTED, "contraction: extent above 2^31");
+        G.ext[d] = uint32_t(v[d].e);
+        G.fd[d]  = make_fastdiv(G.ext[d]);
+        for (int o = 0; o < nops; ++o) G.s[o][d] = v[d].s[o];
+    }
+    return EIN_SUCCESS;
+}
+
+} // namespace
+
+// ====================================================== MFMA gather GEMM
+
+// Marks a row/column outside the problem (offsets themselves may be negative).
+constexpr int64_t NO_ROW = INT64_MIN;
+
+template <typename T, int BM, int BN, int BK, bool A_I_FAST, bool B_J_FAST>
+__global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProblem<T> P) {
+    using Cfg = TileCfg<T, BM, BN, BK>;
+    __shared__ T As[2][BK * Cfg::LDA];
+    __shared__ T Bs[2][BK * Cfg::LDB];
+
+    int const tid  = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const wm   = wave & 1;
+    int const wn   = wave >> 1;
+
+    uint32_t const ntiles  = P.tiles_m * P.tiles_n;
+    uint32_t const logical = xcd_remap(blockIdx.x, ntiles);
+    uint32_t tm, tn;
+    tile_coords(logical, P.tiles_m, P.tiles_n, tm, tn);
+    int64_t const i0 = int64_t(tm) * BM;
+    int64_t const j0 = int64_t(tn) * BN;
+
+    int const split    = blockIdx.y;
+    int64_t const kbeg = int64_t(split) * P.k_chunk;
+    int64_t const kend = kbeg + P.k_chunk < P.k ? kbeg + P.k_chunk : P.k;
+
+    int64_t boff[3];
+    group_decode<3>(uint32_t(blockIdx.z), P.gb, boff);
+    T const *__restrict__ Ab = P.A + boff[0];
+    T const *__restrict__ Bb = P.B + boff[1];
+
+    constexpr int A_STEP = A_I_FAST ? GEMM_THREADS / BM : GEMM_THREADS / BK;
+    int const a_ii       = A_I_FAST ? tid % BM : tid / BK;
+    int const a_pp       = A_I_FAST ? tid / BM : tid % BK;
+    constexpr int B_STEP = B_J_FAST ? GEMM_THREADS / BN : GEMM_THREADS / BK;
+    int const b_jj       = B_J_FAST ? tid % BN : tid / BK;
+    int const b_pp       = B_J_FAST ? tid / BN : tid % BK;
+
+    // Row / column offsets fixed for the whole tile (-1 marks out of range).
+    constexpr int NRA = A_I_FAST ? 1 : Cfg::LA;
+    constexpr int NRB = B_J_FAST ? 1 : Cfg::LB;
+    int64_t rowA[NRA], colB[NRB];
+#pragma unroll
+    for (int r = 0; r < NRA; ++r) {
+        int64_t i = i0 + (A_I_FAST ? a_ii : a_ii + r * A_STEP);
+        int64_t o[2];
+        group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
+        rowA[r] = i < P.m ? o[0] : NO_ROW;
+    }
+#pragma unroll
+    for (int r = 0; r < NRB; ++r) {
+        int64_t j = j0 + (B_J_FAST ? b_jj : b_jj + r * B_STEP);
+        int64_t o[2];
+        group_decode<2>(uint32_t(j < P.n ? j : 0), P.gn, o);
+        colB[r] = j < P.n ? o[0] : NO_ROW;
+    }
+
+    T ra[Cfg::LA], rb[Cfg::LB];
+
+    auto load_slab = [&](int64_t k0) {
+        // k offsets for the p-fast operands: one decode serves both A and B when both
+        // walk p with the same lane mapping
+        int64_t kfix[2] = {0, 0};
+        bool kfix_ok    = false;
+        if (!A_I_FAST || !B_J_FAST) {
+            int64_t p = k0 + (A_I_FAST ? b_pp : a_pp);
+            kfix_ok   = p < kend;
+            group_decode<2>(uint32_t(kfix_ok ? p : 0), P.gk, kfix);
+        }
+#pragma unroll
+        for (int r = 0; r < Cfg::LA; ++r) {
+            T v = T(0);
+            if constexpr (A_I_FAST) {
+                int64_t p = k0 + a_pp + r * A_STEP;
+                if (rowA[0] != NO_ROW && p < kend) {
+                    int64_t o[2];
+                    group_decode<2>(uint32_t(p), P.gk, o);
+                    v = Ab[rowA[0] + o[0]];
+                }
+            } else {
+                if (rowA[r] != NO_ROW && kfix_ok) v = Ab[rowA[r] + kfix[0]];
+            }
+            ra[r] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < Cfg::LB; ++r) {
+            T v = T(0);
+            if constexpr (B_J_FAST) {
+                int64_t p = k0 + b_pp + r * B_STEP;
+                if (colB[0] != NO_ROW && p < kend) {
+                    int64_t o[2];
+                    group_decode<2>(uint32_t(p), P.gk, o);
+                    v = Bb[colB[0] + o[1]];
+                }
+            } else {
+                if (colB[r] != NO_ROW && kfix_ok) v = Bb[colB[r] + kfix[1]];
+            }
+            rb[r] = v;
+        }
+    };
+    auto store_slab = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < Cfg::LA; ++r) {
+            int ii = A_I_FAST ? a_ii : a_ii + r * A_STEP;
+            int pp = A_I_FAST ? a_pp + r * A_STEP : a_pp;
+            As[buf][pp * Cfg::LDA + ii] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < Cfg::LB; ++r) {
+            int jj = B_J_FAST ? b_jj : b_jj + r * B_STEP;
+            int pp = B_J_FAST ? b_pp + r * B_STEP : b_pp;
+            Bs[buf][pp * Cfg::LDB + jj] = rb[r];
+        }
+    };
+
+    WaveAcc<T, Cfg::FM, Cfg::FN> acc;
+    acc.zero();
+    int64_t const nslabs = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+    if (nslabs > 0) {
+        load_slab(kbeg);
+        store_slab(0);
+        __syncthreads();
+        for (int64_t s = 0; s < nslabs; ++s) {
+            int const buf = int(s & 1);
+            if (s + 1 < nslabs) load_slab(kbeg + (s + 1) * BK);
+            mfma_slab<T, BM, BN, BK>(As[buf], Bs[buf], acc, wm, wn, lane);
+            if (s + 1 < nslabs) store_slab(buf ^ 1);
+            __syncthreads();
+        }
+    }
+
+    int const r = lane & 15;
+    int const q = lane >> 4;
+    if (P.splits > 1) {
+        T *__restrict__ part = P.partial + (int64_t(split) * P.batch + blockIdx.z) * P.m * P.n;
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+            for (int b = 0; b < Cfg::FN; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
+                    int64_t j = j0 + wn * Cfg::WN + b * 16 + q * 4 + v;
+                    if (i < P.m && j < P.n) part[i + j * P.m] = acc.c[a][b][v];
+                }
+        return;
+    }
+    T *__restrict__ Cb = P.C + boff[2];
+    int64_t rowC[Cfg::FM];
+#pragma unroll
+    for (int a = 0; a < Cfg::FM; ++a) {
+        int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
+        int64_t o[2];
+        group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
+        rowC[a] = i < P.m ? o[1] : NO_ROW;
+    }
+#pragma unroll
+    for (int b = 0; b < Cfg::FN; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int64_t j = j0 + wn * Cfg::WN + b * 16 + q * 4 + v;
+            if (j >= P.n) continue;
+            int64_t o[2];
+            group_decode<2>(uint32_t(j), P.gn, o);
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a) {
+                if (rowC[a] == NO_ROW) continue;
+                T *cp = Cb + rowC[a] + o[1];
+                T val = P.alpha * acc.c[a][b][v];
+                if (P.beta != T(0)) val += P.beta * *cp;
+                *cp = val;
+            }
+        }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void gather_splitk_reduce_kernel(GatherProblem<T> P) {
+    int64_t const mn    = P.m * P.n;
+    int64_t const total = mn * P.batch;
+    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
+        int64_t bz  = e / mn;
+        int64_t rem = e - bz * mn;
+        int64_t j   = rem / P.m;
+        int64_t i   = rem - j * P.m;
+        T sum       = T(0);
+        for (int s = 0; s < P.splits; ++s) sum += P.partial[(int64_t(s) * P.batch + bz) * mn + rem];
+        int64_t ob[3], om[2], on[2];
+        group_decode<3>(uint32_t(bz), P.gb, ob);
+        group_decode<2>(uint32_t(i), P.gm, om);
+        group_decode<2>(uint32_t(j), P.gn, on);
+        T *cp = P.C + ob[2] + om[1] + on[1];
+        T val = P.alpha * sum;
+        if (P.beta != T(0)) val += P.beta * *cp;
+        *cp = val;
+    }
+}
+
+// ================================================ reduction regimes
+
+// Target group digits carry (A, B, C) strides; K digits carry (A, B).
+template <typename T>
+struct ReduceProblem {
+    IndexGroup gt, gk;
+    int64_t nt, nk;
+    T const *A;
+    T const *B;
+    T *C;
+    T alpha, beta;
+    int32_t splits;
+    int64_t k_chunk;
+    T *partial;
+};
+
+// One thread per target point; the K loop walks the digits as an odometer.
+template <typename T>
+__global__ __launch_bounds__(256) void reduce_thread_kernel(ReduceProblem<T> P) {
+    int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < P.nt; t += stride) {
+        int64_t o[3];
+        group_decode<3>(uint32_t(t), P.gt, o);
+        T acc = T(0);
+        if (P.nk > 0) {
+            uint32_t dig[GMAX];
+#pragma unroll
+            for (int d = 0; d < GMAX; ++d) dig[d] = 0;
+            int64_t oa = o[0], ob = o[1];
+            int const nd = P.gk.n;
+            for (int64_t p = 0; p < P.nk; ++p) {
+                acc += P.A[oa] * P.B[ob];
+                // advance the odometer (innermost digit last)
+                for (int d = nd - 1; d >= 0; --d) {
+                    oa += P.gk.s[0][d];
+                    ob += P.gk.s[1][d];
+                    if (++dig[d] < P.gk.ext[d]) break;
+                    oa -= int64_t(P.gk.ext[d]) * P.gk.s[0][d];
+                    ob -= int64_t(P.gk.ext[d]) * P.gk.s[1][d];
+                    dig[d] = 0;
+                }
+            }
+        }
+        T *cp = P.C + o[2];
+        T val = P.alpha * acc;
+        if (P.beta != T(0)) val += P.beta * *cp;
+        *cp = val;
+    }
+}
+
+// One wave per (target, K chunk); lanes stride the chunk, wave-level tree reduction.
+// With splits == 1 the wave writes C, otherwise a partial that the finish pass sums in
+// fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void reduce_wave_kernel(ReduceProblem<T> P) {
+    int const lane        = threadIdx.x & 63;
+    int64_t const w       = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    int64_t const nwork   = P.nt * P.splits;
+    if (w >= nwork) return;
+    int64_t const t  = w / P.splits;
+    int64_t const sp = w - t * P.splits;
+    int64_t o[3];
+    group_decode<3>(uint32_t(t), P.gt, o);
+    int64_t const kb = sp * P.k_chunk;
+    int64_t const ke = kb + P.k_chunk < P.nk ? kb + P.k_chunk : P.nk;
+    T acc0 = T(0), acc1 = T(0);
+    int64_t p = kb + lane;
+    for (; p + 64 < ke; p += 128) {
+        int64_t k0[2], k1[2];
+        group_decode<2>(uint32_t(p), P.gk, k0);
+        group_decode<2>(uint32_t(p + 64), P.gk, k1);
+        acc0 += P.A[o[0] + k0[0]] * P.B[o[1] + k0[1]];
+        acc1 += P.A[o[0] + k1[0]] * P.B[o[1] + k1[1]];
+    }
+    if (p < ke) {
+        int64_t k0[2];
+        group_decode<2>(uint32_t(p), P.gk, k0);
+        acc0 += P.A[o[0] + k0[0]] * P.B[o[1] + k0[1]];
+    }
+    T s = wave_reduce_sum(acc0 + acc1);
+    if (lane == 0) {
+        if (P.splits > 1) {
+            P.partial[t * P.splits + sp] = s;
+        } else {
+            T *cp = P.C + o[2];
+            T val = P.alpha * s;
+            if (P.beta != T(0)) val += P.beta * *cp;
+            *cp = val;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void reduce_finish_targets_kernel(ReduceProblem<T> P) {
+    int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < P.nt; t += stride) {
+        T s = T(0);
+        for (int sp = 0; sp < P.splits; ++sp) s += P.partial[t * P.splits + sp];
+        int64_t o[3];
+        group_decode<3>(uint32_t(t), P.gt, o);
+        T *cp = P.C + o[2];
+        T val = P.alpha * s;
+        if (P.beta != T(0)) val += P.beta * *cp;
+        *cp = val;
+    }
+}
+
+// ================================================================ driver
+
+namespace {
+
+template <typename T, int BM, int BN>
+void launch_gather(GatherProblem<T> const &P, bool a_i_fast, bool b_j_fast, dim3 grid, hipStream_t stream) {
+    constexpr int BK = 16;
+    if (a_i_fast && b_j_fast)
+        gemm_gather_kernel<T, BM, BN, BK, true, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
+    else if (a_i_fast)
+        gemm_gather_kernel<T, BM, BN, BK, true, false><<<grid, GEMM_THREADS, 0, stream>>>(P);
+    else if (b_j_fast)
+        gemm_gather_kernel<T, BM, BN, BK, false, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
+    else
+        gemm_gather_kernel<T, BM, BN, BK, false, false><<<grid, GEMM_THREADS, 0, stream>>>(P);
+}
+
+} // namespace
+
+template <typename T>
+int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefactor, ein_tensor_desc const &A, T const *a,
+                     ein_tensor_desc const &B, T const *b, hipStream_t stream) {
+    ContractionAnalysis an;
+    EIN_TRY(analyse_contraction(C, A, B, an));
+    if (an.c_size == 0) return EIN_SUCCESS;
+
+    T beta = c_prefactor;
+    if (an.c_repeats) {
+        // C addresses a diagonal: the reference scales/zeroes all of C first, then
+        // accumulates into the diagonal with prefactor 1.
+        EIN_TRY(tensor_scale<T>(C, c, c_prefactor, stream));
+        beta = T(1);
+    }
+
+    std::vector<Digit> vb, vm, vn, vk;
+    bool empty_k = false;
+    for (int u = 0; u < an.nlab; ++u) {
+        int64_t e = an.ext[u];
+        if (e == 0) {
+            if (!an.inC[u]) empty_k = true;
+            continue;
+        }
+        if (e == 1) continue;
+        if (an.inC[u] && an.inA[u] && an.inB[u])
+            vb.push_back({e, {an.sA[u], an.sB[u], an.sC[u]}});
+        else if (an.inC[u] && !an.inB[u])
+            vm.push_back({e, {an.sA[u], an.sC[u], 0}});
+        else if (an.inC[u] && an.inB[u])
+            vn.push_back({e, {an.sB[u], an.sC[u], 0}});
+        else
+            vk.push_back({e, {an.sA[u], an.sB[u], 0}});
+    }
+    if (empty_k || ab_prefactor == T(0)) {
+        // nothing to accumulate: C = beta*C over the target points
+        if (!an.c_repeats) EIN_TRY(tensor_scale<T>(C, c, c_prefactor, stream));
+        return EIN_SUCCESS;
+    }
+
+    int64_t const M = total_of(vm), N = total_of(vn), K = total_of(vk), NB = total_of(vb);
+    bool const use_mfma = M >= 16 && N >= 16 && K >= 16 && M < (int64_t{1} << 31) && N < (int64_t{1} << 31) &&
+                          K < (int64_t{1} << 31) && NB <= 65535;
+
+    if (use_mfma) {
+        auto gm = order_and_fold(vm, 2, 0, 1);
+        auto gn = order_and_fold(vn, 2, 0, 1);
+        bool const a_i_fast = !gm.empty() && std::llabs(gm.back().s[0]) == 1;
+        // K order serves the operand that cannot use its row/column direction
+        auto gk = a_i_fast ? order_and_fold(vk, 2, 1, 0) : order_and_fold(vk, 2, 0, 1);
+        bool const b_j_fast_n = !gn.empty() && std::llabs(gn.back().s[0]) == 1;
+        bool const b_p_unit   = !gk.empty() && std::llabs(gk.back().s[1]) == 1;
+        bool const b_j_fast   = b_j_fast_n || !b_p_unit;
+        bool const a_fast     = a_i_fast || !(!gk.empty() && std::llabs(gk.back().s[0]) == 1);
+        auto gbv              = order_and_fold(vb, 3, 2, 0);
+        GatherProblem<T> P{};
+        EIN_TRY(fill_group(P.gm, gm, 2));
+        EIN_TRY(fill_group(P.gn, gn, 2));
+        EIN_TRY(fill_group(P.gk, gk, 2));
+        EIN_TRY(fill_group(P.gb, gbv, 3));
+        P.m     = M;
+        P.n     = N;
+        P.k     = K;
+        P.batch = NB;
+        P.A     = a;
+        P.B     = b;
+        P.C     = c;
+        P.alpha = ab_prefactor;
+        P.beta  = beta;
+        GemmConfig cfg = choose_gemm_config(M, N, K, NB, int(sizeof(T)));
+        int bm = cfg.bm == 128 ? 128 : 64;
+        int bn = 64;
+        P.tiles_m = uint32_t(ceil_div(M, bm));
+        P.tiles_n = uint32_t(ceil_div(N, bn));
+        int64_t tiles = int64_t(P.tiles_m) * P.tiles_n * NB;
+        int64_t target = int64_t(num_cus()) * 3;
+        int splits = 1;
+        if (tiles < target && K >= 512) splits = int(std::min<int64_t>({ceil_div(target, tiles), K / 256, 64}));
+        P.k_chunk = ceil_div(ceil_div(K, splits), 16) * 16;
+        P.splits  = int(ceil_div(K, P.k_chunk));
+        P.partial = nullptr;
+        if (P.splits > 1) {
+            void *ws = nullptr;
+            EIN_TRY(workspace(stream, size_t(P.splits) * size_t(NB) * size_t(M) * size_t(N) * sizeof(T), &ws));
+            P.partial = static_cast<T *>(ws);
+        }
+        dim3 grid{P.tiles_m * P.tiles_n, uint32_t(P.splits), uint32_t(NB)};
+        if (bm == 128)
+            launch_gather<T, 128, 64>(P, a_fast, b_j_fast, grid, stream);
+        else
+            launch_gather<T, 64, 64>(P, a_fast, b_j_fast, grid, stream);
+        EIN_LAUNCH_CHECK();
+        if (P.splits > 1) {
+            int64_t total = M * N * NB;
+            int blocks    = int(std::min<int64_t>(ceil_div(total, 256), 65536));
+            gather_splitk_reduce_kernel<T><<<blocks, 256, 0, stream>>>(P);
+            EIN_LAUNCH_CHECK();
+        }
+        return EIN_SUCCESS;
+    }
+
+    // ---- reduction regimes: targets = batch + M + N digits, links = K digits
+    std::vector<Digit> vt;
+    for (auto const &d : vb) vt.push_back({d.e, {d.s[0], d.s[1], d.s[2]}});
+    for (auto const &d : vm) vt.push_back({d.e, {d.s[0], 0, d.s[1]}});
+    for (auto const &d : vn) vt.push_back({d.e, {0, d.s[0], d.s[1]}});
+    int64_t const NT = total_of(vt);
+    if (NT >= (int64_t{1} << 31) || K >= (int64_t{1} << 31))
+        return fail(EIN_ERROR_UNSUPPORTED, "contraction: index range above 2^31 in the reduction regime");
+    // K innermost digit: smallest of the A/B strides
+    for (auto &d : vk) d.s[2] = std::min(std::llabs(d.s[0]), std::llabs(d.s[1]));
+    auto gk = order_and_fold(vk, 2, 2, 0);
+    for (auto &d : gk) d.s[2] = 0;
+    bool const k_unit = !gk.empty() && (std::llabs(gk.back().s[0]) == 1 || std::llabs(gk.back().s[1]) == 1);
+    // target order: innermost = unit stride in the operand that is read per point
+    std::vector<Digit> vt2 = vt;
+    std::stable_sort(vt2.begin(), vt2.end(), [](Digit const &x, Digit const &y) {
+        auto key = [](Digit const &d) {
+            int64_t k = std::llabs(d.s[2]);
+            if (d.s[0] != 0) k = std::min<int64_t>(k, std::llabs(d.s[0]));
+            if (d.s[1] != 0) k = std::min<int64_t>(k, std::llabs(d.s[1]));
+            return k;
+        };
+        return key(x) > key(y);
+    });
+    std::vector<Digit> gt;
+    for (auto const &d : vt2) {
+        if (!gt.empty() && gt.back().s[0] == d.s[0] * d.e && gt.back().s[1] == d.s[1] * d.e &&
+            gt.back().s[2] == d.s[2] * d.e) {
+            gt.back().e *= d.e;
+            for (int o = 0; o < 3; ++o) gt.back().s[o] = d.s[o];
+        } else {
+            gt.push_back(d);
+        }
+    }
+    ReduceProblem<T> R{};
+    EIN_TRY(fill_group(R.gt, gt, 3));
+    EIN_TRY(fill_group(R.gk, gk, 2));
+    R.nt    = NT;
+    R.nk    = K;
+    R.A     = a;
+    R.B     = b;
+    R.C     = c;
+    R.alpha = ab_prefactor;
+    R.beta  = beta;
+    R.splits  = 1;
+    R.k_chunk = K;
+    R.partial = nullptr;
+    int const cus = num_cus();
+    bool const thread_mode = K <= 32 || (!k_unit && NT >= 4096);
+    if (thread_mode) {
+        uint32_t grid = uint32_t(std::max<int64_t>(1, std::min<int64_t>(ceil_div(NT, 256), int64_t(cus) * 16)));
+        reduce_thread_kernel<T><<<grid, 256, 0, stream>>>(R);
+        EIN_LAUNCH_CHECK();
+        return EIN_SUCCESS;
+    }
+    int64_t want_waves = int64_t(cus) * 16;
+    if (NT < want_waves && K >= 2048) {
+        int64_t s = std::min<int64_t>(ceil_div(want_waves, NT), ceil_div(K, 1024));
+        s         = std::max<int64_t>(1, std::min<int64_t>(s, 4096));
+        R.k_chunk = ceil_div(K, s);
+        R.splits  = int(ceil_div(K, R.k_chunk));
+    }
+    if (R.splits > 1) {
+        void *ws = nullptr;
+        EIN_TRY(workspace(stream, size_t(NT) * size_t(R.splits) * sizeof(T), &ws));
+        R.partial = static_cast<T *>(ws);
+    }
+    int64_t nwaves = NT * R.splits;
+    reduce_wave_kernel<T><<<uint32_t(ceil_div(nwaves, 4)), 256, 0, stream>>>(R);
+    EIN_LAUNCH_CHECK();
+    if (R.splits > 1) {
+        uint32_t grid = uint32_t(std::max<int64_t>(1, std::min<int64_t>(ceil_div(NT, 256), int64_t(cus) * 16)));
+        reduce_finish_targets_kernel<T><<<grid, 256, 0, stream>>>(R);
+        EIN_LAUNCH_CHECK();
+    }
+    return EIN_SUCCESS;
+}
+
+template int contract_generic<double>(double, ein_tensor_desc const &, double *, double, ein_tensor_desc const &,
+                                      double const *, ein_tensor_desc const &, double const *, hipStream_t);
+template int contract_generic<float>(float, ein_tensor_desc const &, float *, float, ein_tensor_desc const &, float const *,
+                                     ein_tensor_desc const &, float const *, hipStream_t);
+
+} // namespace ein
+
+extern "C" {
+
+int ein_contract_f64(double c_prefactor, const ein_tensor_desc *C, double *c, double ab_prefactor, const ein_tensor_desc *A,
+                     const double *a, const ein_tensor_desc *B, const double *b, ein_stream_t stream) {
+    if (C == nullptr) return -2;
+    if (A == nullptr) return -5;
+    if (B == nullptr) return -7;
+    return ein::contract_generic<double>(c_prefactor, *C, c, ab_prefactor, *A, a, *B, b, ein::as_stream(stream));
+}
+
+int ein_contract_f32(float c_prefactor, const ein_tensor_desc *C, float *c, float ab_prefactor, const ein_tensor_desc *A,
+                     const float *a, const ein_tensor_desc *B, const float *b, ein_stream_t stream) {
+    if (C == nullptr) return -2;
+    if (A == nullptr) return -5;
+    if (B == nullptr) return -7;
+    return ein::contract_generic<float>(c_prefactor, *C, c, ab_prefactor, *A, a, *B, b, ein::as_stream(stream));
+}
+
+} // extern "C"

@@ -1,0 +1,37 @@
+// Host-side interface of the MFMA GEMM family.
+#pragma once
+
+#include "gemm_core.hpp"
+
+namespace ein {
+
+template <typename T>
+struct GemmProblem {
+    int64_t m, n, k, batch;
+    T const *A;
+    int64_t a_sm, a_sk, a_sb;
+    T const *B;
+    int64_t b_sk, b_sn, b_sb;
+    T *C;
+    int64_t c_sm, c_sn, c_sb;
+    T alpha, beta;
+    // filled by the launcher
+    uint32_t tiles_m, tiles_n;
+    int32_t splits;
+    int64_t k_chunk;
+    T *partial;
+};
+
+struct GemmConfig {
+    int bm     = 128;
+    int bn     = 128;
+    int splits = 1;
+};
+
+GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, int elem_bytes);
+
+// Strided (optionally batched) GEMM; `force` overrides the tile heuristic (autotuner).
+template <typename T>
+int gemm_strided(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream);
+
+} // namespace ein

@@ -1,0 +1,111 @@
+// MFMA GEMM building blocks for gfx950 shared by the dense, strided-batched,
+// gather (generic-contraction) and grouped kernels.
+//
+// Tile scheme: a 256-thread workgroup (4 wave64s as 2x2) owns a BM x BN tile of C and
+// walks K in BK-deep slabs staged through double-buffered LDS; each wave owns a
+// (BM/2) x (BN/2) sub-tile made of 16x16 MFMA blocks (v_mfma_f64_16x16x4f64 or
+// v_mfma_f32_16x16x4f32).
+//
+// MFMA operand placement (16x16x4, wave64): lane l supplies A[r=l%16][k=l/16] and
+// B[k=l/16][c=l%16] and receives D[r=4*(l/16)+v][c=l%16], v=0..3. The kernels feed
+// the B side of the contraction as the MFMA "A" operand and the A side as the MFMA
+// "B" operand, so lane%16 runs along C's row index i -- C's unit-stride direction
+// after host-side normalisation -- and every C store is a 128-byte contiguous run.
+#pragma once
+
+#include "ein_common.hpp"
+
+namespace ein {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+    using acc_t = v4d;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mfma<float> {
+    using acc_t = v4f;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int GEMM_THREADS = 256;
+
+template <typename T, int BM, int BN, int BK>
+struct TileCfg {
+    static constexpr int WM  = BM / 2;
+    static constexpr int WN  = BN / 2;
+    static constexpr int FM  = WM / 16;
+    static constexpr int FN  = WN / 16;
+    static constexpr int PAD = 1;
+    static constexpr int LDA = BM + PAD; // LDS row length of the A slab (indexed [k][i])
+    static constexpr int LDB = BN + PAD; // LDS row length of the B slab (indexed [k][j])
+    static constexpr int LA  = BM * BK / GEMM_THREADS;
+    static constexpr int LB  = BN * BK / GEMM_THREADS;
+    static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must split into 2x2 waves of 16x16 blocks");
+    static_assert((BM * BK) % GEMM_THREADS == 0 && (BN * BK) % GEMM_THREADS == 0, "slab must split over threads");
+    static_assert(BK % 4 == 0, "BK is a multiple of the MFMA depth");
+};
+
+// Accumulator block of one wave.
+template <typename T, int FM, int FN>
+struct WaveAcc {
+    typename Mfma<T>::acc_t c[FM][FN];
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+            for (int b = 0; b < FN; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) c[a][b][v] = T(0);
+    }
+};
+
+// One BK-deep slab of MFMAs out of LDS. As is [BK][LDA] (k-major, i fastest),
+// Bs is [BK][LDB] (k-major, j fastest).
+template <typename T, int BM, int BN, int BK>
+__device__ __forceinline__ void mfma_slab(T const *__restrict__ As, T const *__restrict__ Bs,
+                                          WaveAcc<T, TileCfg<T, BM, BN, BK>::FM, TileCfg<T, BM, BN, BK>::FN> &acc,
+                                          int wm, int wn, int lane) {
+    using Cfg  = TileCfg<T, BM, BN, BK>;
+    int const r = lane & 15;
+    int const q = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+        T af[Cfg::FM], bf[Cfg::FN];
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a) af[a] = As[(kk + q) * Cfg::LDA + wm * Cfg::WM + a * 16 + r];
+#pragma unroll
+        for (int b = 0; b < Cfg::FN; ++b) bf[b] = Bs[(kk + q) * Cfg::LDB + wn * Cfg::WN + b * 16 + r];
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+            for (int b = 0; b < Cfg::FN; ++b) acc.c[a][b] = Mfma<T>::mma(bf[b], af[a], acc.c[a][b]);
+    }
+}
+
+// Grouped (L2-friendly) ordering of output tiles: walks GROUP rows of tiles
+// column by column so neighbouring workgroups share A rows and B columns.
+__device__ __forceinline__ void tile_coords(uint32_t logical, uint32_t tiles_m, uint32_t tiles_n, uint32_t &tm,
+                                            uint32_t &tn) {
+    constexpr uint32_t GROUP = 8;
+    uint32_t per_group = GROUP * tiles_n;
+    uint32_t g         = logical / per_group;
+    uint32_t first_m   = g * GROUP;
+    uint32_t gsize     = tiles_m - first_m < GROUP ? tiles_m - first_m : GROUP;
+    uint32_t in_g      = logical - g * per_group;
+    tm                 = first_m + in_g % gsize;
+    tn                 = in_g / gsize;
+}
+
+} // namespace ein

@@ -1,0 +1,282 @@
+// Out-of-place tensor permutation C = beta*C + alpha*perm(A) for gfx950.
+//
+// Axis i of C is axis perm[i] of A (the convention of the reference permute_kernel,
+// TensorAlgebra/Backends/DevicePermute.hpp:34-73, which writes B[.. idx[perm[i]] ..]).
+// After dropping unit axes and merging axes that are contiguous in both tensors:
+//   * if C's unit-stride axis is also A's unit-stride axis, an element-wise pass in C
+//     order moves contiguous runs (both sides coalesced, 16-byte vectors when aligned);
+//   * otherwise a 64x64 tile is staged through LDS (one padded row per tile row) so the
+//     A read runs along A's unit-stride axis and the C write along C's;
+// alpha == 1 and beta == 0 is a pure copy, so the result is bit-exact.
+#include <algorithm>
+#include <vector>
+
+#include "blas1.hpp"
+
+namespace ein {
+
+namespace {
+
+constexpr int PMAX = EIN_MAX_RANK;
+
+struct PermAxes {
+    int32_t n;
+    uint32_t ext[PMAX];
+    FastDiv fd[PMAX];
+    int64_t ext64[PMAX];
+    int64_t sa[PMAX];
+    int64_t sc[PMAX];
+    int64_t total;
+};
+
+template <bool WIDE>
+__device__ __forceinline__ void perm_decode(int64_t e, PermAxes const &P, int n, int64_t &oa, int64_t &oc) {
+    oa = 0;
+    oc = 0;
+    if constexpr (WIDE) {
+        for (int d = n - 1; d >= 0; --d) {
+            int64_t q = e / P.ext64[d];
+            int64_t r = e - q * P.ext64[d];
+            oa += r * P.sa[d];
+            oc += r * P.sc[d];
+            e = q;
+        }
+    } else {
+        uint32_t u = uint32_t(e);
+        for (int d = n - 1; d >= 0; --d) {
+            uint32_t q = fast_div(u, P.fd[d]);
+            uint32_t r = u - q * P.ext[d];
+            oa += int64_t(r) * P.sa[d];
+            oc += int64_t(r) * P.sc[d];
+            u = q;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T perm_value(T a, T const *cptr, T alpha, T beta, bool copy) {
+    if (copy) return a;
+    T v = alpha * a;
+    if (beta != T(0)) v += beta * *cptr;
+    return v;
+}
+
+// Element-wise pass in C order; the last axis is C's fastest. VEC consecutive
+// elements of the last axis per thread (requires unit strides on that axis in both).
+template <typename T, int VEC, bool WIDE>
+__global__ __launch_bounds__(256) void permute_elem_kernel(PermAxes P, T alpha, T const *__restrict__ A, T beta,
+                                                           T *__restrict__ C, int copy) {
+    int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+    int64_t const nvec   = P.total / VEC;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < nvec; e += stride) {
+        int64_t oa, oc;
+        perm_decode<WIDE>(e * VEC, P, P.n, oa, oc);
+        if constexpr (VEC == 2) {
+            using V  = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+            V a      = *reinterpret_cast<V const *>(A + oa);
+            V *cp    = reinterpret_cast<V *>(C + oc);
+            if (copy) {
+                *cp = a;
+            } else {
+                V c = beta != T(0) ? *cp : V{};
+                V r;
+                r.x = alpha * a.x + (beta != T(0) ? beta * c.x : T(0));
+                r.y = alpha * a.y + (beta != T(0) ? beta * c.y : T(0));
+                *cp = r;
+            }
+        } else {
+            C[oc] = perm_value<T>(A[oa], C + oc, alpha, beta, copy != 0);
+        }
+    }
+}
+
+// 64x64 LDS tile transpose. X = C's unit-stride axis (A stride ax), Y = A's unit-stride
+// axis (C stride cy). Remaining axes (P, n entries) are decoded from the batch index.
+template <typename T, bool WIDE>
+__global__ __launch_bounds__(256) void permute_tile_kernel(PermAxes P, int64_t ex, int64_t ax, int64_t ey, int64_t cy,
+                                                           int64_t tiles_x, T alpha, T const *__restrict__ A, T beta,
+                                                           T *__restrict__ C, int copy) {
+    __shared__ T tile[64][65];
+    int64_t const ntiles = tiles_x * ((ey + 63) / 64);
+    int const t          = threadIdx.x;
+    for (int64_t b = blockIdx.y; b < P.total; b += gridDim.y) {
+        int64_t oa, oc;
+        perm_decode<WIDE>(b, P, P.n, oa, oc);
+        for (int64_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
+            int64_t const x0 = (tt % tiles_x) * 64;
+            int64_t const y0 = (tt / tiles_x) * 64;
+            {
+                int const ly = t & 63;
+                int const lx = t >> 6;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    int x = lx + 4 * r;
+                    if (x0 + x < ex && y0 + ly < ey) tile[x][ly] = A[oa + (x0 + x) * ax + (y0 + ly)];
+                }
+            }
+            __syncthreads();
+            {
+                int const lx = t & 63;
+                int const ly = t >> 6;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    int y = ly + 4 * r;
+                    if (x0 + lx < ex && y0 + y < ey) {
+                        T *cp = C + oc + (y0 + y) * cy + (x0 + lx);
+                        *cp   = perm_value<T>(tile[lx][y], cp, alpha, beta, copy != 0);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ext64 holds the true extents; ext/fd are only used by the 32-bit decode, which is
+// selected only when every extent and the index range fit below 2^31.
+void fill_divs(PermAxes &P) {
+    for (int d = 0; d < P.n; ++d) {
+        P.ext[d] = uint32_t(P.ext64[d] < (int64_t{1} << 31) ? P.ext64[d] : 1);
+        P.fd[d]  = make_fastdiv(P.ext[d] == 0 ? 1u : P.ext[d]);
+    }
+}
+
+template <typename T>
+int permute_impl(int32_t rank, int64_t const *dims_A, int32_t const *perm, T alpha, T const *A,
+                 int64_t const *strides_A, T beta, T *C, int64_t const *strides_C, hipStream_t stream) {
+    if (rank < 0 || rank > PMAX) return fail(-1, "permute: rank out of range");
+    if (rank > 0 && (dims_A == nullptr || perm == nullptr || strides_A == nullptr || strides_C == nullptr))
+        return fail(-2, "permute: null shape array");
+    bool seen[PMAX] = {false};
+    for (int i = 0; i < rank; ++i) {
+        if (perm[i] < 0 || perm[i] >= rank || seen[perm[i]]) return fail(-3, "permute: perm is not a permutation");
+        seen[perm[i]] = true;
+        if (dims_A[i] < 0) return fail(-2, "permute: negative extent");
+    }
+    // axes in C order
+    struct Ax {
+        int64_t e, sa, sc;
+    };
+    std::vector<Ax> ax;
+    int64_t total = 1;
+    for (int i = 0; i < rank; ++i) {
+        int64_t e = dims_A[perm[i]];
+        total *= e;
+        if (e == 1) continue;
+        ax.push_back({e, strides_A[perm[i]], strides_C[i]});
+    }
+    if (total == 0) return EIN_SUCCESS;
+    // merge C-adjacent axes contiguous in both tensors
+    std::vector<Ax> m;
+    for (auto const &a : ax) {
+        if (!m.empty() && m.back().sa == a.sa * a.e && m.back().sc == a.sc * a.e) {
+            m.back().e *= a.e;
+            m.back().sa = a.sa;
+            m.back().sc = a.sc;
+        } else {
+            m.push_back(a);
+        }
+    }
+    if (m.empty()) m.push_back({1, 1, 1});
+    if (int(m.size()) > PMAX) return fail(EIN_ERROR_UNSUPPORTED, "permute: too many axes");
+    int const copy = (alpha == T(1) && beta == T(0)) ? 1 : 0;
+
+    // C's fastest axis and A's fastest axis
+    int ic = 0, ia = 0;
+    for (int i = 0; i < int(m.size()); ++i) {
+        if (std::llabs(m[i].sc) < std::llabs(m[ic].sc)) ic = i;
+        if (std::llabs(m[i].sa) < std::llabs(m[ia].sa)) ia = i;
+    }
+    int const nblk_cap = num_cus() * 8;
+
+    if (ic != ia && m[ic].sc == 1 && m[ia].sa == 1 && m[ic].e >= 16 && m[ia].e >= 16) {
+        PermAxes P{};
+        P.n     = 0;
+        P.total = 1;
+        for (int i = 0; i < int(m.size()); ++i) {
+            if (i == ic || i == ia) continue;
+            P.ext64[P.n] = m[i].e;
+            P.sa[P.n]    = m[i].sa;
+            P.sc[P.n]    = m[i].sc;
+            P.total *= m[i].e;
+            ++P.n;
+        }
+        fill_divs(P);
+        int64_t ex = m[ic].e, axs = m[ic].sa, ey = m[ia].e, cy = m[ia].sc;
+        int64_t tiles_x = ceil_div(ex, 64), tiles = tiles_x * ceil_div(ey, 64);
+        int64_t gy      = std::min<int64_t>(P.total, 65535);
+        int64_t gx      = std::min<int64_t>(tiles, std::max<int64_t>(1, nblk_cap / gy + 1));
+        bool const wide = P.total >= (int64_t{1} << 31);
+        if (wide)
+            permute_tile_kernel<T, true><<<dim3(uint32_t(gx), uint32_t(gy)), 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x,
+                                                                                               alpha, A, beta, C, copy);
+        else
+            permute_tile_kernel<T, false><<<dim3(uint32_t(gx), uint32_t(gy)), 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x,
+                                                                                                alpha, A, beta, C, copy);
+        EIN_LAUNCH_CHECK();
+        return EIN_SUCCESS;
+    }
+
+    // element-wise in C order with C's fastest axis last
+    PermAxes P{};
+    P.n     = 0;
+    P.total = total;
+    for (int i = 0; i < int(m.size()); ++i) {
+        if (i == ic) continue;
+        P.ext64[P.n] = m[i].e;
+        P.sa[P.n]    = m[i].sa;
+        P.sc[P.n]    = m[i].sc;
+        ++P.n;
+    }
+    P.ext64[P.n] = m[ic].e;
+    P.sa[P.n]    = m[ic].sa;
+    P.sc[P.n]    = m[ic].sc;
+    ++P.n;
+    fill_divs(P);
+    bool const wide = total >= (int64_t{1} << 31);
+    bool const vec2 = !wide && P.sa[P.n - 1] == 1 && P.sc[P.n - 1] == 1 && (m[ic].e % 2) == 0 &&
+                      (reinterpret_cast<uintptr_t>(A) % (2 * sizeof(T)) == 0) &&
+                      (reinterpret_cast<uintptr_t>(C) % (2 * sizeof(T)) == 0) && [&] {
+                          for (int d = 0; d < P.n - 1; ++d)
+                              if ((P.sa[d] % 2) != 0 || (P.sc[d] % 2) != 0) return false;
+                          return true;
+                      }();
+    int64_t work   = vec2 ? total / 2 : total;
+    uint32_t grid  = uint32_t(std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, 256), int64_t(nblk_cap) * 4)));
+    if (vec2)
+        permute_elem_kernel<T, 2, false><<<grid, 256, 0, stream>>>(P, alpha, A, beta, C, copy);
+    else if (wide)
+        permute_elem_kernel<T, 1, true><<<grid, 256, 0, stream>>>(P, alpha, A, beta, C, copy);
+    else
+        permute_elem_kernel<T, 1, false><<<grid, 256, 0, stream>>>(P, alpha, A, beta, C, copy);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
+} // namespace
+
+template <typename T>
+int permute_tensor(int32_t rank, int64_t const *dims_A, int32_t const *perm, T alpha, T const *A,
+                   int64_t const *strides_A, T beta, T *C, int64_t const *strides_C, hipStream_t stream) {
+    return permute_impl<T>(rank, dims_A, perm, alpha, A, strides_A, beta, C, strides_C, stream);
+}
+template int permute_tensor<double>(int32_t, int64_t const *, int32_t const *, double, double const *, int64_t const *,
+                                    double, double *, int64_t const *, hipStream_t);
+template int permute_tensor<float>(int32_t, int64_t const *, int32_t const *, float, float const *, int64_t const *, float,
+                                   float *, int64_t const *, hipStream_t);
+
+} // namespace ein
+
+extern "C" {
+
+int ein_permute_f64(int32_t rank, const int64_t *dims_A, const int32_t *perm, double alpha, const double *A,
+                    const int64_t *strides_A, double beta, double *C, const int64_t *strides_C, ein_stream_t stream) {
+    return ein::permute_impl<double>(rank, dims_A, perm, alpha, A, strides_A, beta, C, strides_C, ein::as_stream(stream));
+}
+
+int ein_permute_f32(int32_t rank, const int64_t *dims_A, const int32_t *perm, float alpha, const float *A,
+                    const int64_t *strides_A, float beta, float *C, const int64_t *strides_C, ein_stream_t stream) {
+    return ein::permute_impl<float>(rank, dims_A, perm, alpha, A, strides_A, beta, C, strides_C, ein::as_stream(stream));
+}
+
+} // extern "C"

@@ -1,0 +1,167 @@
+/*
+ * einsums_amd.h -- C ABI of the MI355X (gfx950) einsum backend.
+ *
+ * Every entry point is plain C: device pointers, 64-bit sizes, an explicit stream
+ * (a hipStream_t passed as void*, NULL = the null stream) and an int status
+ * (0 = success, negative = -(index of the bad argument) as in BLAS `info`,
+ * positive = an EIN_ERROR_* code; the message is in ein_last_error_string()).
+ * All leaves are asynchronous on `stream`; nothing synchronises unless stated.
+ *
+ * Layout conventions:
+ *   - the BLAS-shaped leaves (gemm/gemv/ger/dot/dirprod/scale/axpby) follow
+ *     column-major BLAS semantics exactly like the reference's blas::gpu:: layer;
+ *   - tensor descriptors (ein_tensor_desc) carry per-axis extents and element
+ *     strides, so any row-major / column-major / strided view can be described.
+ *
+ * Reference interfaces replaced (paths relative to the Einsums source tree):
+ *   libs/Einsums/hipBLAS/include/Einsums/hipBLAS.hpp            (blas::gpu::*)
+ *   libs/Einsums/TensorAlgebra/include/Einsums/TensorAlgebra/Backends/GPUTensorAlgebra.hpp
+ *   libs/Einsums/TensorAlgebra/include/Einsums/TensorAlgebra/Backends/DevicePermute.hpp
+ *   libs/EinsumsPy/TensorAlgebra/src/PyTensorAlgebra.cpp        (compile_plan)
+ */
+#ifndef EINSUMS_AMD_H
+#define EINSUMS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *ein_stream_t;
+
+#define EIN_SUCCESS 0
+#define EIN_ERROR_HIP 1001          /* a HIP runtime call failed */
+#define EIN_ERROR_ALLOC 1002        /* device allocation failed */
+#define EIN_ERROR_DIMENSION 1003    /* extents of a shared label disagree (reference: dimension_error) */
+#define EIN_ERROR_RANK 1004         /* rank / label count mismatch (reference: rank_error) */
+#define EIN_ERROR_UNSUPPORTED 1005  /* a case this backend does not implement */
+#define EIN_ERROR_WORKSPACE 1006    /* workspace too small while a stream is being captured */
+#define EIN_ERROR_VALUE 1007        /* bad value (reference: std::invalid_argument / std::domain_error) */
+
+#define EIN_MAX_RANK 16
+
+/* Algorithm choice, numbered like detail::AlgorithmChoice
+ * (libs/Einsums/TensorAlgebra/include/Einsums/TensorAlgebra/Detail/Utilities.hpp:26-34). */
+#define EIN_ALG_GENERIC 0
+#define EIN_ALG_DOT 1
+#define EIN_ALG_DIRECT 2
+#define EIN_ALG_GER 3
+#define EIN_ALG_GEMV 4
+#define EIN_ALG_GEMM 5
+#define EIN_ALG_INDETERMINATE 6
+
+/* A strided tensor view. labels[i] is any small integer (e.g. a character code)
+ * naming the index on axis i; equal labels on one tensor form a diagonal. */
+typedef struct ein_tensor_desc {
+    int32_t rank;
+    int32_t labels[EIN_MAX_RANK];
+    int64_t dims[EIN_MAX_RANK];
+    int64_t strides[EIN_MAX_RANK]; /* in elements */
+} ein_tensor_desc;
+
+/* ---------------------------------------------------------------- runtime */
+const char *ein_version(void);
+const char *ein_last_error_string(void);
+int ein_init(void);
+/* Stream-ordered pooled device memory (256-byte aligned). */
+int ein_malloc(void **ptr, size_t bytes);
+int ein_free(void *ptr);
+/* Pre-size the per-stream scratch used by split-K partials, reductions and
+ * split-precision slices; required before hipGraph capture of large calls. */
+int ein_workspace_reserve(ein_stream_t stream, size_t bytes);
+size_t ein_workspace_size(ein_stream_t stream);
+int ein_stream_synchronize(ein_stream_t stream);
+/* hipGraph capture of a sequence of calls on one stream. */
+int ein_graph_begin(ein_stream_t stream);
+int ein_graph_end(ein_stream_t stream, void **graph_exec);
+int ein_graph_launch(void *graph_exec, ein_stream_t stream);
+int ein_graph_destroy(void *graph_exec);
+
+/* ------------------------------------------------- BLAS-shaped leaves
+ * Replace blas::gpu::gemm/gemv/dot/ger/scal/axpby/dirprod
+ * (libs/Einsums/hipBLAS/include/Einsums/hipBLAS.hpp:64,133,546,622,668,720,1541). */
+int ein_gemm_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                 const double *B, int64_t ldb, double beta, double *C, int64_t ldc, ein_stream_t stream);
+int ein_gemm_f32(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const float *A, int64_t lda,
+                 const float *B, int64_t ldb, float beta, float *C, int64_t ldc, ein_stream_t stream);
+/* Strided-batched GEMM (batch index shared by A, B and C). Used by batched einsum
+ * patterns; the reference loops over blocks (Dispatch.hpp:1264-1286). */
+int ein_gemm_strided_batched_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+                                 int64_t lda, int64_t strideA, const double *B, int64_t ldb, int64_t strideB, double beta,
+                                 double *C, int64_t ldc, int64_t strideC, int64_t batch, ein_stream_t stream);
+int ein_gemm_strided_batched_f32(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                                 int64_t lda, int64_t strideA, const float *B, int64_t ldb, int64_t strideB, float beta,
+                                 float *C, int64_t ldc, int64_t strideC, int64_t batch, ein_stream_t stream);
+int ein_gemv_f64(char trans, int64_t m, int64_t n, double alpha, const double *A, int64_t lda, const double *x, int64_t incx,
+                 double beta, double *y, int64_t incy, ein_stream_t stream);
+int ein_gemv_f32(char trans, int64_t m, int64_t n, float alpha, const float *A, int64_t lda, const float *x, int64_t incx,
+                 float beta, float *y, int64_t incy, ein_stream_t stream);
+/* result (device pointer) = beta*result + alpha*dot(x,y); deterministic two-pass tree. */
+int ein_dot_f64(int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy, double beta,
+                double *result, ein_stream_t stream);
+int ein_dot_f32(int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float beta, float *result,
+                ein_stream_t stream);
+int ein_ger_f64(int64_t m, int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy, double *A,
+                int64_t lda, ein_stream_t stream);
+int ein_ger_f32(int64_t m, int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float *A,
+                int64_t lda, ein_stream_t stream);
+/* z = alpha*x.*y + beta*z (the reference's dirprod is the beta=1 case). */
+int ein_dirprod_f64(int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy, double beta,
+                    double *z, int64_t incz, ein_stream_t stream);
+int ein_dirprod_f32(int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float beta, float *z,
+                    int64_t incz, ein_stream_t stream);
+int ein_scale_f64(int64_t n, double alpha, double *x, int64_t incx, ein_stream_t stream);
+int ein_scale_f32(int64_t n, float alpha, float *x, int64_t incx, ein_stream_t stream);
+int ein_axpby_f64(int64_t n, double alpha, const double *x, int64_t incx, double beta, double *y, int64_t incy,
+                  ein_stream_t stream);
+int ein_axpby_f32(int64_t n, float alpha, const float *x, int64_t incx, float beta, float *y, int64_t incy,
+                  ein_stream_t stream);
+
+/* ------------------------------------------------------------- tensors */
+/* Whole-view ops on a strided tensor: C = alpha*C (alpha==0 writes zeros), C = value. */
+int ein_tensor_scale_f64(const ein_tensor_desc *C, double *c, double alpha, ein_stream_t stream);
+int ein_tensor_scale_f32(const ein_tensor_desc *C, float *c, float alpha, ein_stream_t stream);
+
+/* C = beta*C + alpha*permute(A): C axis i is A axis perm[i].  Replaces
+ * detail::gpu_permute / permute_kernel (DevicePermute.hpp:25-73) and the LibreTT path
+ * (TensorAlgebra/src/DevicePermute.hip:20-59). alpha==1, beta==0 is a bit-exact copy. */
+int ein_permute_f64(int32_t rank, const int64_t *dims_A, const int32_t *perm, double alpha, const double *A,
+                    const int64_t *strides_A, double beta, double *C, const int64_t *strides_C, ein_stream_t stream);
+int ein_permute_f32(int32_t rank, const int64_t *dims_A, const int32_t *perm, float alpha, const float *A,
+                    const int64_t *strides_A, float beta, float *C, const int64_t *strides_C, ein_stream_t stream);
+
+/* Generic contraction C = c_prefactor*C + ab_prefactor * sum_links A*B over any label
+ * pattern (Hadamard diagonals, traces, broadcasts, rank-0 C). Replaces the device
+ * einsum_generic_algorithm launcher (GPUTensorAlgebra.hpp:200-342) with atomic-free
+ * kernels: MFMA strided-gather GEMM, gather GEMV, or a deterministic reduction. */
+int ein_contract_f64(double c_prefactor, const ein_tensor_desc *C, double *c, double ab_prefactor, const ein_tensor_desc *A,
+                     const double *a, const ein_tensor_desc *B, const double *b, ein_stream_t stream);
+int ein_contract_f32(float c_prefactor, const ein_tensor_desc *C, float *c, float ab_prefactor, const ein_tensor_desc *A,
+                     const float *a, const ein_tensor_desc *B, const float *b, ein_stream_t stream);
+
+/* Runtime pattern analysis with the precedence of compile_plan
+ * (libs/EinsumsPy/TensorAlgebra/src/PyTensorAlgebra.cpp:169-300); writes EIN_ALG_*. */
+int ein_einsum_classify(const ein_tensor_desc *C, const ein_tensor_desc *A, const ein_tensor_desc *B, int32_t *algorithm);
+/* Full einsum: classify, fold to 2-D views, run the matching kernel, fall back to the
+ * generic contraction when a view does not fold (Dispatch.hpp:859-957 semantics).
+ * *algorithm_used (may be NULL) receives the EIN_ALG_* that actually ran. */
+int ein_einsum_f64(double c_prefactor, const ein_tensor_desc *C, double *c, double ab_prefactor, const ein_tensor_desc *A,
+                   const double *a, const ein_tensor_desc *B, const double *b, int32_t *algorithm_used, ein_stream_t stream);
+int ein_einsum_f32(float c_prefactor, const ein_tensor_desc *C, float *c, float ab_prefactor, const ein_tensor_desc *A,
+                   const float *a, const ein_tensor_desc *B, const float *b, int32_t *algorithm_used, ein_stream_t stream);
+/* Execute with a pattern already chosen by the caller (the C++ headers decide it at
+ * compile time, Dispatch.hpp:894-955); views that do not fold run the generic kernel
+ * and *algorithm_used reports EIN_ALG_GENERIC. EIN_ALG_INDETERMINATE classifies first. */
+int ein_einsum_execute_f64(int32_t algorithm, double c_prefactor, const ein_tensor_desc *C, double *c, double ab_prefactor,
+                           const ein_tensor_desc *A, const double *a, const ein_tensor_desc *B, const double *b,
+                           int32_t *algorithm_used, ein_stream_t stream);
+int ein_einsum_execute_f32(int32_t algorithm, float c_prefactor, const ein_tensor_desc *C, float *c, float ab_prefactor,
+                           const ein_tensor_desc *A, const float *a, const ein_tensor_desc *B, const float *b,
+                           int32_t *algorithm_used, ein_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EINSUMS_AMD_H */

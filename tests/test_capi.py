@@ -1,0 +1,110 @@
+"""CPU checks of the C ABI: the library loads, exports every declared entry point, and
+its host-side analysis (pattern classification, argument and dimension checks) agrees
+with the oracle -- none of these calls touch a GPU."""
+import ctypes
+import json
+import re
+import zlib
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "einsums_amd.h"
+DISPATCH = json.loads((ROOT / "tests" / "golden" / "dispatch_table.json").read_text())
+
+
+def declared_symbols():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(ein_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from einsums_amd import _lib
+
+    syms = declared_symbols()
+    assert len(syms) > 30
+    missing = [s for s in syms if not hasattr(_lib.lib, s)]
+    assert missing == []
+    assert set(_lib.SYMBOLS) == set(syms)
+
+
+def test_version_string():
+    from einsums_amd import _lib
+
+    assert "gfx950" in _lib.version()
+
+
+@pytest.mark.parametrize("case", DISPATCH["cases"], ids=lambda c: f"{c['C']}<-{c['A']},{c['B']}")
+def test_compile_plan_types(core, case):
+    plan = core.compile_plan(case["C"], case["A"], case["B"])
+    expected = {"GENERIC": core.EinsumGenericPlan, "DOT": core.EinsumDotPlan, "DIRECT": core.EinsumDirectProductPlan,
+                "GER": core.EinsumGerPlan, "GEMV": core.EinsumGemvPlan, "GEMM": core.EinsumGemmPlan}[case["alg"]]
+    assert type(plan) is expected
+
+
+def _random_pattern(rng):
+    letters = "abcdefgh"
+    n = int(rng.integers(1, 6))
+    labs = letters[:n]
+
+    def pick(k):
+        repl = bool(rng.integers(0, 4) == 0) or k > n
+        return "".join(rng.choice(list(labs), size=k, replace=repl))
+
+    return pick(int(rng.integers(0, 4))), pick(int(rng.integers(1, 5))), pick(int(rng.integers(1, 5)))
+
+
+def test_classification_agrees_with_oracle(core, oracle):
+    rng = np.random.default_rng(zlib.crc32(b"classify"))
+    seen = set()
+    for _ in range(3000):
+        C, A, B = _random_pattern(rng)
+        if (C, A, B) in seen:
+            continue
+        seen.add((C, A, B))
+        got = core.algorithm_name(core.classify(C, A, B))
+        assert got == oracle.classify(C, A, B), (C, A, B)
+
+
+def test_dimension_mismatch_is_reported_without_gpu():
+    from einsums_amd import _lib
+
+    C = _lib.make_desc("ij", [3, 4], [4, 1])
+    A = _lib.make_desc("ik", [3, 5], [5, 1])
+    B = _lib.make_desc("kj", [6, 4], [4, 1])  # k: 5 vs 6
+    st = _lib.lib.ein_einsum_f64(0.0, ctypes.byref(C), None, 1.0, ctypes.byref(A), None, ctypes.byref(B), None, None,
+                                 None)
+    assert st == _lib.EIN_ERROR_DIMENSION
+    with pytest.raises(_lib.DimensionError):
+        _lib.check(st)
+
+
+def test_gemm_argument_errors_without_gpu():
+    from einsums_amd import _lib
+
+    f = _lib.lib.ein_gemm_f64
+    assert f(b"X", b"N", 2, 2, 2, 1.0, None, 2, None, 2, 0.0, None, 2, None) == -1
+    assert f(b"N", b"X", 2, 2, 2, 1.0, None, 2, None, 2, 0.0, None, 2, None) == -2
+    assert f(b"N", b"N", -1, 2, 2, 1.0, None, 2, None, 2, 0.0, None, 2, None) == -3
+    assert f(b"N", b"N", 4, 2, 2, 1.0, None, 2, None, 2, 0.0, None, 4, None) == -8
+    assert f(b"N", b"N", 2, 2, 4, 1.0, None, 2, None, 2, 0.0, None, 2, None) == -10
+    assert f(b"N", b"N", 4, 2, 2, 1.0, None, 4, None, 2, 0.0, None, 2, None) == -13
+    # quick return: nothing to do for an empty problem
+    assert f(b"N", b"N", 0, 0, 0, 1.0, None, 1, None, 1, 0.0, None, 1, None) == 0
+
+
+def test_permute_rejects_non_permutation_without_gpu():
+    from einsums_amd import _lib
+
+    I64, I32 = ctypes.c_int64 * 2, ctypes.c_int32 * 2
+    st = _lib.lib.ein_permute_f64(2, I64(2, 2), I32(0, 0), 1.0, None, I64(2, 1), 0.0, None, I64(2, 1), None)
+    assert st == -3
+
+
+def test_rank_mismatch_in_python_layer(core):
+    import torch
+
+    with pytest.raises(core.RankError):
+        core._desc(torch.zeros(2, 2), "ijk")
