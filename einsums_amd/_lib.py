@@ -25,6 +25,7 @@ EIN_ERROR_VALUE = 1007
 
 # detail::AlgorithmChoice numbering (TensorAlgebra/Detail/Utilities.hpp:26-34)
 GENERIC, DOT, DIRECT, GER, GEMV, GEMM, INDETERMINATE = range(7)
+SLICE_BF16, SLICE_FP16 = 0, 1
 ALGORITHM_NAMES = {GENERIC: "GENERIC", DOT: "DOT", DIRECT: "DIRECT", GER: "GER", GEMV: "GEMV", GEMM: "GEMM",
                    INDETERMINATE: "INDETERMINATE"}
 
@@ -123,6 +124,10 @@ def _load():
                                              ctypes.POINTER(ctypes.c_int32), _vp])
         sig[f"ein_einsum_execute_{sfx}"] = (_c_int, [ctypes.c_int32, T, _desc_p, _dp, T, _desc_p, _dp, _desc_p, _dp,
                                                      ctypes.POINTER(ctypes.c_int32), _vp])
+    sig["ein_gemm_split_f64"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64, _dp,
+                                          _i64, ctypes.c_double, _dp, _i64, ctypes.c_int32, ctypes.c_int32, _vp])
+    sig["ein_einsum_split_f64"] = (_c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _desc_p, _dp, ctypes.c_double,
+                                            _desc_p, _dp, _desc_p, _dp, ctypes.POINTER(ctypes.c_int32), _vp])
     sig["ein_einsum_classify"] = (_c_int, [_desc_p, _desc_p, _desc_p, ctypes.POINTER(ctypes.c_int32)])
     for name, (res, args) in sig.items():
         _proto(lib, name, res, args)

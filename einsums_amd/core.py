@@ -138,12 +138,15 @@ class EinsumGenericPlan:
     def __repr__(self) -> str:
         return f"{type(self).__name__}({self.C_indices!r} <- {self.A_indices!r} * {self.B_indices!r})"
 
-    def execute(self, C_prefactor, C, AB_prefactor, A, B, stream: torch.cuda.Stream | None = None) -> None:
+    def execute(self, C_prefactor, C, AB_prefactor, A, B, stream: torch.cuda.Stream | None = None,
+                split: int = 0, slice_type: str = "bf16") -> None:
+        """Run the plan. ``split=s`` (3..6) runs an FP64 GEMM-mapped contraction with the
+        split-precision (Ozaki) GEMM using s BF16 or FP16 slices per operand."""
         if isinstance(C, (list, tuple)):
             if not (isinstance(A, (list, tuple)) and isinstance(B, (list, tuple))) or not (len(C) == len(A) == len(B)):
                 raise ValueError("The lengths of the collections need to be the same!")
             for c, a, b in zip(C, A, B):
-                self.execute(C_prefactor, c, AB_prefactor, a, b, stream)
+                self.execute(C_prefactor, c, AB_prefactor, a, b, stream, split, slice_type)
             return
         dC, finC = _as_device(C, write=True)
         dA, _ = _as_device(A, write=False)
@@ -153,9 +156,17 @@ class EinsumGenericPlan:
             raise TypeError(f"execute: operands must share float64 or float32, got {dC.dtype}, {dA.dtype}, {dB.dtype}")
         cd, ad, bd = _desc(dC, self.C_indices), _desc(dA, self.A_indices), _desc(dB, self.B_indices)
         used = ctypes.c_int32(-1)
-        fn = _fn("ein_einsum_execute", dtype)
-        status = fn(self.algorithm, float(C_prefactor), ctypes.byref(cd), _ptr(dC), float(AB_prefactor),
-                    ctypes.byref(ad), _ptr(dA), ctypes.byref(bd), _ptr(dB), ctypes.byref(used), stream_handle(stream))
+        if split:
+            if dtype != torch.float64:
+                raise TypeError("split precision applies to float64 operands")
+            st = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16}[slice_type]
+            status = lib.ein_einsum_split_f64(int(split), st, float(C_prefactor), ctypes.byref(cd), _ptr(dC),
+                                              float(AB_prefactor), ctypes.byref(ad), _ptr(dA), ctypes.byref(bd), _ptr(dB),
+                                              ctypes.byref(used), stream_handle(stream))
+        else:
+            fn = _fn("ein_einsum_execute", dtype)
+            status = fn(self.algorithm, float(C_prefactor), ctypes.byref(cd), _ptr(dC), float(AB_prefactor),
+                        ctypes.byref(ad), _ptr(dA), ctypes.byref(bd), _ptr(dB), ctypes.byref(used), stream_handle(stream))
         check(status, f"einsum {self.C_indices} <- {self.A_indices},{self.B_indices}")
         self.last_algorithm = used.value
         if finC is not None:
@@ -194,10 +205,10 @@ def compile_plan(C_indices: str, A_indices: str, B_indices: str) -> EinsumGeneri
 
 
 def einsum(C_indices: str, C, A_indices: str, A, B_indices: str, B, C_prefactor: float = 0.0,
-           AB_prefactor: float = 1.0, stream=None) -> int:
+           AB_prefactor: float = 1.0, stream=None, split: int = 0, slice_type: str = "bf16") -> int:
     """C = C_prefactor*C + AB_prefactor * contraction; returns the algorithm that ran."""
     plan = compile_plan(C_indices, A_indices, B_indices)
-    plan.execute(C_prefactor, C, AB_prefactor, A, B, stream)
+    plan.execute(C_prefactor, C, AB_prefactor, A, B, stream, split, slice_type)
     return plan.last_algorithm
 
 
@@ -249,6 +260,22 @@ def gemm(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch.Tenso
     status = _fn("ein_gemm", C.dtype)(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0),
                                       _ptr(A), A.stride(0), float(beta), _ptr(C), C.stride(0), stream_handle(stream))
     check(status, "gemm")
+
+
+def gemm_split(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch.Tensor, beta: float, C: torch.Tensor,
+               nslices: int = 5, slice_type: str = "bf16", stream=None) -> None:
+    """Row-major C = alpha*op(A) op(B) + beta*C in FP64 through the split-precision GEMM."""
+    for t in (A, B, C):
+        if t.dim() != 2 or t.stride(1) != 1 or t.dtype != torch.float64:
+            raise ValueError("gemm_split: 2-D row-major float64 tensors with unit column stride required")
+    ta, tb = transA.upper() != "N", transB.upper() != "N"
+    m = A.shape[1] if ta else A.shape[0]
+    k = A.shape[0] if ta else A.shape[1]
+    n = B.shape[0] if tb else B.shape[1]
+    st = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16}[slice_type]
+    status = lib.ein_gemm_split_f64(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0), _ptr(A),
+                                    A.stride(0), float(beta), _ptr(C), C.stride(0), int(nslices), st, stream_handle(stream))
+    check(status, "gemm_split")
 
 
 def algorithm_name(alg: int) -> str:

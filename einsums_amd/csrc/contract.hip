@@ -276,7 +276,7 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProb
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
-                    int64_t j = j0 + wn * Cfg::WN + b * 16 + q * 4 + v;
+                    int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
                     if (i < P.m && j < P.n) part[i + j * P.m] = acc.c[a][b][v];
                 }
         return;
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProb
     for (int b = 0; b < Cfg::FN; ++b)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            int64_t j = j0 + wn * Cfg::WN + b * 16 + q * 4 + v;
+            int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
             if (j >= P.n) continue;
             int64_t o[2];
             group_decode<2>(uint32_t(j), P.gn, o);

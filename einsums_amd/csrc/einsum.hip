@@ -15,10 +15,23 @@
 
 #include "contract.hpp"
 #include "einsum.hpp"
+#include "ozaki.hpp"
 
 namespace ein {
 
 namespace {
+
+// Split-precision setting for FP64 GEMM-mapped einsums on this thread (0 = plain FP64).
+thread_local int t_split_slices = 0;
+thread_local int t_split_type   = EIN_SLICE_BF16;
+
+template <typename T>
+int run_gemm(GemmProblem<T> const &P, hipStream_t stream) {
+    if constexpr (std::is_same_v<T, double>) {
+        if (t_split_slices > 0) return gemm_split(P, t_split_slices, t_split_type, stream);
+    }
+    return gemm_strided<T>(P, nullptr, stream);
+}
 
 using Labels = std::vector<int>;
 using PosList = std::vector<std::pair<int, int>>; // (label, position)
@@ -322,7 +335,7 @@ int einsum_execute(int alg, T cp, ein_tensor_desc const &C, T *c, T abp, ein_ten
         P.c_sn  = c_sn;
         P.alpha = abp;
         P.beta  = cp;
-        return gemm_strided<T>(P, nullptr, stream);
+        return run_gemm<T>(P, stream);
     }
     default:
         break;
@@ -371,6 +384,19 @@ int ein_einsum_execute_f64(int32_t algorithm, double c_prefactor, const ein_tens
     if (B == nullptr) return -8;
     return ein::einsum_execute<double>(algorithm, c_prefactor, *C, c, ab_prefactor, *A, a, *B, b, algorithm_used,
                                        ein::as_stream(stream));
+}
+
+int ein_einsum_split_f64(int32_t nslices, int32_t slice_type, double c_prefactor, const ein_tensor_desc *C, double *c,
+                         double ab_prefactor, const ein_tensor_desc *A, const double *a, const ein_tensor_desc *B,
+                         const double *b, int32_t *algorithm_used, ein_stream_t stream) {
+    if (nslices < 3 || nslices > 6) return ein::fail(-1, "einsum_split: nslices must be 3..6");
+    if (slice_type != EIN_SLICE_BF16 && slice_type != EIN_SLICE_FP16) return ein::fail(-2, "einsum_split: bad slice type");
+    ein::t_split_slices = nslices;
+    ein::t_split_type   = slice_type;
+    int st = ein_einsum_execute_f64(EIN_ALG_INDETERMINATE, c_prefactor, C, c, ab_prefactor, A, a, B, b, algorithm_used,
+                                    stream);
+    ein::t_split_slices = 0;
+    return st;
 }
 
 int ein_einsum_execute_f32(int32_t algorithm, float c_prefactor, const ein_tensor_desc *C, float *c, float ab_prefactor,

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
-                    int64_t j = j0 + wn * Cfg::WN + b * 16 + q * 4 + v;
+                    int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
                     if (i < P.m && j < P.n) part[i + j * P.m] = acc.c[a][b][v];
                 }
         return;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
-                int64_t j = j0 + wn * Cfg::WN + b * 16 + q * 4 + v;
+                int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
                 if (i < P.m && j < P.n) {
                     T *cp = Cb + i * P.c_sm + j * P.c_sn;
                     T val = P.alpha * acc.c[a][b][v];

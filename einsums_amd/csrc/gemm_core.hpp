@@ -7,7 +7,9 @@
 // v_mfma_f32_16x16x4f32).
 //
 // MFMA operand placement (16x16x4, wave64): lane l supplies A[r=l%16][k=l/16] and
-// B[k=l/16][c=l%16] and receives D[r=4*(l/16)+v][c=l%16], v=0..3. The kernels feed
+// B[k=l/16][c=l%16] and receives four results D[r][c=l%16], v=0..3, with
+// r = 4*(l/16)+v for the FP32 instruction and r = 4*v+(l/16) for the FP64 one
+// (Mfma<T>::row gives the mapping; both were checked against known answers). The kernels feed
 // the B side of the contraction as the MFMA "A" operand and the A side as the MFMA
 // "B" operand, so lane%16 runs along C's row index i -- C's unit-stride direction
 // after host-side normalisation -- and every C store is a 128-byte contiguous run.
@@ -29,6 +31,8 @@ struct Mfma<double> {
     static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    // row of the 16x16 result held in accumulator element v by lane group q = lane/16
+    static __host__ __device__ constexpr int row(int q, int v) { return 4 * v + q; }
 };
 
 template <>
@@ -37,6 +41,7 @@ struct Mfma<float> {
     static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __host__ __device__ constexpr int row(int q, int v) { return 4 * q + v; }
 };
 
 constexpr int GEMM_THREADS = 256;

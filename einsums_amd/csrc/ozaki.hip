@@ -1,0 +1,436 @@
+// Split-precision FP64 GEMM for gfx950 (Ozaki scheme on BF16/FP16 MFMA).
+//
+// Each row i of op(A) is scaled by 2^-e_i (|A(i,:)| < 2^e_i) and cut into NS integer
+// slices of BETA = 8 bits:  A(i,p) = 2^e_i * sum_t a_t(i,p) 2^(-8(t+1)),  |a_0| <= 256,
+// |a_t| <= 128 for t > 0 -- integers that BF16 and FP16 hold exactly. B is cut per
+// column j the same way (exponent f_j). Then
+//     (A B)(i,j) = 2^(e_i+f_j) sum_{t,u} 2^(-8(t+u+2)) sum_p a_t(i,p) b_u(p,j)
+// and the pairs with t+u >= NS are dropped (their weight is below 2^(-8(NS+1))).
+// Every inner sum is an exact integer while it stays below 2^24, so the MFMA FP32
+// accumulators for one diagonal d = t+u (at most d+1 pairs, each product <= 2^16 for d=0,
+// 2^15 / 2^14 otherwise) are exact over chunks of KC = 128 contraction steps:
+// (d+3) * 2^14 * 128 < 2^24 for d <= 4, and 2^16 * 128 < 2^24 for d = 0. After every chunk
+// the diagonals are converted to FP64, scaled by 2^(-8(d+2)) and added to FP64
+// accumulators; the epilogue applies 2^(e_i+f_j), alpha and beta in FP64.
+// With NS = 5 the representation error is 2^-40 relative to the row/column maximum.
+#include <algorithm>
+#include <cmath>
+
+#include "gemm.hpp"
+#include "ozaki.hpp"
+
+namespace ein {
+
+namespace {
+
+constexpr int OZ_BETA = 8;
+constexpr int OZ_KC   = 128; // exact-accumulation chunk (multiple of 32)
+constexpr int OZ_LDS_ROW = 40; // 32 used + 8 padding elements per LDS row (80 bytes)
+
+template <typename H>
+struct HalfMfma;
+template <>
+struct HalfMfma<__bf16> {
+    typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ v4f mma(v8 a, v8 b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct HalfMfma<_Float16> {
+    typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ v4f mma(v8 a, v8 b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// ----------------------------------------------------------- row exponents
+// e[r] = exponent with max_p |X(r,p)| < 2^e (0 for an all-zero row).
+__global__ __launch_bounds__(256) void row_exp_rows_fast(SliceSrc S, int32_t *__restrict__ e) {
+    int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= S.rows) return;
+    double mx = 0.0;
+    for (int64_t p = 0; p < S.k; ++p) mx = fmax(mx, fabs(S.X[r * S.s_row + p * S.s_k]));
+    int ex = 0;
+    if (mx > 0.0) (void)frexp(mx, &ex);
+    e[r] = ex;
+}
+
+__global__ __launch_bounds__(256) void row_exp_k_fast(SliceSrc S, int32_t *__restrict__ e) {
+    int const lane = threadIdx.x & 63;
+    int64_t r      = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (r >= S.rows) return;
+    double mx = 0.0;
+    for (int64_t p = lane; p < S.k; p += 64) mx = fmax(mx, fabs(S.X[r * S.s_row + p * S.s_k]));
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if (lane == 0) {
+        int ex = 0;
+        if (mx > 0.0) (void)frexp(mx, &ex);
+        e[r] = ex;
+    }
+}
+
+// ----------------------------------------------------------------- slicing
+// 64 x 64 tiles of X staged through LDS so that the read follows X's unit-stride axis
+// and the slice writes run along p (slice layout: out[t][r][kp], p fastest).
+template <typename H, bool ROWS_FAST>
+__global__ __launch_bounds__(256) void slice_kernel(SliceSrc S, int32_t const *__restrict__ e, int ns,
+                                                    H *__restrict__ out) {
+    __shared__ double tile[64][65];
+    int64_t const r0 = int64_t(blockIdx.y) * 64;
+    int64_t const p0 = int64_t(blockIdx.x) * 64;
+    int const t      = threadIdx.x;
+    if constexpr (ROWS_FAST) {
+        int const lr = t & 63;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            int lp     = (t >> 6) + 4 * q;
+            int64_t r  = r0 + lr, p = p0 + lp;
+            tile[lr][lp] = (r < S.rows && p < S.k) ? S.X[r * S.s_row + p * S.s_k] : 0.0;
+        }
+    } else {
+        int const lp = t & 63;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            int lr     = (t >> 6) + 4 * q;
+            int64_t r  = r0 + lr, p = p0 + lp;
+            tile[lr][lp] = (r < S.rows && p < S.k) ? S.X[r * S.s_row + p * S.s_k] : 0.0;
+        }
+    }
+    __syncthreads();
+    int const lp = t & 63;
+    int64_t const p = p0 + lp;
+    if (p >= S.kp) return;
+#pragma unroll 1
+    for (int q = 0; q < 16; ++q) {
+        int lr    = (t >> 6) + 4 * q;
+        int64_t r = r0 + lr;
+        if (r >= S.rows) continue;
+        double x = ldexp(tile[lr][lp], -e[r]);
+        for (int s = 0; s < ns; ++s) {
+            double y = ldexp(x, OZ_BETA);
+            double d = rint(y);
+            x        = y - d;
+            out[(int64_t(s) * S.rows + r) * S.kp + p] = static_cast<H>(static_cast<float>(d));
+        }
+    }
+}
+
+// ------------------------------------------------------------- slice GEMM
+// C(i,j) = alpha * 2^(e_i+f_j) * sum_d 2^(-8(d+2)) D_d(i,j) + beta * C(i,j);
+// workgroup tile BM x BN, waves of 32 x 32 (2 x 2 MFMA blocks of 16 x 16 x 32).
+template <typename H, int BM, int BN, int NS>
+__global__ __launch_bounds__((BM / 32) * (BN / 32) * 64) void ozaki_gemm_kernel(OzProblem P) {
+    using M8                 = typename HalfMfma<H>::v8;
+    constexpr int WAVES_M    = BM / 32;
+    constexpr int NT         = WAVES_M * (BN / 32) * 64;
+    constexpr int A_CHUNKS   = NS * BM * 4; // 16-byte chunks per stage
+    constexpr int B_CHUNKS   = NS * BN * 4;
+    constexpr int A_PER_T    = (A_CHUNKS + NT - 1) / NT;
+    constexpr int B_PER_T    = (B_CHUNKS + NT - 1) / NT;
+    constexpr int A_STAGE    = NS * BM * OZ_LDS_ROW;
+    constexpr int B_STAGE    = NS * BN * OZ_LDS_ROW;
+    __shared__ __attribute__((aligned(16))) H lds[2 * (A_STAGE + B_STAGE)];
+
+    int const tid  = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const wm   = wave % WAVES_M;
+    int const wn   = wave / WAVES_M;
+    int const r    = lane & 15;
+    int const q    = lane >> 4;
+
+    uint32_t const ntiles  = P.tiles_m * P.tiles_n;
+    uint32_t const logical = xcd_remap(blockIdx.x, ntiles);
+    uint32_t tm, tn;
+    tile_coords(logical, P.tiles_m, P.tiles_n, tm, tn);
+    int64_t const i0 = int64_t(tm) * BM;
+    int64_t const j0 = int64_t(tn) * BN;
+
+    H const *__restrict__ Ag = static_cast<H const *>(P.As_raw);
+    H const *__restrict__ Bg = static_cast<H const *>(P.Bs_raw);
+    uint4 ra[A_PER_T], rb[B_PER_T];
+    auto load_stage = [&](int64_t k0) {
+#pragma unroll
+        for (int c = 0; c < A_PER_T; ++c) {
+            int ch = tid + c * NT;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (ch < A_CHUNKS) {
+                int s = ch / (BM * 4), rr = (ch / 4) % BM, kc = ch % 4;
+                int64_t i = i0 + rr;
+                if (i < P.m) v = *reinterpret_cast<uint4 const *>(Ag + (int64_t(s) * P.m + i) * P.kp + k0 + kc * 8);
+            }
+            ra[c] = v;
+        }
+#pragma unroll
+        for (int c = 0; c < B_PER_T; ++c) {
+            int ch = tid + c * NT;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (ch < B_CHUNKS) {
+                int s = ch / (BN * 4), rr = (ch / 4) % BN, kc = ch % 4;
+                int64_t j = j0 + rr;
+                if (j < P.n) v = *reinterpret_cast<uint4 const *>(Bg + (int64_t(s) * P.n + j) * P.kp + k0 + kc * 8);
+            }
+            rb[c] = v;
+        }
+    };
+    auto store_stage = [&](int buf) {
+        H *As = lds + buf * (A_STAGE + B_STAGE);
+        H *Bs = As + A_STAGE;
+#pragma unroll
+        for (int c = 0; c < A_PER_T; ++c) {
+            int ch = tid + c * NT;
+            if (ch < A_CHUNKS) {
+                int s = ch / (BM * 4), rr = (ch / 4) % BM, kc = ch % 4;
+                *reinterpret_cast<uint4 *>(As + (s * BM + rr) * OZ_LDS_ROW + kc * 8) = ra[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < B_PER_T; ++c) {
+            int ch = tid + c * NT;
+            if (ch < B_CHUNKS) {
+                int s = ch / (BN * 4), rr = (ch / 4) % BN, kc = ch % 4;
+                *reinterpret_cast<uint4 *>(Bs + (s * BN + rr) * OZ_LDS_ROW + kc * 8) = rb[c];
+            }
+        }
+    };
+
+    v4f accf[NS][2][2];
+    v4d accd[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) accd[a][b][v] = 0.0;
+#pragma unroll
+            for (int d = 0; d < NS; ++d)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) accf[d][a][b][v] = 0.f;
+        }
+
+    auto flush = [&]() {
+#pragma unroll
+        for (int d = 0; d < NS; ++d) {
+            double const sc = ldexp(1.0, -OZ_BETA * (d + 2));
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        accd[a][b][v] += double(accf[d][a][b][v]) * sc;
+                        accf[d][a][b][v] = 0.f;
+                    }
+        }
+    };
+
+    int64_t const nstages = P.kp / 32;
+    if (nstages > 0) {
+        load_stage(0);
+        store_stage(0);
+        __syncthreads();
+    }
+    for (int64_t st = 0; st < nstages; ++st) {
+        int const buf = int(st & 1);
+        if (st + 1 < nstages) load_stage((st + 1) * 32);
+        H const *As = lds + buf * (A_STAGE + B_STAGE);
+        H const *Bs = As + A_STAGE;
+        M8 af[NS][2], bfr[NS][2];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                af[s][a]  = *reinterpret_cast<M8 const *>(As + (s * BM + wm * 32 + a * 16 + r) * OZ_LDS_ROW + q * 8);
+                bfr[s][a] = *reinterpret_cast<M8 const *>(Bs + (s * BN + wn * 32 + a * 16 + r) * OZ_LDS_ROW + q * 8);
+            }
+#pragma unroll
+        for (int t = 0; t < NS; ++t)
+#pragma unroll
+            for (int u = 0; u < NS - t; ++u)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) accf[t + u][a][b] = HalfMfma<H>::mma(bfr[u][b], af[t][a], accf[t + u][a][b]);
+        if (((st + 1) * 32) % OZ_KC == 0 || st + 1 == nstages) flush();
+        if (st + 1 < nstages) store_stage(buf ^ 1);
+        __syncthreads();
+    }
+
+    // epilogue: FP32-MFMA result layout D[4*(lane/16)+v][lane%16]; rows of D run along j
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        int64_t i = i0 + wm * 32 + a * 16 + r;
+        if (i >= P.m) continue;
+        int const ei = P.ea[i];
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int64_t j = j0 + wn * 32 + b * 16 + q * 4 + v;
+                if (j >= P.n) continue;
+                double *cp = P.C + i * P.c_sm + j * P.c_sn;
+                double val = P.alpha * ldexp(accd[a][b][v], ei + P.eb[j]);
+                if (P.beta != 0.0) val += P.beta * *cp;
+                *cp = val;
+            }
+    }
+}
+
+template <typename H, int NS>
+void launch_oz(OzProblem const &P, int bm, hipStream_t stream) {
+    if constexpr (NS <= 5) {
+        if (bm == 128) {
+            dim3 grid{P.tiles_m * P.tiles_n};
+            ozaki_gemm_kernel<H, 128, 64, NS><<<grid, 512, 0, stream>>>(P);
+            return;
+        }
+    }
+    {
+        dim3 grid{P.tiles_m * P.tiles_n};
+        ozaki_gemm_kernel<H, 64, 64, NS><<<grid, 256, 0, stream>>>(P);
+    }
+}
+
+template <typename H>
+int slice_operand(SliceSrc const &S, int ns, int32_t *e, H *out, hipStream_t stream) {
+    bool const rows_fast = S.s_row == 1 || (S.s_k != 1 && std::llabs(S.s_row) < std::llabs(S.s_k));
+    if (rows_fast) {
+        row_exp_rows_fast<<<uint32_t(ceil_div(S.rows, 256)), 256, 0, stream>>>(S, e);
+    } else {
+        row_exp_k_fast<<<uint32_t(ceil_div(S.rows, 4)), 256, 0, stream>>>(S, e);
+    }
+    EIN_LAUNCH_CHECK();
+    dim3 grid{uint32_t(ceil_div(S.kp, 64)), uint32_t(ceil_div(S.rows, 64))};
+    if (grid.y > 65535) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: more than 4M rows");
+    if (rows_fast)
+        slice_kernel<H, true><<<grid, 256, 0, stream>>>(S, e, ns, out);
+    else
+        slice_kernel<H, false><<<grid, 256, 0, stream>>>(S, e, ns, out);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
+template <typename H>
+int gemm_split_typed(GemmProblem<double> P, int ns, int bm, hipStream_t stream) {
+    int64_t const kp = ceil_div(P.k, 32) * 32;
+    size_t const a_bytes = size_t(ns) * size_t(P.m) * size_t(kp) * sizeof(H);
+    size_t const b_bytes = size_t(ns) * size_t(P.n) * size_t(kp) * sizeof(H);
+    size_t const a_off = 0;
+    size_t const b_off = (a_bytes + 255) / 256 * 256;
+    size_t const ea_off = b_off + (b_bytes + 255) / 256 * 256;
+    size_t const eb_off = ea_off + (size_t(P.m) * 4 + 255) / 256 * 256;
+    size_t const total = eb_off + (size_t(P.n) * 4 + 255) / 256 * 256;
+    void *ws = nullptr;
+    EIN_TRY(workspace(stream, total, &ws));
+    char *base = static_cast<char *>(ws);
+    H *As = reinterpret_cast<H *>(base + a_off);
+    H *Bs = reinterpret_cast<H *>(base + b_off);
+    int32_t *ea = reinterpret_cast<int32_t *>(base + ea_off);
+    int32_t *eb = reinterpret_cast<int32_t *>(base + eb_off);
+
+    SliceSrc SA{P.m, P.k, kp, P.A, P.a_sm, P.a_sk};
+    SliceSrc SB{P.n, P.k, kp, P.B, P.b_sn, P.b_sk};
+    EIN_TRY(slice_operand<H>(SA, ns, ea, As, stream));
+    EIN_TRY(slice_operand<H>(SB, ns, eb, Bs, stream));
+
+    OzProblem Q{};
+    Q.m = P.m;
+    Q.n = P.n;
+    Q.kp = kp;
+    Q.As_raw = As;
+    Q.Bs_raw = Bs;
+    Q.ea = ea;
+    Q.eb = eb;
+    Q.C = P.C;
+    Q.c_sm = P.c_sm;
+    Q.c_sn = P.c_sn;
+    Q.alpha = P.alpha;
+    Q.beta = P.beta;
+    Q.tiles_m = uint32_t(ceil_div(P.m, bm));
+    Q.tiles_n = uint32_t(ceil_div(P.n, 64));
+    switch (ns) {
+    case 3: launch_oz<H, 3>(Q, bm, stream); break;
+    case 4: launch_oz<H, 4>(Q, bm, stream); break;
+    case 5: launch_oz<H, 5>(Q, bm, stream); break;
+    case 6:
+        Q.tiles_m = uint32_t(ceil_div(P.m, 64));
+        launch_oz<H, 6>(Q, 64, stream);
+        break;
+    default: return fail(EIN_ERROR_VALUE, "split gemm: nslices must be 3..6");
+    }
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
+} // namespace
+
+int gemm_split(GemmProblem<double> P, int nslices, int slice_type, hipStream_t stream) {
+    if (P.m <= 0 || P.n <= 0) return EIN_SUCCESS;
+    if (nslices < 3 || nslices > 6) return fail(EIN_ERROR_VALUE, "split gemm: nslices must be 3..6");
+    if (P.batch != 1) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: batched problems are not supported");
+    // C's unit-stride direction must be i (lane-contiguous stores): transpose the problem otherwise
+    bool const c_row_fast = (P.c_sn == 1 && P.c_sm != 1) || (P.c_sm != 1 && P.c_sn != 1 && P.c_sn < P.c_sm);
+    if (c_row_fast) {
+        GemmProblem<double> Q = P;
+        Q.m = P.n;
+        Q.n = P.m;
+        Q.A = P.B;
+        Q.a_sm = P.b_sn;
+        Q.a_sk = P.b_sk;
+        Q.B = P.A;
+        Q.b_sk = P.a_sk;
+        Q.b_sn = P.a_sm;
+        Q.c_sm = P.c_sn;
+        Q.c_sn = P.c_sm;
+        P = Q;
+    }
+    if (P.k <= 0 || P.alpha == 0.0) {
+        GemmConfig cfg;
+        return gemm_strided<double>(P, &cfg, stream); // scales C by beta
+    }
+    int64_t const t128 = ceil_div(P.m, 128) * ceil_div(P.n, 64);
+    int const bm = t128 >= int64_t(num_cus()) * 2 ? 128 : 64;
+    if (slice_type == EIN_SLICE_BF16) return gemm_split_typed<__bf16>(P, nslices, bm, stream);
+    if (slice_type == EIN_SLICE_FP16) return gemm_split_typed<_Float16>(P, nslices, bm, stream);
+    return fail(EIN_ERROR_VALUE, "split gemm: slice_type must be EIN_SLICE_BF16 or EIN_SLICE_FP16");
+}
+
+} // namespace ein
+
+extern "C" {
+
+int ein_gemm_split_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                       const double *B, int64_t ldb, double beta, double *C, int64_t ldc, int32_t nslices,
+                       int32_t slice_type, ein_stream_t stream) {
+    auto notrans = [](char t) { return t == 'N' || t == 'n'; };
+    auto valid = [&](char t) { return notrans(t) || t == 'T' || t == 't' || t == 'C' || t == 'c'; };
+    if (!valid(transa)) return ein::fail(-1, "gemm_split: invalid transa");
+    if (!valid(transb)) return ein::fail(-2, "gemm_split: invalid transb");
+    if (m < 0) return ein::fail(-3, "gemm_split: m < 0");
+    if (n < 0) return ein::fail(-4, "gemm_split: n < 0");
+    if (k < 0) return ein::fail(-5, "gemm_split: k < 0");
+    if (lda < std::max<int64_t>(1, notrans(transa) ? m : k)) return ein::fail(-8, "gemm_split: lda too small");
+    if (ldb < std::max<int64_t>(1, notrans(transb) ? k : n)) return ein::fail(-10, "gemm_split: ldb too small");
+    if (ldc < std::max<int64_t>(1, m)) return ein::fail(-13, "gemm_split: ldc too small");
+    ein::GemmProblem<double> P{};
+    P.m = m;
+    P.n = n;
+    P.k = k;
+    P.batch = 1;
+    P.A = A;
+    P.B = B;
+    P.C = C;
+    P.alpha = alpha;
+    P.beta = beta;
+    P.a_sm = notrans(transa) ? 1 : lda;
+    P.a_sk = notrans(transa) ? lda : 1;
+    P.b_sk = notrans(transb) ? 1 : ldb;
+    P.b_sn = notrans(transb) ? ldb : 1;
+    P.c_sm = 1;
+    P.c_sn = ldc;
+    return ein::gemm_split(P, nslices, slice_type, ein::as_stream(stream));
+}
+
+} // extern "C"

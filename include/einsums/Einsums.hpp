@@ -1,0 +1,6 @@
+// Umbrella header of the MI355X einsum backend's C++ API.
+#pragma once
+
+#include "DeviceTensor.hpp"
+#include "Index.hpp"
+#include "TensorAlgebra.hpp"

@@ -1,0 +1,252 @@
+// einsum / permute on device tensors with compile-time pattern analysis.
+//
+// Reference API kept: einsum(C_prefactor, Indices{...}, &C, AB_prefactor, Indices{...}, A,
+// Indices{...}, B, &algorithm_choice) and the default-prefactor form
+// (libs/Einsums/TensorAlgebra/include/Einsums/TensorAlgebra/TensorAlgebra.hpp:48-146),
+// permute(C_prefactor, Indices{...}, &C, A_prefactor, Indices{...}, A) (Permute.hpp:330-407),
+// and detail::AlgorithmChoice (Detail/Utilities.hpp:26-34).
+//
+// The label sets are analysed at compile time: every label tag carries a constexpr id and
+// the predicates of Dispatch.hpp:194-262,407-439,565-609 (precedence :894-955) are
+// evaluated as constexpr functions over fixed-capacity id lists. The chosen algorithm is a
+// constant of each instantiation; at run time the call only builds the strided views and
+// hands them to the backend (ein_einsum_execute_*), which folds the index groups and falls
+// back to the generic kernel when a view does not fold.
+#pragma once
+
+#include <tuple>
+#include <type_traits>
+
+#include "DeviceTensor.hpp"
+#include "Index.hpp"
+
+namespace einsums::tensor_algebra {
+
+namespace detail {
+
+enum AlgorithmChoice { GENERIC, DOT, DIRECT, GER, GEMV, GEMM, INDETERMINATE };
+
+inline char const *algorithm_name(AlgorithmChoice a) {
+    constexpr char const *names[] = {"GENERIC", "DOT", "DIRECT", "GER", "GEMV", "GEMM", "INDETERMINATE"};
+    return names[int(a)];
+}
+
+// Fixed-capacity list of label ids (or (label, position) pairs) usable in constexpr code.
+struct LList {
+    int32_t v[48] = {};
+    int32_t p[48] = {};
+    int n         = 0;
+    constexpr void push(int32_t x, int32_t pos = -1) {
+        v[n] = x;
+        p[n] = pos;
+        ++n;
+    }
+    constexpr bool has(int32_t x) const {
+        for (int i = 0; i < n; ++i)
+            if (v[i] == x) return true;
+        return false;
+    }
+    constexpr bool same_labels(LList const &o) const {
+        if (n != o.n) return false;
+        for (int i = 0; i < n; ++i)
+            if (v[i] != o.v[i]) return false;
+        return true;
+    }
+};
+
+template <typename... Ls>
+constexpr LList make_list() {
+    LList l;
+    (l.push(std::remove_cvref_t<Ls>::id), ...);
+    return l;
+}
+
+// IntersectT: elements of a that are also in b (order of a).
+constexpr LList intersect(LList const &a, LList const &b) {
+    LList o;
+    for (int i = 0; i < a.n; ++i)
+        if (b.has(a.v[i])) o.push(a.v[i]);
+    return o;
+}
+// DifferenceT: elements of a not in b.
+constexpr LList difference(LList const &a, LList const &b) {
+    LList o;
+    for (int i = 0; i < a.n; ++i)
+        if (!b.has(a.v[i])) o.push(a.v[i]);
+    return o;
+}
+// UniqueT: first occurrences.
+constexpr LList unique(LList const &a) {
+    LList o;
+    for (int i = 0; i < a.n; ++i)
+        if (!o.has(a.v[i])) o.push(a.v[i]);
+    return o;
+}
+// find_type_with_position: for each label of a, every position where it occurs in b.
+constexpr LList find_with_pos(LList const &a, LList const &b) {
+    LList o;
+    for (int i = 0; i < a.n; ++i)
+        for (int j = 0; j < b.n; ++j)
+            if (a.v[i] == b.v[j]) o.push(a.v[i], j);
+    return o;
+}
+// contiguous_positions
+constexpr bool contiguous(LList const &x) {
+    for (int i = 0; i + 1 < x.n; ++i)
+        if (x.p[i] != x.p[i + 1] - 1) return false;
+    return true;
+}
+// is_same_ordering (false for empty lists, as in the reference)
+constexpr bool same_ordering(LList const &x, LList const &y) {
+    if (x.n == 0 || y.n == 0 || x.n != y.n) return false;
+    for (int i = 0; i < x.n; ++i)
+        if (x.v[i] != y.v[i]) return false;
+    return true;
+}
+
+constexpr bool is_matrix_vector(LList const &C, LList const &X, LList const &Y) {
+    LList Cu = unique(C);
+    LList links = unique(difference(intersect(X, Y), C));
+    LList lX = find_with_pos(links, X), lY = find_with_pos(links, Y);
+    LList tX = find_with_pos(Cu, X), tY = find_with_pos(Cu, Y);
+    LList X_in_C = find_with_pos(X, C), Y_in_C = find_with_pos(Y, C);
+    return contiguous(lX) && contiguous(lY) && contiguous(tX) && same_ordering(lX, lY) && same_ordering(tX, X_in_C) &&
+           !same_ordering(tY, Y_in_C) && Y_in_C.n == 0;
+}
+
+// detail::einsum precedence for real basic tensors (Dispatch.hpp:894-955).
+constexpr AlgorithmChoice classify(LList const &C, LList const &A, LList const &B) {
+    if (unique(C).n != C.n || unique(A).n != A.n || unique(B).n != B.n) return GENERIC; // Hadamard
+    if (C.n == 0 && A.same_labels(B)) return DOT;
+    if (C.same_labels(A) && C.same_labels(B)) return DIRECT;
+    LList linksAB = intersect(A, B);
+    LList Cu = unique(C);
+    LList tA = find_with_pos(Cu, A), tB = find_with_pos(Cu, B);
+    if (linksAB.n == 0 && contiguous(tA) && contiguous(tB)) return GER;
+    if (is_matrix_vector(C, A, B) || is_matrix_vector(C, B, A)) return GEMV;
+    if (C.n >= 2 && A.n >= 2 && B.n >= 2) {
+        LList links = unique(difference(linksAB, C));
+        LList lA = find_with_pos(links, A), lB = find_with_pos(links, B);
+        LList A_in_C = find_with_pos(A, C), B_in_C = find_with_pos(B, C);
+        bool cond = difference(C, A).n > 0 && difference(C, B).n > 0 && contiguous(lA) && contiguous(lB) &&
+                    contiguous(tA) && contiguous(tB) && contiguous(A_in_C) && contiguous(B_in_C) &&
+                    same_ordering(lA, lB) && same_ordering(tA, A_in_C) && same_ordering(tB, B_in_C);
+        if (cond) return GEMM;
+    }
+    return GENERIC;
+}
+
+template <typename... Cs, typename... As, typename... Bs>
+constexpr AlgorithmChoice classify(std::tuple<Cs...> const &, std::tuple<As...> const &, std::tuple<Bs...> const &) {
+    return classify(make_list<Cs...>(), make_list<As...>(), make_list<Bs...>());
+}
+
+template <typename... Ls>
+constexpr std::array<int32_t, sizeof...(Ls) == 0 ? 1 : sizeof...(Ls)> label_ids(std::tuple<Ls...> const &) {
+    if constexpr (sizeof...(Ls) == 0)
+        return {0};
+    else
+        return {std::remove_cvref_t<Ls>::id...};
+}
+
+template <typename T>
+concept DeviceTensorConcept = requires(T const &t) {
+    typename T::ValueType;
+    { T::Rank } -> std::convertible_to<size_t>;
+    t.data();
+    t.dims();
+    t.strides();
+};
+
+} // namespace detail
+
+template <bool ConjA = false, bool ConjB = false, detail::DeviceTensorConcept CType, detail::DeviceTensorConcept AType,
+          detail::DeviceTensorConcept BType, typename U, typename... CIndices, typename... AIndices, typename... BIndices>
+void einsum(U const C_prefactor, std::tuple<CIndices...> const &C_indices, CType *C, U const AB_prefactor,
+            std::tuple<AIndices...> const &A_indices, AType const &A, std::tuple<BIndices...> const &B_indices,
+            BType const &B, detail::AlgorithmChoice *algorithm_choice = nullptr) {
+    using T = typename CType::ValueType;
+    static_assert(std::is_same_v<T, typename AType::ValueType> && std::is_same_v<T, typename BType::ValueType>,
+                  "mixed element types are not supported on the device path");
+    static_assert(std::is_same_v<T, double> || std::is_same_v<T, float>, "FP64 and FP32 device tensors only");
+    static_assert(!ConjA && !ConjB, "conjugation applies to complex tensors, which this backend does not provide");
+    static_assert(sizeof...(CIndices) == CType::Rank, "Rank of C does not match Indices given for C.");
+    static_assert(sizeof...(AIndices) == AType::Rank, "Rank of A does not match Indices given for A.");
+    static_assert(sizeof...(BIndices) == BType::Rank, "Rank of B does not match Indices given for B.");
+
+    constexpr detail::AlgorithmChoice alg = detail::classify(std::tuple<CIndices...>{}, std::tuple<AIndices...>{},
+                                                             std::tuple<BIndices...>{});
+    constexpr auto c_ids = detail::label_ids(std::tuple<CIndices...>{});
+    constexpr auto a_ids = detail::label_ids(std::tuple<AIndices...>{});
+    constexpr auto b_ids = detail::label_ids(std::tuple<BIndices...>{});
+    (void)C_indices;
+    (void)A_indices;
+    (void)B_indices;
+    ein_tensor_desc cd = C->desc(c_ids.data()), ad = A.desc(a_ids.data()), bd = B.desc(b_ids.data());
+    int32_t used = -1;
+    int st;
+    if constexpr (std::is_same_v<T, double>)
+        st = ein_einsum_execute_f64(int32_t(alg), double(C_prefactor), &cd, C->data(), double(AB_prefactor), &ad, A.data(),
+                                    &bd, B.data(), &used, gpu::stream());
+    else
+        st = ein_einsum_execute_f32(int32_t(alg), float(C_prefactor), &cd, C->data(), float(AB_prefactor), &ad, A.data(),
+                                    &bd, B.data(), &used, gpu::stream());
+    check(st, "einsum");
+    if (algorithm_choice != nullptr) *algorithm_choice = detail::AlgorithmChoice(used);
+}
+
+// Default prefactors: C = A*B (C_prefactor 0, AB_prefactor 1).
+template <bool ConjA = false, bool ConjB = false, detail::DeviceTensorConcept CType, detail::DeviceTensorConcept AType,
+          detail::DeviceTensorConcept BType, typename... CIndices, typename... AIndices, typename... BIndices>
+void einsum(std::tuple<CIndices...> const &C_indices, CType *C, std::tuple<AIndices...> const &A_indices, AType const &A,
+            std::tuple<BIndices...> const &B_indices, BType const &B,
+            detail::AlgorithmChoice *algorithm_choice = nullptr) {
+    einsum<ConjA, ConjB>(typename CType::ValueType(0), C_indices, C, typename CType::ValueType(1), A_indices, A, B_indices, B,
+                         algorithm_choice);
+}
+
+// C = C_prefactor*C + A_prefactor*perm(A); the permutation is resolved at compile time.
+template <detail::DeviceTensorConcept CType, detail::DeviceTensorConcept AType, typename U, typename... CIndices,
+          typename... AIndices>
+void permute(U const C_prefactor, std::tuple<CIndices...> const &, CType *C, U const A_prefactor,
+             std::tuple<AIndices...> const &, AType const &A) {
+    using T = typename CType::ValueType;
+    static_assert(std::is_same_v<T, typename AType::ValueType>, "permute: element types differ");
+    constexpr size_t R = sizeof...(AIndices);
+    static_assert(sizeof...(CIndices) == R && CType::Rank == R && AType::Rank == R, "permute: rank mismatch");
+    constexpr auto perm = [] {
+        constexpr std::array<int32_t, R == 0 ? 1 : R> a{std::remove_cvref_t<AIndices>::id...};
+        constexpr std::array<int32_t, R == 0 ? 1 : R> c{std::remove_cvref_t<CIndices>::id...};
+        std::array<int32_t, R == 0 ? 1 : R> p{};
+        for (size_t i = 0; i < R; ++i) {
+            p[i] = -1;
+            for (size_t j = 0; j < R; ++j)
+                if (a[j] == c[i]) p[i] = int32_t(j);
+        }
+        return p;
+    }();
+    for (size_t i = 0; i < R; ++i)
+        if (perm[i] < 0) throw std::invalid_argument("permute: index sets differ");
+    std::array<int64_t, R == 0 ? 1 : R> dA{}, sA{}, sC{};
+    for (size_t i = 0; i < R; ++i) {
+        dA[i] = A.dim(i);
+        sA[i] = A.stride(i);
+        sC[i] = C->stride(i);
+        if (C->dim(i) != A.dim(size_t(perm[i]))) throw dimension_error("permute: extents do not match");
+    }
+    int st;
+    if constexpr (std::is_same_v<T, double>)
+        st = ein_permute_f64(int32_t(R), dA.data(), perm.data(), double(A_prefactor), A.data(), sA.data(),
+                             double(C_prefactor), C->data(), sC.data(), gpu::stream());
+    else
+        st = ein_permute_f32(int32_t(R), dA.data(), perm.data(), float(A_prefactor), A.data(), sA.data(),
+                             float(C_prefactor), C->data(), sC.data(), gpu::stream());
+    check(st, "permute");
+}
+
+template <detail::DeviceTensorConcept CType, detail::DeviceTensorConcept AType, typename... CIndices, typename... AIndices>
+void permute(std::tuple<CIndices...> const &Ci, CType *C, std::tuple<AIndices...> const &Ai, AType const &A) {
+    permute(typename CType::ValueType(0), Ci, C, typename CType::ValueType(1), Ai, A);
+}
+
+} // namespace einsums::tensor_algebra

@@ -6,7 +6,8 @@ import torch
 
 
 def dev(a: np.ndarray) -> torch.Tensor:
-    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    a = np.asarray(a)
+    return torch.from_numpy(a.copy(order="C")).to("cuda")
 
 
 def host(t: torch.Tensor) -> np.ndarray:

@@ -1,0 +1,31 @@
+"""The C++ header API (include/einsums): compile-time dispatch checks (CPU) and the
+DeviceTensor einsum/permute program (GPU). The programs are built by
+__graft_entry__.build() from tests/cpp/Makefile."""
+import subprocess
+from pathlib import Path
+
+import pytest
+
+BIN = Path(__file__).resolve().parent / "cpp" / "bin"
+
+
+def _run(name, timeout):
+    exe = BIN / name
+    if not exe.exists():
+        subprocess.run(["make", "-s", "-C", str(exe.parent.parent)], check=True)
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=timeout)
+    print(res.stdout)
+    print(res.stderr)
+    return res
+
+
+def test_compile_time_dispatch_program():
+    res = _run("test_compile_time", 60)
+    assert res.returncode == 0 and "passed" in res.stdout
+
+
+@pytest.mark.gpu
+def test_device_einsum_program():
+    res = _run("test_device_einsum", 120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "PASS test_device_einsum" in res.stdout

@@ -28,6 +28,23 @@ def vp(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+_alive = []
+
+
+@pytest.fixture(autouse=True)
+def _release_device_temporaries():
+    yield
+    torch.cuda.synchronize()
+    _alive.clear()
+
+
+def vd(a):
+    """Device copy of a host array kept alive until the test ends, as a void*."""
+    t = dev(a)
+    _alive.append(t)
+    return vp(t)
+
+
 @pytest.mark.parametrize("trans", ["NN", "TN", "NT", "TT"])
 def test_gemm_3x3_known_answers(trans):
     L = _lib()
@@ -75,7 +92,7 @@ def test_gemm_beta_zero_clears_nan():
     rng = np.random.default_rng(5)
     A, B = rng.uniform(-1, 1, 40 * 30), rng.uniform(-1, 1, 30 * 20)
     dC = dev(np.full(40 * 20, np.nan))
-    L.check(L.lib.ein_gemm_f64(b"N", b"N", 40, 20, 30, 1.0, vp(dev(A)), 40, vp(dev(B)), 30, 0.0, vp(dC), 40, None))
+    L.check(L.lib.ein_gemm_f64(b"N", b"N", 40, 20, 30, 1.0, vd((A)), 40, vd((B)), 30, 0.0, vp(dC), 40, None))
     assert np.isfinite(host(dC)).all()
 
 
@@ -113,7 +130,7 @@ def test_gemv(trans, m, n, incx, incy):
     yv = (Am @ xv) if trans == "N" else (Am.T @ xv)
     ref[::incy][:ly] = 0.5 * yv - 2.0 * y0[::incy][:ly]
     dy = dev(y0)
-    L.check(L.lib.ein_gemv_f64(trans.encode(), m, n, 0.5, vp(dev(A)), lda, vp(dev(x)), incx, -2.0, vp(dy), incy, None))
+    L.check(L.lib.ein_gemv_f64(trans.encode(), m, n, 0.5, vd((A)), lda, vd((x)), incx, -2.0, vp(dy), incy, None))
     assert rel_fro(host(dy), ref) < 1e-13
 
 
@@ -124,12 +141,12 @@ def test_dot(n, inc):
     rng = np.random.default_rng(n + inc)
     x, y = rng.uniform(-1, 1, n * inc), rng.uniform(-1, 1, n * inc)
     r = dev(np.array([0.3]))
-    L.check(L.lib.ein_dot_f64(n, 0.25, vp(dev(x)), inc, vp(dev(y)), inc, 2.0, vp(r), None))
+    L.check(L.lib.ein_dot_f64(n, 0.25, vd((x)), inc, vd((y)), inc, 2.0, vp(r), None))
     ref = 0.25 * float(np.dot(x[::inc][:n].astype(np.longdouble), y[::inc][:n].astype(np.longdouble))) + 0.6
     assert abs(host(r)[0] - ref) <= 1e-13 * max(1.0, abs(ref))
     # deterministic: a second call gives the identical bits
     r2 = dev(np.array([0.3]))
-    L.check(L.lib.ein_dot_f64(n, 0.25, vp(dev(x)), inc, vp(dev(y)), inc, 2.0, vp(r2), None))
+    L.check(L.lib.ein_dot_f64(n, 0.25, vd((x)), inc, vd((y)), inc, 2.0, vp(r2), None))
     assert host(r2)[0] == host(r)[0]
 
 
@@ -140,14 +157,14 @@ def test_ger_dirprod_scale_axpby():
     x, y = rng.uniform(-1, 1, m), rng.uniform(-1, 1, n)
     A = rng.uniform(-1, 1, (n, m + 5))  # column-major lda = m + 5
     dA = dev(A)
-    L.check(L.lib.ein_ger_f64(m, n, 1.5, vp(dev(x)), 1, vp(dev(y)), 1, vp(dA), m + 5, None))
+    L.check(L.lib.ein_ger_f64(m, n, 1.5, vd((x)), 1, vd((y)), 1, vp(dA), m + 5, None))
     ref = A.copy()
     ref[:, :m] += 1.5 * np.outer(y, x)
     assert rel_fro(host(dA), ref) < 1e-14
     z0 = rng.uniform(-1, 1, 1000)
     xx, yy = rng.uniform(-1, 1, 1000), rng.uniform(-1, 1, 1000)
     dz = dev(z0)
-    L.check(L.lib.ein_dirprod_f64(1000, 2.0, vp(dev(xx)), 1, vp(dev(yy)), 1, -1.0, vp(dz), 1, None))
+    L.check(L.lib.ein_dirprod_f64(1000, 2.0, vd((xx)), 1, vd((yy)), 1, -1.0, vp(dz), 1, None))
     assert rel_fro(host(dz), 2.0 * xx * yy - z0) < 1e-15
     dz = dev(z0)
     L.check(L.lib.ein_scale_f64(500, 3.0, vp(dz), 2, None))
@@ -155,5 +172,5 @@ def test_ger_dirprod_scale_axpby():
     ref[::2] *= 3.0
     assert np.array_equal(host(dz), ref)
     dz = dev(z0)
-    L.check(L.lib.ein_axpby_f64(1000, 2.0, vp(dev(xx)), 1, 0.5, vp(dz), 1, None))
+    L.check(L.lib.ein_axpby_f64(1000, 2.0, vd((xx)), 1, 0.5, vp(dz), 1, None))
     assert rel_fro(host(dz), 2.0 * xx + 0.5 * z0) < 1e-15

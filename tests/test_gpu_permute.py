@@ -23,8 +23,8 @@ def labels(n):
 
 
 CASES = [
-    ((24, 24, 96, 96), "iajb"),   # CCD ijab -> iajb: innermost axis kept (row copy, 2-wide vectors)
-    ((24, 24, 96, 96), "abij"),   # innermost axis changes (LDS tile transpose)
+    ((24, 24, 96, 96), "ikjl"),   # CCD ijab -> iajb: innermost axis kept (row copy, 2-wide vectors)
+    ((24, 24, 96, 96), "klij"),   # ijab -> abij: innermost axis changes (LDS tile transpose)
     ((5, 7, 9), "kji"),
     ((64, 65), "ji"),
     ((1, 130, 1, 67), "lkji"),
@@ -39,7 +39,6 @@ CASES = [
 def test_permute_bit_exact(core, oracle, shape, out, dtype):
     rng = seeded(shape, out, dtype.__name__)
     inp = labels(len(shape))
-    out = out.replace("m", "m")
     A = rng.standard_normal(shape).astype(dtype)
     A.flat[:: max(1, A.size // 7)] = -0.0  # signed zeros must survive
     perm = [inp.index(c) for c in out]
