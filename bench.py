@@ -34,7 +34,27 @@ def parse():
     p.add_argument("--v", type=int, default=96)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--traffic", type=str, default=None, help="JSON file with measured HBM bytes per Wabef launch")
+    p.add_argument("--split", type=int, default=5, help="slices for the split-precision comparison (0: skip)")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                   help="only run a plain device copy of BYTES bytes (counter calibration) and exit")
     return p.parse_args()
+
+
+def calibrate_copy(nbytes: int) -> None:
+    """A plain copy kernel of known size (identity permute, 16-byte vectors) for scaling
+    the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+
+    import einsums_amd.core as core
+
+    n = nbytes // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        core.permute(0.0, "i", dst, 1.0, "i", src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst, src)
+    print(json.dumps({"calibrate_copy_bytes_read": n * 8, "calibrate_copy_bytes_written": n * 8, "launches": 3}))
 
 
 def cpu_baseline(I, o, v, flops):
@@ -63,8 +83,45 @@ def cpu_baseline(I, o, v, flops):
                       f"host BLAS: {os.path.basename(pyoracle.blas_name())}"}
 
 
+def split_comparison(core, plans, W, I, ns, reps=10):
+    """Wabef and Wmnij through the split-precision GEMM vs plain FP64 on the same inputs:
+    speedup of the kernel time and relative Frobenius error of the contraction against
+    the plain FP64 result (itself within 1e-13 of the CPU oracle in the parity tests)."""
+    import numpy as np
+    import torch
+
+    out = {"nslices": ns, "slice_type": "bf16"}
+    for name in ("Wabef", "Wmnij"):
+        plan, A, B, abp, _ = plans[name]
+        base = W[name].clone()
+        ref = torch.zeros_like(base)
+        plan.execute(0.0, ref, abp, A, B)
+        got = torch.zeros_like(base)
+        plan.execute(0.0, got, abp, A, B, split=ns)
+        torch.cuda.synchronize()
+        err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+        times = {}
+        for mode in (0, ns):
+            for _ in range(2):
+                plan.execute(0.0, got, abp, A, B, split=mode)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                plan.execute(0.0, got, abp, A, B, split=mode)
+            e1.record()
+            torch.cuda.synchronize()
+            times[mode] = e0.elapsed_time(e1) / reps
+        out[name] = {"fp64_ms": round(times[0], 4), "split_ms": round(times[ns], 4),
+                     "speedup": round(times[0] / times[ns], 3), "rel_err_vs_fp64": float(f"{err:.3e}")}
+    return out
+
+
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
@@ -156,6 +213,8 @@ def main():
                      "traffic": traffic},
         "kernels": kernels,
     }
+    if args.split:
+        out["split"] = split_comparison(core, plans, W, I, args.split)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(I, o, v, flops)
     print(json.dumps(out))

@@ -149,179 +149,198 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProb
     int const wm   = wave & 1;
     int const wn   = wave >> 1;
 
-    uint32_t const ntiles  = P.tiles_m * P.tiles_n;
-    uint32_t const logical = xcd_remap(blockIdx.x, ntiles);
-    uint32_t tm, tn;
-    tile_coords(logical, P.tiles_m, P.tiles_n, tm, tn);
-    int64_t const i0 = int64_t(tm) * BM;
-    int64_t const j0 = int64_t(tn) * BN;
-
-    int const split    = blockIdx.y;
-    int64_t const kbeg = int64_t(split) * P.k_chunk;
-    int64_t const kend = kbeg + P.k_chunk < P.k ? kbeg + P.k_chunk : P.k;
-
-    int64_t boff[3];
-    group_decode<3>(uint32_t(blockIdx.z), P.gb, boff);
-    T const *__restrict__ Ab = P.A + boff[0];
-    T const *__restrict__ Bb = P.B + boff[1];
-
     constexpr int A_STEP = A_I_FAST ? GEMM_THREADS / BM : GEMM_THREADS / BK;
     int const a_ii       = A_I_FAST ? tid % BM : tid / BK;
     int const a_pp       = A_I_FAST ? tid / BM : tid % BK;
     constexpr int B_STEP = B_J_FAST ? GEMM_THREADS / BN : GEMM_THREADS / BK;
     int const b_jj       = B_J_FAST ? tid % BN : tid / BK;
     int const b_pp       = B_J_FAST ? tid / BN : tid % BK;
+    int const r          = lane & 15;
+    int const q          = lane >> 4;
 
-    // Row / column offsets fixed for the whole tile (-1 marks out of range).
-    constexpr int NRA = A_I_FAST ? 1 : Cfg::LA;
-    constexpr int NRB = B_J_FAST ? 1 : Cfg::LB;
-    int64_t rowA[NRA], colB[NRB];
+    StreamK const &sk       = P.sk;
+    uint64_t const S        = sk.slabs;
+    uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
+    uint64_t u0, u1;
+    sk_range(sk, u0, u1);
+    uint64_t const wlog = sk.units ? u0 / sk.units : 0;
+
+    for (uint64_t u = u0; u < u1;) {
+        uint64_t const t  = u / S;
+        uint64_t const s0 = u - t * S;
+        uint64_t const s1 = (S - s0 < u1 - u) ? S : s0 + (u1 - u);
+        u += s1 - s0;
+
+        uint32_t const bz = uint32_t(t / tiles_mn);
+        uint32_t tm, tn;
+        tile_coords(uint32_t(t % tiles_mn), P.tiles_m, P.tiles_n, tm, tn);
+        int64_t const i0   = int64_t(tm) * BM;
+        int64_t const j0   = int64_t(tn) * BN;
+        int64_t const kbeg = int64_t(s0) * BK;
+        int64_t const kend = int64_t(s1) * BK < P.k ? int64_t(s1) * BK : P.k;
+
+        int64_t boff[3];
+        group_decode<3>(bz, P.gb, boff);
+        T const *__restrict__ Ab = P.A + boff[0];
+        T const *__restrict__ Bb = P.B + boff[1];
+
+        // Row / column offsets fixed for the whole tile (NO_ROW marks out of range).
+        constexpr int NRA = A_I_FAST ? 1 : Cfg::LA;
+        constexpr int NRB = B_J_FAST ? 1 : Cfg::LB;
+        int64_t rowA[NRA], colB[NRB];
 #pragma unroll
-    for (int r = 0; r < NRA; ++r) {
-        int64_t i = i0 + (A_I_FAST ? a_ii : a_ii + r * A_STEP);
-        int64_t o[2];
-        group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
-        rowA[r] = i < P.m ? o[0] : NO_ROW;
-    }
-#pragma unroll
-    for (int r = 0; r < NRB; ++r) {
-        int64_t j = j0 + (B_J_FAST ? b_jj : b_jj + r * B_STEP);
-        int64_t o[2];
-        group_decode<2>(uint32_t(j < P.n ? j : 0), P.gn, o);
-        colB[r] = j < P.n ? o[0] : NO_ROW;
-    }
-
-    T ra[Cfg::LA], rb[Cfg::LB];
-
-    auto load_slab = [&](int64_t k0) {
-        // k offsets for the p-fast operands: one decode serves both A and B when both
-        // walk p with the same lane mapping
-        int64_t kfix[2] = {0, 0};
-        bool kfix_ok    = false;
-        if (!A_I_FAST || !B_J_FAST) {
-            int64_t p = k0 + (A_I_FAST ? b_pp : a_pp);
-            kfix_ok   = p < kend;
-            group_decode<2>(uint32_t(kfix_ok ? p : 0), P.gk, kfix);
+        for (int rr = 0; rr < NRA; ++rr) {
+            int64_t i = i0 + (A_I_FAST ? a_ii : a_ii + rr * A_STEP);
+            int64_t o[2];
+            group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
+            rowA[rr] = i < P.m ? o[0] : NO_ROW;
         }
 #pragma unroll
-        for (int r = 0; r < Cfg::LA; ++r) {
-            T v = T(0);
-            if constexpr (A_I_FAST) {
-                int64_t p = k0 + a_pp + r * A_STEP;
-                if (rowA[0] != NO_ROW && p < kend) {
-                    int64_t o[2];
-                    group_decode<2>(uint32_t(p), P.gk, o);
-                    v = Ab[rowA[0] + o[0]];
-                }
-            } else {
-                if (rowA[r] != NO_ROW && kfix_ok) v = Ab[rowA[r] + kfix[0]];
+        for (int rr = 0; rr < NRB; ++rr) {
+            int64_t j = j0 + (B_J_FAST ? b_jj : b_jj + rr * B_STEP);
+            int64_t o[2];
+            group_decode<2>(uint32_t(j < P.n ? j : 0), P.gn, o);
+            colB[rr] = j < P.n ? o[0] : NO_ROW;
+        }
+
+        T ra[Cfg::LA], rb[Cfg::LB];
+        auto load_slab = [&](int64_t k0) {
+            // k offsets for the p-fast operands: one decode serves both A and B when both
+            // walk p with the same lane mapping
+            int64_t kfix[2] = {0, 0};
+            bool kfix_ok    = false;
+            if (!A_I_FAST || !B_J_FAST) {
+                int64_t p = k0 + (A_I_FAST ? b_pp : a_pp);
+                kfix_ok   = p < kend;
+                group_decode<2>(uint32_t(kfix_ok ? p : 0), P.gk, kfix);
             }
-            ra[r] = v;
-        }
 #pragma unroll
-        for (int r = 0; r < Cfg::LB; ++r) {
-            T v = T(0);
-            if constexpr (B_J_FAST) {
-                int64_t p = k0 + b_pp + r * B_STEP;
-                if (colB[0] != NO_ROW && p < kend) {
-                    int64_t o[2];
-                    group_decode<2>(uint32_t(p), P.gk, o);
-                    v = Bb[colB[0] + o[1]];
+            for (int rr = 0; rr < Cfg::LA; ++rr) {
+                T v = T(0);
+                if constexpr (A_I_FAST) {
+                    int64_t p = k0 + a_pp + rr * A_STEP;
+                    if (rowA[0] != NO_ROW && p < kend) {
+                        int64_t o[2];
+                        group_decode<2>(uint32_t(p), P.gk, o);
+                        v = Ab[rowA[0] + o[0]];
+                    }
+                } else {
+                    if (rowA[rr] != NO_ROW && kfix_ok) v = Ab[rowA[rr] + kfix[0]];
                 }
-            } else {
-                if (colB[r] != NO_ROW && kfix_ok) v = Bb[colB[r] + kfix[1]];
+                ra[rr] = v;
             }
-            rb[r] = v;
-        }
-    };
-    auto store_slab = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < Cfg::LA; ++r) {
-            int ii = A_I_FAST ? a_ii : a_ii + r * A_STEP;
-            int pp = A_I_FAST ? a_pp + r * A_STEP : a_pp;
-            As[buf][pp * Cfg::LDA + ii] = ra[r];
-        }
+            for (int rr = 0; rr < Cfg::LB; ++rr) {
+                T v = T(0);
+                if constexpr (B_J_FAST) {
+                    int64_t p = k0 + b_pp + rr * B_STEP;
+                    if (colB[0] != NO_ROW && p < kend) {
+                        int64_t o[2];
+                        group_decode<2>(uint32_t(p), P.gk, o);
+                        v = Bb[colB[0] + o[1]];
+                    }
+                } else {
+                    if (colB[rr] != NO_ROW && kfix_ok) v = Bb[colB[rr] + kfix[1]];
+                }
+                rb[rr] = v;
+            }
+        };
+        auto store_slab = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < Cfg::LB; ++r) {
-            int jj = B_J_FAST ? b_jj : b_jj + r * B_STEP;
-            int pp = B_J_FAST ? b_pp + r * B_STEP : b_pp;
-            Bs[buf][pp * Cfg::LDB + jj] = rb[r];
-        }
-    };
+            for (int rr = 0; rr < Cfg::LA; ++rr) {
+                int ii = A_I_FAST ? a_ii : a_ii + rr * A_STEP;
+                int pp = A_I_FAST ? a_pp + rr * A_STEP : a_pp;
+                As[buf][pp * Cfg::LDA + ii] = ra[rr];
+            }
+#pragma unroll
+            for (int rr = 0; rr < Cfg::LB; ++rr) {
+                int jj = B_J_FAST ? b_jj : b_jj + rr * B_STEP;
+                int pp = B_J_FAST ? b_pp + rr * B_STEP : b_pp;
+                Bs[buf][pp * Cfg::LDB + jj] = rb[rr];
+            }
+        };
 
-    WaveAcc<T, Cfg::FM, Cfg::FN> acc;
-    acc.zero();
-    int64_t const nslabs = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
-    if (nslabs > 0) {
+        WaveAcc<T, Cfg::FM, Cfg::FN> acc;
+        acc.zero();
+        int64_t const nslabs = int64_t(s1 - s0);
         load_slab(kbeg);
         store_slab(0);
         __syncthreads();
-        for (int64_t s = 0; s < nslabs; ++s) {
-            int const buf = int(s & 1);
-            if (s + 1 < nslabs) load_slab(kbeg + (s + 1) * BK);
+        for (int64_t sl = 0; sl < nslabs; ++sl) {
+            int const buf = int(sl & 1);
+            if (sl + 1 < nslabs) load_slab(kbeg + (sl + 1) * BK);
             mfma_slab<T, BM, BN, BK>(As[buf], Bs[buf], acc, wm, wn, lane);
-            if (s + 1 < nslabs) store_slab(buf ^ 1);
+            if (sl + 1 < nslabs) store_slab(buf ^ 1);
             __syncthreads();
         }
-    }
 
-    int const r = lane & 15;
-    int const q = lane >> 4;
-    if (P.splits > 1) {
-        T *__restrict__ part = P.partial + (int64_t(split) * P.batch + blockIdx.z) * P.m * P.n;
+        if (!(s0 == 0 && s1 == S)) {
+            T *__restrict__ piece = P.partial + sk_slot(sk, wlog, t) * (BM * BN);
 #pragma unroll
-        for (int a = 0; a < Cfg::FM; ++a)
+            for (int a = 0; a < Cfg::FM; ++a)
 #pragma unroll
-            for (int b = 0; b < Cfg::FN; ++b)
+                for (int b = 0; b < Cfg::FN; ++b)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
-                    int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
-                    if (i < P.m && j < P.n) part[i + j * P.m] = acc.c[a][b][v];
-                }
-        return;
-    }
-    T *__restrict__ Cb = P.C + boff[2];
-    int64_t rowC[Cfg::FM];
-#pragma unroll
-    for (int a = 0; a < Cfg::FM; ++a) {
-        int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
-        int64_t o[2];
-        group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
-        rowC[a] = i < P.m ? o[1] : NO_ROW;
-    }
-#pragma unroll
-    for (int b = 0; b < Cfg::FN; ++b)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
-            if (j >= P.n) continue;
-            int64_t o[2];
-            group_decode<2>(uint32_t(j), P.gn, o);
-#pragma unroll
-            for (int a = 0; a < Cfg::FM; ++a) {
-                if (rowC[a] == NO_ROW) continue;
-                T *cp = Cb + rowC[a] + o[1];
-                T val = P.alpha * acc.c[a][b][v];
-                if (P.beta != T(0)) val += P.beta * *cp;
-                *cp = val;
-            }
+                    for (int v = 0; v < 4; ++v) {
+                        int ii = wm * Cfg::WM + a * 16 + r;
+                        int jj = wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
+                        piece[jj * BM + ii] = acc.c[a][b][v];
+                    }
+            continue;
         }
+        T *__restrict__ Cb = P.C + boff[2];
+        int64_t rowC[Cfg::FM];
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a) {
+            int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
+            int64_t o[2];
+            group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
+            rowC[a] = i < P.m ? o[1] : NO_ROW;
+        }
+#pragma unroll
+        for (int b = 0; b < Cfg::FN; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
+                if (j >= P.n) continue;
+                int64_t o[2];
+                group_decode<2>(uint32_t(j), P.gn, o);
+#pragma unroll
+                for (int a = 0; a < Cfg::FM; ++a) {
+                    if (rowC[a] == NO_ROW) continue;
+                    T *cp = Cb + rowC[a] + o[1];
+                    T val = P.alpha * acc.c[a][b][v];
+                    if (P.beta != T(0)) val += P.beta * *cp;
+                    *cp = val;
+                }
+            }
+    }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void gather_splitk_reduce_kernel(GatherProblem<T> P) {
-    int64_t const mn    = P.m * P.n;
-    int64_t const total = mn * P.batch;
-    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
-        int64_t bz  = e / mn;
-        int64_t rem = e - bz * mn;
-        int64_t j   = rem / P.m;
-        int64_t i   = rem - j * P.m;
-        T sum       = T(0);
-        for (int s = 0; s < P.splits; ++s) sum += P.partial[(int64_t(s) * P.batch + bz) * mn + rem];
-        int64_t ob[3], om[2], on[2];
-        group_decode<3>(uint32_t(bz), P.gb, ob);
+// Stream-K fix-up for the gather GEMM: sums the pieces of every cut tile in workgroup
+// order and writes C through the decoded row/column offsets.
+template <typename T, int BM, int BN>
+__global__ __launch_bounds__(256) void gather_streamk_fixup(GatherProblem<T> P) {
+    StreamK const &sk = P.sk;
+    uint64_t const t  = blockIdx.x;
+    uint64_t const S  = sk.slabs;
+    uint64_t const g0 = (t * S) / sk.units;
+    uint64_t const g1 = ((t + 1) * S - 1) / sk.units;
+    if (g0 == g1) return;
+    uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
+    uint32_t const bz       = uint32_t(t / tiles_mn);
+    uint32_t tm, tn;
+    tile_coords(uint32_t(t % tiles_mn), P.tiles_m, P.tiles_n, tm, tn);
+    int64_t const i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
+    int64_t ob[3];
+    group_decode<3>(bz, P.gb, ob);
+    for (int e = threadIdx.x; e < BM * BN; e += 256) {
+        int ii = e % BM, jj = e / BM;
+        int64_t i = i0 + ii, j = j0 + jj;
+        if (i >= P.m || j >= P.n) continue;
+        T sum = T(0);
+        for (uint64_t g = g0; g <= g1; ++g) sum += P.partial[sk_slot(sk, g, t) * (BM * BN) + e];
+        int64_t om[2], on[2];
         group_decode<2>(uint32_t(i), P.gm, om);
         group_decode<2>(uint32_t(j), P.gn, on);
         T *cp = P.C + ob[2] + om[1] + on[1];
@@ -526,32 +545,30 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         P.alpha = ab_prefactor;
         P.beta  = beta;
         GemmConfig cfg = choose_gemm_config(M, N, K, NB, int(sizeof(T)));
-        int bm = cfg.bm == 128 ? 128 : 64;
-        int bn = 64;
-        P.tiles_m = uint32_t(ceil_div(M, bm));
-        P.tiles_n = uint32_t(ceil_div(N, bn));
-        int64_t tiles = int64_t(P.tiles_m) * P.tiles_n * NB;
-        int64_t target = int64_t(num_cus()) * 3;
-        int splits = 1;
-        if (tiles < target && K >= 512) splits = int(std::min<int64_t>({ceil_div(target, tiles), K / 256, 64}));
-        P.k_chunk = ceil_div(ceil_div(K, splits), 16) * 16;
-        P.splits  = int(ceil_div(K, P.k_chunk));
+        int const bm   = cfg.bm == 128 ? 128 : 64;
+        constexpr int BN = 64;
+        P.tiles_m        = uint32_t(ceil_div(M, bm));
+        P.tiles_n        = uint32_t(ceil_div(N, BN));
+        uint64_t tiles   = uint64_t(P.tiles_m) * P.tiles_n * uint64_t(NB);
+        if (tiles > 0x7fffffffu) return fail(EIN_ERROR_UNSUPPORTED, "contraction tile grid too large");
+        P.sk      = plan_stream_k(tiles, uint64_t(ceil_div(K, 16)), uint64_t(num_cus()) * 2);
         P.partial = nullptr;
-        if (P.splits > 1) {
+        if (P.sk.units != 0) {
             void *ws = nullptr;
-            EIN_TRY(workspace(stream, size_t(P.splits) * size_t(NB) * size_t(M) * size_t(N) * sizeof(T), &ws));
+            EIN_TRY(workspace(stream, size_t(P.sk.grid) * 2 * size_t(bm) * BN * sizeof(T), &ws));
             P.partial = static_cast<T *>(ws);
         }
-        dim3 grid{P.tiles_m * P.tiles_n, uint32_t(P.splits), uint32_t(NB)};
+        dim3 grid{P.sk.grid};
         if (bm == 128)
             launch_gather<T, 128, 64>(P, a_fast, b_j_fast, grid, stream);
         else
             launch_gather<T, 64, 64>(P, a_fast, b_j_fast, grid, stream);
         EIN_LAUNCH_CHECK();
-        if (P.splits > 1) {
-            int64_t total = M * N * NB;
-            int blocks    = int(std::min<int64_t>(ceil_div(total, 256), 65536));
-            gather_splitk_reduce_kernel<T><<<blocks, 256, 0, stream>>>(P);
+        if (P.sk.units != 0) {
+            if (bm == 128)
+                gather_streamk_fixup<T, 128, 64><<<uint32_t(tiles), 256, 0, stream>>>(P);
+            else
+                gather_streamk_fixup<T, 64, 64><<<uint32_t(tiles), 256, 0, stream>>>(P);
             EIN_LAUNCH_CHECK();
         }
         return EIN_SUCCESS;

@@ -41,8 +41,7 @@ struct GatherProblem {
     T *C;
     T alpha, beta;
     uint32_t tiles_m, tiles_n;
-    int32_t splits;
-    int64_t k_chunk;
+    StreamK sk;
     T *partial;
 };
 

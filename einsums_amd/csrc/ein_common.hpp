@@ -89,7 +89,7 @@ __device__ __forceinline__ T wave_reduce_sum(T v) {
 // Maps a linear workgroup id to one that keeps consecutive logical tiles on the same
 // XCD: the hardware dispatches workgroup ids round-robin over the 8 XCDs, so the
 // XCD of id w is w % 8. The returned id enumerates all ids of XCD 0 first, then XCD 1...
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t wg, uint32_t nwg) {
+__host__ __device__ __forceinline__ uint32_t xcd_remap(uint32_t wg, uint32_t nwg) {
     constexpr uint32_t NXCD = 8;
     uint32_t xcd  = wg % NXCD;
     uint32_t idx  = wg / NXCD;

@@ -1,5 +1,5 @@
-// Dense strided GEMM on MFMA for gfx950 (FP64 and FP32), with strided batching and
-// deterministic split-K.
+// Dense strided GEMM on MFMA for gfx950 (FP64 and FP32), strided batching, and a
+// deterministic stream-K schedule for launches whose tile count does not fill the GPU.
 //
 // C(i,j) = alpha * sum_p A(i,p) B(p,j) + beta * C(i,j) with
 //   A(i,p) = A[i*a_sm + p*a_sk + b*a_sb], B(p,j) = B[p*b_sk + j*b_sn + b*b_sb],
@@ -25,128 +25,143 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
     int const wave = tid >> 6;
     int const wm   = wave & 1;
     int const wn   = wave >> 1;
+    int const r    = lane & 15;
+    int const q    = lane >> 4;
 
-    uint32_t const ntiles = P.tiles_m * P.tiles_n;
-    uint32_t const logical = xcd_remap(blockIdx.x, ntiles);
-    uint32_t tm, tn;
-    tile_coords(logical, P.tiles_m, P.tiles_n, tm, tn);
-    int64_t const i0 = int64_t(tm) * BM;
-    int64_t const j0 = int64_t(tn) * BN;
-
-    int const split      = blockIdx.y;
-    int64_t const kbeg   = int64_t(split) * P.k_chunk;
-    int64_t const kend   = kbeg + P.k_chunk < P.k ? kbeg + P.k_chunk : P.k;
-    int64_t const batch  = blockIdx.z;
-    T const *__restrict__ Ab = P.A + batch * P.a_sb;
-    T const *__restrict__ Bb = P.B + batch * P.b_sb;
-
-    // ---- per-thread load coordinates
-    // A slab: BM x BK. i-fast: consecutive threads walk i; else walk p.
+    // per-thread load coordinates: A slab BM x BK, B slab BK x BN
     constexpr int A_STEP = A_I_FAST ? GEMM_THREADS / BM : GEMM_THREADS / BK;
-    int const a_ii = A_I_FAST ? tid % BM : tid / BK;
-    int const a_pp = A_I_FAST ? tid / BM : tid % BK;
+    int const a_ii       = A_I_FAST ? tid % BM : tid / BK;
+    int const a_pp       = A_I_FAST ? tid / BM : tid % BK;
     constexpr int B_STEP = B_J_FAST ? GEMM_THREADS / BN : GEMM_THREADS / BK;
-    int const b_jj = B_J_FAST ? tid % BN : tid / BK;
-    int const b_pp = B_J_FAST ? tid / BN : tid % BK;
+    int const b_jj       = B_J_FAST ? tid % BN : tid / BK;
+    int const b_pp       = B_J_FAST ? tid / BN : tid % BK;
 
-    T ra[Cfg::LA], rb[Cfg::LB];
+    StreamK const &sk       = P.sk;
+    uint64_t const S        = sk.slabs;
+    uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
+    uint64_t u0, u1;
+    sk_range(sk, u0, u1);
+    uint64_t const wlog = sk.units ? u0 / sk.units : 0;
 
-    auto load_slab = [&](int64_t k0) {
-#pragma unroll
-        for (int r = 0; r < Cfg::LA; ++r) {
-            int ii    = A_I_FAST ? a_ii : a_ii + r * A_STEP;
-            int pp    = A_I_FAST ? a_pp + r * A_STEP : a_pp;
-            int64_t i = i0 + ii, p = k0 + pp;
-            ra[r]     = (i < P.m && p < kend) ? Ab[i * P.a_sm + p * P.a_sk] : T(0);
-        }
-#pragma unroll
-        for (int r = 0; r < Cfg::LB; ++r) {
-            int jj    = B_J_FAST ? b_jj : b_jj + r * B_STEP;
-            int pp    = B_J_FAST ? b_pp + r * B_STEP : b_pp;
-            int64_t j = j0 + jj, p = k0 + pp;
-            rb[r]     = (j < P.n && p < kend) ? Bb[p * P.b_sk + j * P.b_sn] : T(0);
-        }
-    };
-    auto store_slab = [&](int buf) {
-#pragma unroll
-        for (int r = 0; r < Cfg::LA; ++r) {
-            int ii = A_I_FAST ? a_ii : a_ii + r * A_STEP;
-            int pp = A_I_FAST ? a_pp + r * A_STEP : a_pp;
-            As[buf][pp * Cfg::LDA + ii] = ra[r];
-        }
-#pragma unroll
-        for (int r = 0; r < Cfg::LB; ++r) {
-            int jj = B_J_FAST ? b_jj : b_jj + r * B_STEP;
-            int pp = B_J_FAST ? b_pp + r * B_STEP : b_pp;
-            Bs[buf][pp * Cfg::LDB + jj] = rb[r];
-        }
-    };
+    for (uint64_t u = u0; u < u1;) {
+        uint64_t const t  = u / S;
+        uint64_t const s0 = u - t * S;
+        uint64_t const s1 = (S - s0 < u1 - u) ? S : s0 + (u1 - u);
+        u += s1 - s0;
 
-    WaveAcc<T, Cfg::FM, Cfg::FN> acc;
-    acc.zero();
+        int64_t const batch = int64_t(t / tiles_mn);
+        uint32_t tm, tn;
+        tile_coords(uint32_t(t % tiles_mn), P.tiles_m, P.tiles_n, tm, tn);
+        int64_t const i0   = int64_t(tm) * BM;
+        int64_t const j0   = int64_t(tn) * BN;
+        int64_t const kbeg = int64_t(s0) * BK;
+        int64_t const kend = int64_t(s1) * BK < P.k ? int64_t(s1) * BK : P.k;
+        T const *__restrict__ Ab = P.A + batch * P.a_sb;
+        T const *__restrict__ Bb = P.B + batch * P.b_sb;
 
-    int64_t const nslabs = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
-    if (nslabs > 0) {
+        T ra[Cfg::LA], rb[Cfg::LB];
+        auto load_slab = [&](int64_t k0) {
+#pragma unroll
+            for (int rr = 0; rr < Cfg::LA; ++rr) {
+                int ii    = A_I_FAST ? a_ii : a_ii + rr * A_STEP;
+                int pp    = A_I_FAST ? a_pp + rr * A_STEP : a_pp;
+                int64_t i = i0 + ii, p = k0 + pp;
+                ra[rr]    = (i < P.m && p < kend) ? Ab[i * P.a_sm + p * P.a_sk] : T(0);
+            }
+#pragma unroll
+            for (int rr = 0; rr < Cfg::LB; ++rr) {
+                int jj    = B_J_FAST ? b_jj : b_jj + rr * B_STEP;
+                int pp    = B_J_FAST ? b_pp + rr * B_STEP : b_pp;
+                int64_t j = j0 + jj, p = k0 + pp;
+                rb[rr]    = (j < P.n && p < kend) ? Bb[p * P.b_sk + j * P.b_sn] : T(0);
+            }
+        };
+        auto store_slab = [&](int buf) {
+#pragma unroll
+            for (int rr = 0; rr < Cfg::LA; ++rr) {
+                int ii = A_I_FAST ? a_ii : a_ii + rr * A_STEP;
+                int pp = A_I_FAST ? a_pp + rr * A_STEP : a_pp;
+                As[buf][pp * Cfg::LDA + ii] = ra[rr];
+            }
+#pragma unroll
+            for (int rr = 0; rr < Cfg::LB; ++rr) {
+                int jj = B_J_FAST ? b_jj : b_jj + rr * B_STEP;
+                int pp = B_J_FAST ? b_pp + rr * B_STEP : b_pp;
+                Bs[buf][pp * Cfg::LDB + jj] = rb[rr];
+            }
+        };
+
+        WaveAcc<T, Cfg::FM, Cfg::FN> acc;
+        acc.zero();
+        int64_t const nslabs = int64_t(s1 - s0);
         load_slab(kbeg);
         store_slab(0);
         __syncthreads();
-        for (int64_t s = 0; s < nslabs; ++s) {
-            int const buf = int(s & 1);
-            if (s + 1 < nslabs) load_slab(kbeg + (s + 1) * BK);
+        for (int64_t sl = 0; sl < nslabs; ++sl) {
+            int const buf = int(sl & 1);
+            if (sl + 1 < nslabs) load_slab(kbeg + (sl + 1) * BK);
             mfma_slab<T, BM, BN, BK>(As[buf], Bs[buf], acc, wm, wn, lane);
-            if (s + 1 < nslabs) store_slab(buf ^ 1);
+            if (sl + 1 < nslabs) store_slab(buf ^ 1);
             __syncthreads();
         }
-    }
 
-    // ---- epilogue
-    int const r = lane & 15;
-    int const q = lane >> 4;
-    if (P.splits > 1) {
-        T *__restrict__ part = P.partial + (int64_t(split) * P.batch + batch) * P.m * P.n;
+        if (s0 == 0 && s1 == S) {
+            T *__restrict__ Cb = P.C + batch * P.c_sb;
 #pragma unroll
-        for (int a = 0; a < Cfg::FM; ++a)
+            for (int a = 0; a < Cfg::FM; ++a)
 #pragma unroll
-            for (int b = 0; b < Cfg::FN; ++b)
+                for (int b = 0; b < Cfg::FN; ++b)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
-                    int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
-                    if (i < P.m && j < P.n) part[i + j * P.m] = acc.c[a][b][v];
-                }
-        return;
+                    for (int v = 0; v < 4; ++v) {
+                        int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
+                        int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
+                        if (i < P.m && j < P.n) {
+                            T *cp = Cb + i * P.c_sm + j * P.c_sn;
+                            T val = P.alpha * acc.c[a][b][v];
+                            if (P.beta != T(0)) val += P.beta * *cp;
+                            *cp = val;
+                        }
+                    }
+        } else {
+            // tile cut by the range ends: raw accumulators to this workgroup's slot, [BN][BM]
+            T *__restrict__ piece = P.partial + sk_slot(sk, wlog, t) * (BM * BN);
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+                for (int b = 0; b < Cfg::FN; ++b)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        int ii = wm * Cfg::WM + a * 16 + r;
+                        int jj = wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
+                        piece[jj * BM + ii] = acc.c[a][b][v];
+                    }
+        }
     }
-    T *__restrict__ Cb = P.C + batch * P.c_sb;
-#pragma unroll
-    for (int a = 0; a < Cfg::FM; ++a)
-#pragma unroll
-        for (int b = 0; b < Cfg::FN; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
-                int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
-                if (i < P.m && j < P.n) {
-                    T *cp = Cb + i * P.c_sm + j * P.c_sn;
-                    T val = P.alpha * acc.c[a][b][v];
-                    if (P.beta != T(0)) val += P.beta * *cp;
-                    *cp = val;
-                }
-            }
 }
 
-// C = alpha * sum_s partial[s] + beta * C, one thread per element, i fastest.
-template <typename T>
-__global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmProblem<T> P) {
-    int64_t const mn    = P.m * P.n;
-    int64_t const total = mn * P.batch;
-    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
-        int64_t b   = e / mn;
-        int64_t rem = e - b * mn;
-        int64_t j   = rem / P.m;
-        int64_t i   = rem - j * P.m;
-        T sum       = T(0);
-        for (int s = 0; s < P.splits; ++s) sum += P.partial[(int64_t(s) * P.batch + b) * mn + rem];
-        T *cp = P.C + b * P.c_sb + i * P.c_sm + j * P.c_sn;
+// Stream-K fix-up: one workgroup per tile; tiles finished in place return at once. The
+// pieces of a cut tile are summed in workgroup order, then alpha and beta are applied.
+template <typename T, int BM, int BN>
+__global__ __launch_bounds__(256) void gemm_streamk_fixup(GemmProblem<T> P) {
+    StreamK const &sk = P.sk;
+    uint64_t const t  = blockIdx.x;
+    uint64_t const S  = sk.slabs;
+    uint64_t const g0 = (t * S) / sk.units;
+    uint64_t const g1 = ((t + 1) * S - 1) / sk.units;
+    if (g0 == g1) return;
+    uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
+    int64_t const batch     = int64_t(t / tiles_mn);
+    uint32_t tm, tn;
+    tile_coords(uint32_t(t % tiles_mn), P.tiles_m, P.tiles_n, tm, tn);
+    int64_t const i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
+    T *__restrict__ Cb = P.C + batch * P.c_sb;
+    for (int e = threadIdx.x; e < BM * BN; e += 256) {
+        int ii = e % BM, jj = e / BM;
+        int64_t i = i0 + ii, j = j0 + jj;
+        if (i >= P.m || j >= P.n) continue;
+        T sum = T(0);
+        for (uint64_t g = g0; g <= g1; ++g) sum += P.partial[sk_slot(sk, g, t) * (BM * BN) + e];
+        T *cp = Cb + i * P.c_sm + j * P.c_sn;
         T val = P.alpha * sum;
         if (P.beta != T(0)) val += P.beta * *cp;
         *cp = val;
@@ -171,9 +186,23 @@ __global__ __launch_bounds__(256) void scale_2d_kernel(GemmProblem<T> P) {
 namespace {
 
 template <typename T, int BM, int BN, int BK>
-void launch_dense(GemmProblem<T> const &P, dim3 grid, hipStream_t stream) {
+int launch_dense(GemmProblem<T> &P, hipStream_t stream) {
     bool const a_i_fast = (P.a_sm == 1) || (P.a_sk != 1 && P.a_sm <= P.a_sk);
     bool const b_j_fast = (P.b_sn == 1) || (P.b_sk != 1 && P.b_sn <= P.b_sk);
+    P.tiles_m      = uint32_t(ceil_div(P.m, BM));
+    P.tiles_n      = uint32_t(ceil_div(P.n, BN));
+    uint64_t tiles = uint64_t(P.tiles_m) * P.tiles_n * uint64_t(P.batch);
+    uint64_t slabs = uint64_t(ceil_div(P.k, BK));
+    if (tiles > 0x7fffffffu) return fail(EIN_ERROR_UNSUPPORTED, "gemm tile grid too large");
+    uint64_t slots = P.force_dp ? tiles : uint64_t(num_cus()) * 2;
+    P.sk           = plan_stream_k(tiles, slabs, slots);
+    P.partial      = nullptr;
+    if (P.sk.units != 0) {
+        void *ws = nullptr;
+        EIN_TRY(workspace(stream, size_t(P.sk.grid) * 2 * BM * BN * sizeof(T), &ws));
+        P.partial = static_cast<T *>(ws);
+    }
+    dim3 grid{P.sk.grid};
     if (a_i_fast && b_j_fast)
         gemm_dense_kernel<T, BM, BN, BK, true, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
     else if (a_i_fast && !b_j_fast)
@@ -182,32 +211,29 @@ void launch_dense(GemmProblem<T> const &P, dim3 grid, hipStream_t stream) {
         gemm_dense_kernel<T, BM, BN, BK, false, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
     else
         gemm_dense_kernel<T, BM, BN, BK, false, false><<<grid, GEMM_THREADS, 0, stream>>>(P);
+    EIN_LAUNCH_CHECK();
+    if (P.sk.units != 0) {
+        gemm_streamk_fixup<T, BM, BN><<<uint32_t(tiles), 256, 0, stream>>>(P);
+        EIN_LAUNCH_CHECK();
+    }
+    return EIN_SUCCESS;
 }
 
 } // namespace
 
 GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, int elem_bytes) {
     (void)elem_bytes;
+    (void)k;
     GemmConfig cfg;
-    int const cus  = num_cus();
-    int64_t t128   = ceil_div(m, 128) * ceil_div(n, 128) * batch;
-    int64_t t64    = ceil_div(m, 64) * ceil_div(n, 64) * batch;
-    // Two workgroups are resident per CU; aim for at least ~1.5 waves of workgroups.
-    int64_t target = int64_t(cus) * 3;
-    if (t128 >= target || (m >= 128 && n >= 128 && t128 >= int64_t(cus) * 2)) {
+    int64_t const slots = int64_t(num_cus()) * 2;
+    int64_t const t128  = ceil_div(m, 128) * ceil_div(n, 128) * batch;
+    // big tiles when they fill the machine; stream-K spreads the remainder
+    if (m >= 128 && n >= 128 && t128 >= slots) {
         cfg.bm = 128;
         cfg.bn = 128;
     } else {
         cfg.bm = 64;
         cfg.bn = 64;
-    }
-    int64_t tiles = cfg.bm == 128 ? t128 : t64;
-    cfg.splits    = 1;
-    if (tiles < target && k >= 512) {
-        int64_t s = ceil_div(target, tiles);
-        s         = std::min<int64_t>(s, k / 256);
-        s         = std::max<int64_t>(s, 1);
-        cfg.splits = int(std::min<int64_t>(s, 64));
     }
     return cfg;
 }
@@ -220,13 +246,13 @@ int gemm_strided(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream) 
     bool const c_row_fast = (P.c_sn == 1 && P.c_sm != 1) || (P.c_sm != 1 && P.c_sn != 1 && P.c_sn < P.c_sm);
     if (c_row_fast) {
         GemmProblem<T> Q = P;
-        Q.m = P.n;
-        Q.n = P.m;
-        Q.A = P.B;
+        Q.m    = P.n;
+        Q.n    = P.m;
+        Q.A    = P.B;
         Q.a_sm = P.b_sn;
         Q.a_sk = P.b_sk;
         Q.a_sb = P.b_sb;
-        Q.B = P.A;
+        Q.B    = P.A;
         Q.b_sk = P.a_sk;
         Q.b_sn = P.a_sm;
         Q.b_sb = P.a_sb;
@@ -241,39 +267,13 @@ int gemm_strided(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream) 
         EIN_LAUNCH_CHECK();
         return EIN_SUCCESS;
     }
-    if (P.batch > 65535) return fail(EIN_ERROR_UNSUPPORTED, "gemm batch count above 65535");
     GemmConfig cfg = force ? *force : choose_gemm_config(P.m, P.n, P.k, P.batch, int(sizeof(T)));
+    P.force_dp     = cfg.data_parallel;
     constexpr int BK = 16;
-    P.tiles_m        = uint32_t(ceil_div(P.m, cfg.bm));
-    P.tiles_n        = uint32_t(ceil_div(P.n, cfg.bn));
-    P.splits         = std::max(cfg.splits, 1);
-    P.k_chunk        = ceil_div(ceil_div(P.k, P.splits), BK) * BK;
-    P.splits         = int(ceil_div(P.k, P.k_chunk));
-    P.partial        = nullptr;
-    if (P.splits > 1) {
-        void *ws     = nullptr;
-        size_t bytes = size_t(P.splits) * size_t(P.batch) * size_t(P.m) * size_t(P.n) * sizeof(T);
-        EIN_TRY(workspace(stream, bytes, &ws));
-        P.partial = static_cast<T *>(ws);
-    }
-    int64_t ntiles = int64_t(P.tiles_m) * P.tiles_n;
-    if (ntiles > 0x7fffffff) return fail(EIN_ERROR_UNSUPPORTED, "gemm tile grid too large");
-    dim3 grid{uint32_t(ntiles), uint32_t(P.splits), uint32_t(P.batch)};
-    if (cfg.bm == 128 && cfg.bn == 128)
-        launch_dense<T, 128, 128, BK>(P, grid, stream);
-    else if (cfg.bm == 128 && cfg.bn == 64)
-        launch_dense<T, 128, 64, BK>(P, grid, stream);
-    else if (cfg.bm == 64 && cfg.bn == 128)
-        launch_dense<T, 64, 128, BK>(P, grid, stream);
-    else
-        launch_dense<T, 64, 64, BK>(P, grid, stream);
-    EIN_LAUNCH_CHECK();
-    if (P.splits > 1) {
-        int blocks = int(std::min<int64_t>(ceil_div(total, 256), 65536));
-        splitk_reduce_kernel<T><<<blocks, 256, 0, stream>>>(P);
-        EIN_LAUNCH_CHECK();
-    }
-    return EIN_SUCCESS;
+    if (cfg.bm == 128 && cfg.bn == 128) return launch_dense<T, 128, 128, BK>(P, stream);
+    if (cfg.bm == 128 && cfg.bn == 64) return launch_dense<T, 128, 64, BK>(P, stream);
+    if (cfg.bm == 64 && cfg.bn == 128) return launch_dense<T, 64, 128, BK>(P, stream);
+    return launch_dense<T, 64, 64, BK>(P, stream);
 }
 
 template int gemm_strided<double>(GemmProblem<double>, GemmConfig const *, hipStream_t);

@@ -17,15 +17,15 @@ struct GemmProblem {
     T alpha, beta;
     // filled by the launcher
     uint32_t tiles_m, tiles_n;
-    int32_t splits;
-    int64_t k_chunk;
+    StreamK sk;
+    int32_t force_dp;
     T *partial;
 };
 
 struct GemmConfig {
-    int bm     = 128;
-    int bn     = 128;
-    int splits = 1;
+    int bm            = 128;
+    int bn            = 128;
+    int data_parallel = 0; // 1: one tile per workgroup even where stream-K would balance better
 };
 
 GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, int elem_bytes);

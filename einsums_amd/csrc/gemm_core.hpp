@@ -101,7 +101,7 @@ __device__ __forceinline__ void mfma_slab(T const *__restrict__ As, T const *__r
 
 // Grouped (L2-friendly) ordering of output tiles: walks GROUP rows of tiles
 // column by column so neighbouring workgroups share A rows and B columns.
-__device__ __forceinline__ void tile_coords(uint32_t logical, uint32_t tiles_m, uint32_t tiles_n, uint32_t &tm,
+__host__ __device__ __forceinline__ void tile_coords(uint32_t logical, uint32_t tiles_m, uint32_t tiles_n, uint32_t &tm,
                                             uint32_t &tn) {
     constexpr uint32_t GROUP = 8;
     uint32_t per_group = GROUP * tiles_n;
@@ -111,6 +111,65 @@ __device__ __forceinline__ void tile_coords(uint32_t logical, uint32_t tiles_m, 
     uint32_t in_g      = logical - g * per_group;
     tm                 = first_m + in_g % gsize;
     tn                 = in_g / gsize;
+}
+
+// ------------------------------------------------------- stream-K schedule
+// The work of a GEMM-like launch is (tile, BK-slab) units, tile-major. In data-parallel
+// mode every workgroup owns one whole tile. In stream-K mode workgroup w owns the
+// contiguous unit range [w*U, min(W, (w+1)*U)): whole tiles inside it are finished in
+// place, and the (at most two) tiles cut by its range ends are written as partial
+// accumulators to piece slots 2w (first tile it touches) and 2w+1 (last tile); a fix-up
+// pass sums the pieces of every cut tile in workgroup order, so results are
+// deterministic and no atomics are used.
+struct StreamK {
+    uint32_t tiles;  // tiles in the launch (batch included)
+    uint32_t slabs;  // BK-slabs per tile
+    uint64_t units;  // units per workgroup; 0 = data-parallel
+    uint32_t grid;   // workgroups launched
+};
+
+// Decides between data-parallel and stream-K for `tiles` tiles of `slabs` slabs with
+// `slots` workgroups resident at once.
+inline StreamK plan_stream_k(uint64_t tiles, uint64_t slabs, uint64_t slots) {
+    StreamK sk{};
+    sk.tiles = uint32_t(tiles);
+    sk.slabs = uint32_t(slabs);
+    uint64_t rounds = (tiles + slots - 1) / slots;
+    double eff      = double(tiles) / double(rounds * slots);
+    if (tiles == 0 || slabs == 0 || eff >= 0.97 || tiles * slabs < 2 * slots) {
+        sk.units = 0;
+        sk.grid  = uint32_t(tiles);
+        return sk;
+    }
+    uint64_t W = tiles * slabs;
+    uint64_t G = slots < W ? slots : W;
+    sk.units   = (W + G - 1) / G;
+    sk.grid    = uint32_t((W + sk.units - 1) / sk.units);
+    return sk;
+}
+
+// Unit range of launched workgroup `block` (remapped so that neighbouring ranges share
+// an XCD).
+__host__ __device__ __forceinline__ void sk_range_of(StreamK const &sk, uint32_t block, uint64_t &u0, uint64_t &u1) {
+    uint64_t const W = uint64_t(sk.tiles) * sk.slabs;
+    uint64_t const w = xcd_remap(block, sk.grid);
+    if (sk.units == 0) {
+        u0 = w * sk.slabs;
+        u1 = u0 + sk.slabs;
+    } else {
+        u0 = w * sk.units;
+        u1 = u0 + sk.units < W ? u0 + sk.units : W;
+    }
+}
+
+__device__ __forceinline__ void sk_range(StreamK const &sk, uint64_t &u0, uint64_t &u1) {
+    sk_range_of(sk, blockIdx.x, u0, u1);
+}
+
+// Piece slot of tile t for workgroup w (w being the remapped index).
+__host__ __device__ __forceinline__ uint64_t sk_slot(StreamK const &sk, uint64_t w, uint64_t t) {
+    uint64_t first_tile = (w * sk.units) / sk.slabs;
+    return 2 * w + (t == first_tile ? 0 : 1);
 }
 
 } // namespace ein

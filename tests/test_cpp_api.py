@@ -24,6 +24,12 @@ def test_compile_time_dispatch_program():
     assert res.returncode == 0 and "passed" in res.stdout
 
 
+def test_streamk_schedule_program():
+    """Host check of the stream-K unit ownership, piece slots and fix-up coverage."""
+    res = _run("test_streamk_schedule", 120)
+    assert res.returncode == 0 and "0 failures" in res.stdout
+
+
 @pytest.mark.gpu
 def test_device_einsum_program():
     res = _run("test_device_einsum", 120)
