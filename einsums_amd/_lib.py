@@ -55,6 +55,14 @@ class TensorDesc(ctypes.Structure):
     ]
 
 
+class ContractionItem(ctypes.Structure):
+    _fields_ = [
+        ("C", TensorDesc), ("A", TensorDesc), ("B", TensorDesc),
+        ("c", ctypes.c_void_p), ("a", ctypes.c_void_p), ("b", ctypes.c_void_p),
+        ("c_prefactor", ctypes.c_double), ("ab_prefactor", ctypes.c_double),
+    ]
+
+
 def make_desc(labels, dims, strides) -> TensorDesc:
     d = TensorDesc()
     r = len(dims)
@@ -122,12 +130,21 @@ def _load():
         sig[f"ein_contract_{sfx}"] = (_c_int, [T, _desc_p, _dp, T, _desc_p, _dp, _desc_p, _dp, _vp])
         sig[f"ein_einsum_{sfx}"] = (_c_int, [T, _desc_p, _dp, T, _desc_p, _dp, _desc_p, _dp,
                                              ctypes.POINTER(ctypes.c_int32), _vp])
+        sig[f"ein_einsum_adjoint_{sfx}"] = (_c_int, [ctypes.c_int32, T, _desc_p, _dp, _desc_p, _dp, _desc_p, _dp, T,
+                                                     _desc_p, _dp, ctypes.POINTER(ctypes.c_int32), _vp])
         sig[f"ein_einsum_execute_{sfx}"] = (_c_int, [ctypes.c_int32, T, _desc_p, _dp, T, _desc_p, _dp, _desc_p, _dp,
                                                      ctypes.POINTER(ctypes.c_int32), _vp])
     sig["ein_gemm_split_f64"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64, _dp,
                                           _i64, ctypes.c_double, _dp, _i64, ctypes.c_int32, ctypes.c_int32, _vp])
     sig["ein_einsum_split_f64"] = (_c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _desc_p, _dp, ctypes.c_double,
                                             _desc_p, _dp, _desc_p, _dp, ctypes.POINTER(ctypes.c_int32), _vp])
+    sig["ein_grouped_plan_create"] = (_c_int, [ctypes.c_int32, ctypes.POINTER(ContractionItem), ctypes.c_int32,
+                                               ctypes.POINTER(ctypes.c_void_p)])
+    sig["ein_grouped_plan_execute"] = (_c_int, [_vp, _vp])
+    sig["ein_grouped_plan_autotune"] = (_c_int, [_vp, _vp])
+    sig["ein_grouped_plan_info"] = (_c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                             ctypes.POINTER(ctypes.c_int64)])
+    sig["ein_grouped_plan_destroy"] = (_c_int, [_vp])
     sig["ein_einsum_classify"] = (_c_int, [_desc_p, _desc_p, _desc_p, ctypes.POINTER(ctypes.c_int32)])
     for name, (res, args) in sig.items():
         _proto(lib, name, res, args)

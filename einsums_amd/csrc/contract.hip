@@ -137,18 +137,18 @@ int fill_group(IndexGroup &G, std::vector<Digit> const &v, int nops) {
 // Marks a row/column outside the problem (offsets themselves may be negative).
 constexpr int64_t NO_ROW = INT64_MIN;
 
+// One tile (or the slab range [s0, s1) of it) of a gather-GEMM problem. `As`/`Bs` are
+// the workgroup's double-buffered LDS slabs; with `piece` != nullptr the raw accumulators
+// go there ([BN][BM], i fastest) instead of through the epilogue into C.
 template <typename T, int BM, int BN, int BK, bool A_I_FAST, bool B_J_FAST>
-__global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProblem<T> P) {
-    using Cfg = TileCfg<T, BM, BN, BK>;
-    __shared__ T As[2][BK * Cfg::LDA];
-    __shared__ T Bs[2][BK * Cfg::LDB];
-
-    int const tid  = threadIdx.x;
-    int const lane = tid & 63;
-    int const wave = tid >> 6;
-    int const wm   = wave & 1;
-    int const wn   = wave >> 1;
-
+__device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64_t t, uint64_t s0, uint64_t s1, T *As,
+                                               T *Bs, T *piece) {
+    using Cfg            = TileCfg<T, BM, BN, BK>;
+    int const tid        = threadIdx.x;
+    int const lane       = tid & 63;
+    int const wave       = tid >> 6;
+    int const wm         = wave & 1;
+    int const wn         = wave >> 1;
     constexpr int A_STEP = A_I_FAST ? GEMM_THREADS / BM : GEMM_THREADS / BK;
     int const a_ii       = A_I_FAST ? tid % BM : tid / BK;
     int const a_pp       = A_I_FAST ? tid / BM : tid % BK;
@@ -157,163 +157,169 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProb
     int const b_pp       = B_J_FAST ? tid / BN : tid % BK;
     int const r          = lane & 15;
     int const q          = lane >> 4;
-
-    StreamK const &sk       = P.sk;
-    uint64_t const S        = sk.slabs;
     uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
+
+    uint32_t const bz = uint32_t(t / tiles_mn);
+    uint32_t tm, tn;
+    tile_coords(uint32_t(t % tiles_mn), P.tiles_m, P.tiles_n, tm, tn);
+    int64_t const i0   = int64_t(tm) * BM;
+    int64_t const j0   = int64_t(tn) * BN;
+    int64_t const kbeg = int64_t(s0) * BK;
+    int64_t const kend = int64_t(s1) * BK < P.k ? int64_t(s1) * BK : P.k;
+
+    int64_t boff[3];
+    group_decode<3>(bz, P.gb, boff);
+    T const *__restrict__ Ab = P.A + boff[0];
+    T const *__restrict__ Bb = P.B + boff[1];
+
+    // Row / column offsets fixed for the whole tile (NO_ROW marks out of range).
+    constexpr int NRA = A_I_FAST ? 1 : Cfg::LA;
+    constexpr int NRB = B_J_FAST ? 1 : Cfg::LB;
+    int64_t rowA[NRA], colB[NRB];
+#pragma unroll
+    for (int rr = 0; rr < NRA; ++rr) {
+        int64_t i = i0 + (A_I_FAST ? a_ii : a_ii + rr * A_STEP);
+        int64_t o[2];
+        group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
+        rowA[rr] = i < P.m ? o[0] : NO_ROW;
+    }
+#pragma unroll
+    for (int rr = 0; rr < NRB; ++rr) {
+        int64_t j = j0 + (B_J_FAST ? b_jj : b_jj + rr * B_STEP);
+        int64_t o[2];
+        group_decode<2>(uint32_t(j < P.n ? j : 0), P.gn, o);
+        colB[rr] = j < P.n ? o[0] : NO_ROW;
+    }
+
+    T ra[Cfg::LA], rb[Cfg::LB];
+    auto load_slab = [&](int64_t k0) {
+        // k offsets for the p-fast operands: one decode serves both A and B when both
+        // walk p with the same lane mapping
+        int64_t kfix[2] = {0, 0};
+        bool kfix_ok    = false;
+        if (!A_I_FAST || !B_J_FAST) {
+            int64_t p = k0 + (A_I_FAST ? b_pp : a_pp);
+            kfix_ok   = p < kend;
+            group_decode<2>(uint32_t(kfix_ok ? p : 0), P.gk, kfix);
+        }
+#pragma unroll
+        for (int rr = 0; rr < Cfg::LA; ++rr) {
+            T v = T(0);
+            if constexpr (A_I_FAST) {
+                int64_t p = k0 + a_pp + rr * A_STEP;
+                if (rowA[0] != NO_ROW && p < kend) {
+                    int64_t o[2];
+                    group_decode<2>(uint32_t(p), P.gk, o);
+                    v = Ab[rowA[0] + o[0]];
+                }
+            } else {
+                if (rowA[rr] != NO_ROW && kfix_ok) v = Ab[rowA[rr] + kfix[0]];
+            }
+            ra[rr] = v;
+        }
+#pragma unroll
+        for (int rr = 0; rr < Cfg::LB; ++rr) {
+            T v = T(0);
+            if constexpr (B_J_FAST) {
+                int64_t p = k0 + b_pp + rr * B_STEP;
+                if (colB[0] != NO_ROW && p < kend) {
+                    int64_t o[2];
+                    group_decode<2>(uint32_t(p), P.gk, o);
+                    v = Bb[colB[0] + o[1]];
+                }
+            } else {
+                if (colB[rr] != NO_ROW && kfix_ok) v = Bb[colB[rr] + kfix[1]];
+            }
+            rb[rr] = v;
+        }
+    };
+    auto store_slab = [&](int buf) {
+#pragma unroll
+        for (int rr = 0; rr < Cfg::LA; ++rr) {
+            int ii = A_I_FAST ? a_ii : a_ii + rr * A_STEP;
+            int pp = A_I_FAST ? a_pp + rr * A_STEP : a_pp;
+            As[buf * (BK * Cfg::LDA) + pp * Cfg::LDA + ii] = ra[rr];
+        }
+#pragma unroll
+        for (int rr = 0; rr < Cfg::LB; ++rr) {
+            int jj = B_J_FAST ? b_jj : b_jj + rr * B_STEP;
+            int pp = B_J_FAST ? b_pp + rr * B_STEP : b_pp;
+            Bs[buf * (BK * Cfg::LDB) + pp * Cfg::LDB + jj] = rb[rr];
+        }
+    };
+
+    WaveAcc<T, Cfg::FM, Cfg::FN> acc;
+    acc.zero();
+    int64_t const nslabs = int64_t(s1 - s0);
+    load_slab(kbeg);
+    store_slab(0);
+    __syncthreads();
+    for (int64_t sl = 0; sl < nslabs; ++sl) {
+        int const buf = int(sl & 1);
+        if (sl + 1 < nslabs) load_slab(kbeg + (sl + 1) * BK);
+        mfma_slab<T, BM, BN, BK>(As + buf * (BK * Cfg::LDA), Bs + buf * (BK * Cfg::LDB), acc, wm, wn, lane);
+        if (sl + 1 < nslabs) store_slab(buf ^ 1);
+        __syncthreads();
+    }
+
+    if (piece != nullptr) {
+        #pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+            for (int b = 0; b < Cfg::FN; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    int ii = wm * Cfg::WM + a * 16 + r;
+                    int jj = wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
+                    piece[jj * BM + ii] = acc.c[a][b][v];
+                }
+        return;
+    }
+    T *__restrict__ Cb = P.C + boff[2];
+    int64_t rowC[Cfg::FM];
+#pragma unroll
+    for (int a = 0; a < Cfg::FM; ++a) {
+        int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
+        int64_t o[2];
+        group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
+        rowC[a] = i < P.m ? o[1] : NO_ROW;
+    }
+#pragma unroll
+    for (int b = 0; b < Cfg::FN; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
+            if (j >= P.n) continue;
+            int64_t o[2];
+            group_decode<2>(uint32_t(j), P.gn, o);
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a) {
+                if (rowC[a] == NO_ROW) continue;
+                T *cp = Cb + rowC[a] + o[1];
+                T val = P.alpha * acc.c[a][b][v];
+                if (P.beta != T(0)) val += P.beta * *cp;
+                *cp = val;
+            }
+        }
+}
+
+template <typename T, int BM, int BN, int BK, bool A_I_FAST, bool B_J_FAST>
+__global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProblem<T> P) {
+    using Cfg = TileCfg<T, BM, BN, BK>;
+    __shared__ T As[2 * BK * Cfg::LDA];
+    __shared__ T Bs[2 * BK * Cfg::LDB];
+    StreamK const &sk = P.sk;
+    uint64_t const S  = sk.slabs;
     uint64_t u0, u1;
     sk_range(sk, u0, u1);
     uint64_t const wlog = sk.units ? u0 / sk.units : 0;
-
     for (uint64_t u = u0; u < u1;) {
         uint64_t const t  = u / S;
         uint64_t const s0 = u - t * S;
         uint64_t const s1 = (S - s0 < u1 - u) ? S : s0 + (u1 - u);
         u += s1 - s0;
-
-        uint32_t const bz = uint32_t(t / tiles_mn);
-        uint32_t tm, tn;
-        tile_coords(uint32_t(t % tiles_mn), P.tiles_m, P.tiles_n, tm, tn);
-        int64_t const i0   = int64_t(tm) * BM;
-        int64_t const j0   = int64_t(tn) * BN;
-        int64_t const kbeg = int64_t(s0) * BK;
-        int64_t const kend = int64_t(s1) * BK < P.k ? int64_t(s1) * BK : P.k;
-
-        int64_t boff[3];
-        group_decode<3>(bz, P.gb, boff);
-        T const *__restrict__ Ab = P.A + boff[0];
-        T const *__restrict__ Bb = P.B + boff[1];
-
-        // Row / column offsets fixed for the whole tile (NO_ROW marks out of range).
-        constexpr int NRA = A_I_FAST ? 1 : Cfg::LA;
-        constexpr int NRB = B_J_FAST ? 1 : Cfg::LB;
-        int64_t rowA[NRA], colB[NRB];
-#pragma unroll
-        for (int rr = 0; rr < NRA; ++rr) {
-            int64_t i = i0 + (A_I_FAST ? a_ii : a_ii + rr * A_STEP);
-            int64_t o[2];
-            group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
-            rowA[rr] = i < P.m ? o[0] : NO_ROW;
-        }
-#pragma unroll
-        for (int rr = 0; rr < NRB; ++rr) {
-            int64_t j = j0 + (B_J_FAST ? b_jj : b_jj + rr * B_STEP);
-            int64_t o[2];
-            group_decode<2>(uint32_t(j < P.n ? j : 0), P.gn, o);
-            colB[rr] = j < P.n ? o[0] : NO_ROW;
-        }
-
-        T ra[Cfg::LA], rb[Cfg::LB];
-        auto load_slab = [&](int64_t k0) {
-            // k offsets for the p-fast operands: one decode serves both A and B when both
-            // walk p with the same lane mapping
-            int64_t kfix[2] = {0, 0};
-            bool kfix_ok    = false;
-            if (!A_I_FAST || !B_J_FAST) {
-                int64_t p = k0 + (A_I_FAST ? b_pp : a_pp);
-                kfix_ok   = p < kend;
-                group_decode<2>(uint32_t(kfix_ok ? p : 0), P.gk, kfix);
-            }
-#pragma unroll
-            for (int rr = 0; rr < Cfg::LA; ++rr) {
-                T v = T(0);
-                if constexpr (A_I_FAST) {
-                    int64_t p = k0 + a_pp + rr * A_STEP;
-                    if (rowA[0] != NO_ROW && p < kend) {
-                        int64_t o[2];
-                        group_decode<2>(uint32_t(p), P.gk, o);
-                        v = Ab[rowA[0] + o[0]];
-                    }
-                } else {
-                    if (rowA[rr] != NO_ROW && kfix_ok) v = Ab[rowA[rr] + kfix[0]];
-                }
-                ra[rr] = v;
-            }
-#pragma unroll
-            for (int rr = 0; rr < Cfg::LB; ++rr) {
-                T v = T(0);
-                if constexpr (B_J_FAST) {
-                    int64_t p = k0 + b_pp + rr * B_STEP;
-                    if (colB[0] != NO_ROW && p < kend) {
-                        int64_t o[2];
-                        group_decode<2>(uint32_t(p), P.gk, o);
-                        v = Bb[colB[0] + o[1]];
-                    }
-                } else {
-                    if (colB[rr] != NO_ROW && kfix_ok) v = Bb[colB[rr] + kfix[1]];
-                }
-                rb[rr] = v;
-            }
-        };
-        auto store_slab = [&](int buf) {
-#pragma unroll
-            for (int rr = 0; rr < Cfg::LA; ++rr) {
-                int ii = A_I_FAST ? a_ii : a_ii + rr * A_STEP;
-                int pp = A_I_FAST ? a_pp + rr * A_STEP : a_pp;
-                As[buf][pp * Cfg::LDA + ii] = ra[rr];
-            }
-#pragma unroll
-            for (int rr = 0; rr < Cfg::LB; ++rr) {
-                int jj = B_J_FAST ? b_jj : b_jj + rr * B_STEP;
-                int pp = B_J_FAST ? b_pp + rr * B_STEP : b_pp;
-                Bs[buf][pp * Cfg::LDB + jj] = rb[rr];
-            }
-        };
-
-        WaveAcc<T, Cfg::FM, Cfg::FN> acc;
-        acc.zero();
-        int64_t const nslabs = int64_t(s1 - s0);
-        load_slab(kbeg);
-        store_slab(0);
-        __syncthreads();
-        for (int64_t sl = 0; sl < nslabs; ++sl) {
-            int const buf = int(sl & 1);
-            if (sl + 1 < nslabs) load_slab(kbeg + (sl + 1) * BK);
-            mfma_slab<T, BM, BN, BK>(As[buf], Bs[buf], acc, wm, wn, lane);
-            if (sl + 1 < nslabs) store_slab(buf ^ 1);
-            __syncthreads();
-        }
-
-        if (!(s0 == 0 && s1 == S)) {
-            T *__restrict__ piece = P.partial + sk_slot(sk, wlog, t) * (BM * BN);
-#pragma unroll
-            for (int a = 0; a < Cfg::FM; ++a)
-#pragma unroll
-                for (int b = 0; b < Cfg::FN; ++b)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        int ii = wm * Cfg::WM + a * 16 + r;
-                        int jj = wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
-                        piece[jj * BM + ii] = acc.c[a][b][v];
-                    }
-            continue;
-        }
-        T *__restrict__ Cb = P.C + boff[2];
-        int64_t rowC[Cfg::FM];
-#pragma unroll
-        for (int a = 0; a < Cfg::FM; ++a) {
-            int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
-            int64_t o[2];
-            group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
-            rowC[a] = i < P.m ? o[1] : NO_ROW;
-        }
-#pragma unroll
-        for (int b = 0; b < Cfg::FN; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
-                if (j >= P.n) continue;
-                int64_t o[2];
-                group_decode<2>(uint32_t(j), P.gn, o);
-#pragma unroll
-                for (int a = 0; a < Cfg::FM; ++a) {
-                    if (rowC[a] == NO_ROW) continue;
-                    T *cp = Cb + rowC[a] + o[1];
-                    T val = P.alpha * acc.c[a][b][v];
-                    if (P.beta != T(0)) val += P.beta * *cp;
-                    *cp = val;
-                }
-            }
+        T *piece = (s0 == 0 && s1 == S) ? nullptr : P.partial + sk_slot(sk, wlog, t) * (BM * BN);
+        gather_segment<T, BM, BN, BK, A_I_FAST, B_J_FAST>(P, t, s0, s1, As, Bs, piece);
     }
 }
 
@@ -675,6 +681,400 @@ int ein_contract_f32(float c_prefactor, const ein_tensor_desc *C, float *c, floa
     if (A == nullptr) return -5;
     if (B == nullptr) return -7;
     return ein::contract_generic<float>(c_prefactor, *C, c, ab_prefactor, *A, a, *B, b, ein::as_stream(stream));
+}
+
+} // extern "C"
+
+// ===================================================== grouped launches
+//
+// A grouped plan holds many independent contractions (the blocks of a block-sparse
+// einsum, a batch of uneven GEMMs). Each becomes a gather-GEMM problem; all their tiles
+// form one list sorted by contraction length (longest first) and a persistent launch of
+// about two workgroups per CU walks the list with a fixed stride -- one launch per wave
+// of items whose outputs do not overlap. Descriptors live in device memory, so executing
+// a plan is a pure kernel launch and can be captured into a hipGraph.
+
+namespace ein {
+
+template <typename T>
+struct GroupedItem {
+    GatherProblem<T> P;
+    int32_t flags; // bit 0: A walks rows (i-fast), bit 1: B walks columns (j-fast)
+    int32_t pad;
+};
+
+struct GroupedTile {
+    uint32_t item;
+    uint32_t tile;
+};
+
+template <typename T, int BM>
+__global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_grouped_kernel(GroupedItem<T> const *__restrict__ items,
+                                                                       GroupedTile const *__restrict__ tiles,
+                                                                       uint32_t ntiles) {
+    constexpr int BN = 64, BK = 16;
+    using Cfg = TileCfg<T, BM, BN, BK>;
+    __shared__ T As[2 * BK * Cfg::LDA];
+    __shared__ T Bs[2 * BK * Cfg::LDB];
+    for (uint32_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
+        GroupedTile const e           = tiles[tt];
+        GroupedItem<T> const &it      = items[e.item];
+        uint64_t const S              = uint64_t((it.P.k + BK - 1) / BK);
+        switch (it.flags) {
+        case 3: gather_segment<T, BM, BN, BK, true, true>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        case 1: gather_segment<T, BM, BN, BK, true, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        case 2: gather_segment<T, BM, BN, BK, false, true>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        default: gather_segment<T, BM, BN, BK, false, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        }
+    }
+}
+
+namespace {
+
+struct PreScale {
+    ein_tensor_desc C;
+    void *c;
+    double factor;
+};
+
+struct GroupedWave {
+    std::vector<PreScale> prescale;        // C scales that must precede this wave's tiles
+    void *d_items[2] = {nullptr, nullptr}; // per tile height (64, 128)
+    void *d_tiles[2] = {nullptr, nullptr};
+    uint32_t ntiles[2] = {0, 0};
+};
+
+struct GroupedPlan {
+    int elem = 8;
+    int bm_index = 0; // 0: 64-row tiles, 1: 128-row tiles
+    bool tuned = false;
+    std::vector<GroupedWave> waves;
+    std::vector<ein_contraction_item> items; // host copy (for the autotuner)
+    std::vector<int> wave_of;                // wave of each kept item
+    int64_t tiles_total[2] = {0, 0};
+};
+
+int64_t span_of(ein_tensor_desc const &d) {
+    int64_t hi = 0;
+    for (int i = 0; i < d.rank; ++i)
+        if (d.dims[i] > 0) hi += (d.dims[i] - 1) * std::llabs(d.strides[i]);
+    return hi + 1;
+}
+
+// Byte range [lo, hi) a descriptor addresses from base p (strides of either sign).
+struct ByteRange {
+    uintptr_t lo, hi;
+    bool overlaps(ByteRange const &o) const { return lo < o.hi && o.lo < hi; }
+};
+
+ByteRange range_of(ein_tensor_desc const &d, void const *p, size_t elem) {
+    int64_t lo = 0, hi = 0;
+    for (int i = 0; i < d.rank; ++i) {
+        if (d.dims[i] == 0) return {0, 0};
+        int64_t reach = (d.dims[i] - 1) * d.strides[i];
+        if (reach < 0)
+            lo += reach;
+        else
+            hi += reach;
+    }
+    auto const base = reinterpret_cast<uintptr_t>(p);
+    return {base + uintptr_t(lo * int64_t(elem)), base + uintptr_t((hi + 1) * int64_t(elem))};
+}
+
+// Analyse one item into a gather problem (MFMA path for every size). kind: 0 = tiles to
+// run, 1 = only a scale of C, 2 = nothing to do.
+template <typename T>
+int build_item(ein_contraction_item const &in, GroupedItem<T> (&out)[2], int &kind, bool &c_repeats) {
+    ContractionAnalysis an;
+    EIN_TRY(analyse_contraction(in.C, in.A, in.B, an));
+    c_repeats = an.c_repeats;
+    kind      = 0;
+    if (an.c_size == 0) {
+        kind = 2;
+        return EIN_SUCCESS;
+    }
+    std::vector<Digit> vb, vm, vn, vk;
+    bool empty_k = false;
+    for (int u = 0; u < an.nlab; ++u) {
+        int64_t e = an.ext[u];
+        if (e == 0) {
+            if (!an.inC[u]) empty_k = true;
+            continue;
+        }
+        if (e == 1) continue;
+        if (an.inC[u] && an.inA[u] && an.inB[u])
+            vb.push_back({e, {an.sA[u], an.sB[u], an.sC[u]}});
+        else if (an.inC[u] && !an.inB[u])
+            vm.push_back({e, {an.sA[u], an.sC[u], 0}});
+        else if (an.inC[u] && an.inB[u])
+            vn.push_back({e, {an.sB[u], an.sC[u], 0}});
+        else
+            vk.push_back({e, {an.sA[u], an.sB[u], 0}});
+    }
+    if (empty_k || in.ab_prefactor == 0.0) {
+        kind = 1;
+        return EIN_SUCCESS;
+    }
+    int64_t const M = total_of(vm), N = total_of(vn), K = total_of(vk), NB = total_of(vb);
+    if (M >= (int64_t{1} << 31) || N >= (int64_t{1} << 31) || K >= (int64_t{1} << 31) || NB >= (int64_t{1} << 31))
+        return fail(EIN_ERROR_UNSUPPORTED, "grouped contraction: index range above 2^31");
+    auto gm = order_and_fold(vm, 2, 0, 1);
+    auto gn = order_and_fold(vn, 2, 0, 1);
+    bool const a_i_fast = !gm.empty() && std::llabs(gm.back().s[0]) == 1;
+    auto gk = a_i_fast ? order_and_fold(vk, 2, 1, 0) : order_and_fold(vk, 2, 0, 1);
+    bool const b_j_fast = (!gn.empty() && std::llabs(gn.back().s[0]) == 1) ||
+                          !(!gk.empty() && std::llabs(gk.back().s[1]) == 1);
+    bool const a_fast = a_i_fast || !(!gk.empty() && std::llabs(gk.back().s[0]) == 1);
+    auto gbv = order_and_fold(vb, 3, 2, 0);
+    GatherProblem<T> P{};
+    EIN_TRY(fill_group(P.gm, gm, 2));
+    EIN_TRY(fill_group(P.gn, gn, 2));
+    EIN_TRY(fill_group(P.gk, gk, 2));
+    EIN_TRY(fill_group(P.gb, gbv, 3));
+    P.m = M;
+    P.n = N;
+    P.k = K;
+    P.batch = NB;
+    P.A = static_cast<T const *>(in.a);
+    P.B = static_cast<T const *>(in.b);
+    P.C = static_cast<T *>(in.c);
+    P.alpha = T(in.ab_prefactor);
+    P.beta = c_repeats ? T(1) : T(in.c_prefactor);
+    P.partial = nullptr;
+    int const bms[2] = {64, 128};
+    for (int v = 0; v < 2; ++v) {
+        out[v].P = P;
+        out[v].P.tiles_m = uint32_t(ceil_div(M, bms[v]));
+        out[v].P.tiles_n = uint32_t(ceil_div(N, 64));
+        out[v].flags = (a_fast ? 1 : 0) | (b_j_fast ? 2 : 0);
+        out[v].pad = 0;
+    }
+    return EIN_SUCCESS;
+}
+
+// The result equals running the items one after another in list order. Item j depends on
+// an earlier item i when j's output overlaps i's output or inputs, or j's inputs overlap
+// i's output; j then runs in a wave after i's. Each item goes to the wave after the last
+// one holding an item it depends on, so accumulations into one output keep list order and
+// a wave's items never touch each other's outputs. Scales of C (c_prefactor with repeated
+// C labels, or an item that only scales) run at the start of the item's own wave.
+template <typename T>
+int plan_create(ein_contraction_item const *items, int count, GroupedPlan &plan,
+                std::vector<int> const *forced_waves = nullptr) {
+    plan.elem = int(sizeof(T));
+    std::vector<std::vector<GroupedItem<T>>> wave_items[2];
+    struct Placed {
+        ByteRange out, in_a, in_b;
+        int wave;
+    };
+    std::vector<Placed> placed;
+    size_t const eb = sizeof(T);
+    for (int i = 0; i < count; ++i) {
+        GroupedItem<T> gi[2];
+        int kind = 0;
+        bool rep = false;
+        EIN_TRY(build_item<T>(items[i], gi, kind, rep));
+        if (kind == 2) continue;
+        Placed me{range_of(items[i].C, items[i].c, eb), range_of(items[i].A, items[i].a, eb),
+                  range_of(items[i].B, items[i].b, eb), 0};
+        if (forced_waves != nullptr) {
+            me.wave = (*forced_waves)[plan.items.size()];
+        } else {
+            for (auto const &p : placed)
+                if (p.wave >= me.wave &&
+                    (me.out.overlaps(p.out) || me.out.overlaps(p.in_a) || me.out.overlaps(p.in_b) ||
+                     (kind == 0 && (me.in_a.overlaps(p.out) || me.in_b.overlaps(p.out)))))
+                    me.wave = p.wave + 1;
+        }
+        size_t const w = size_t(me.wave);
+        while (w >= plan.waves.size()) {
+            plan.waves.emplace_back();
+            wave_items[0].emplace_back();
+            wave_items[1].emplace_back();
+        }
+        placed.push_back(me);
+        plan.items.push_back(items[i]);
+        plan.wave_of.push_back(me.wave);
+        if (kind == 1 || rep) plan.waves[w].prescale.push_back({items[i].C, items[i].c, items[i].c_prefactor});
+        if (kind == 1) continue;
+        wave_items[0][w].push_back(gi[0]);
+        wave_items[1][w].push_back(gi[1]);
+    }
+    for (size_t w = 0; w < plan.waves.size(); ++w) {
+        for (int v = 0; v < 2; ++v) {
+            auto const &its = wave_items[v][w];
+            std::vector<GroupedTile> tl;
+            std::vector<int64_t> cost;
+            for (size_t q = 0; q < its.size(); ++q) {
+                uint64_t nt = uint64_t(its[q].P.tiles_m) * its[q].P.tiles_n * uint64_t(its[q].P.batch);
+                if (nt > 0xffffffffu) return fail(EIN_ERROR_UNSUPPORTED, "grouped contraction: too many tiles");
+                for (uint64_t t = 0; t < nt; ++t) {
+                    tl.push_back({uint32_t(q), uint32_t(t)});
+                    cost.push_back(its[q].P.k);
+                }
+            }
+            // longest contraction first (LPT over a fixed-stride persistent walk)
+            std::vector<size_t> order(tl.size());
+            for (size_t z = 0; z < order.size(); ++z) order[z] = z;
+            std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+            std::vector<GroupedTile> sorted(tl.size());
+            for (size_t z = 0; z < order.size(); ++z) sorted[z] = tl[order[z]];
+            size_t ib = its.size() * sizeof(GroupedItem<T>), tb = sorted.size() * sizeof(GroupedTile);
+            EIN_TRY(ein_malloc(&plan.waves[w].d_items[v], std::max<size_t>(ib, 256)));
+            EIN_TRY(ein_malloc(&plan.waves[w].d_tiles[v], std::max<size_t>(tb, 256)));
+            if (ib) EIN_HIP_CHECK(hipMemcpy(plan.waves[w].d_items[v], its.data(), ib, hipMemcpyHostToDevice));
+            if (tb) EIN_HIP_CHECK(hipMemcpy(plan.waves[w].d_tiles[v], sorted.data(), tb, hipMemcpyHostToDevice));
+            plan.waves[w].ntiles[v] = uint32_t(sorted.size());
+            plan.tiles_total[v] += int64_t(sorted.size());
+        }
+    }
+    // default tile height: 128 rows only when it still gives >= 2 tiles per resident slot
+    plan.bm_index = plan.tiles_total[1] >= int64_t(num_cus()) * 4 ? 1 : 0;
+    return EIN_SUCCESS;
+}
+
+template <typename T>
+int plan_launch(GroupedPlan const &plan, int bm_index, hipStream_t stream) {
+    uint32_t const slots = uint32_t(num_cus()) * 2;
+    for (auto const &w : plan.waves) {
+        for (auto const &ps : w.prescale)
+            EIN_TRY(tensor_scale<T>(ps.C, static_cast<T *>(ps.c), T(ps.factor), stream));
+        uint32_t nt = w.ntiles[bm_index];
+        if (nt == 0) continue;
+        uint32_t grid = nt < slots ? nt : slots;
+        if (bm_index == 1)
+            gemm_grouped_kernel<T, 128><<<grid, GEMM_THREADS, 0, stream>>>(
+                static_cast<GroupedItem<T> const *>(w.d_items[1]), static_cast<GroupedTile const *>(w.d_tiles[1]), nt);
+        else
+            gemm_grouped_kernel<T, 64><<<grid, GEMM_THREADS, 0, stream>>>(
+                static_cast<GroupedItem<T> const *>(w.d_items[0]), static_cast<GroupedTile const *>(w.d_tiles[0]), nt);
+        EIN_LAUNCH_CHECK();
+    }
+    return EIN_SUCCESS;
+}
+
+// Autotuner: times both tile heights on scratch operands laid out like the plan's own (so
+// the user's tensors are untouched) and keeps the faster one.
+template <typename T>
+int plan_autotune(GroupedPlan &plan, hipStream_t stream) {
+    if (plan.items.empty()) return EIN_SUCCESS;
+    int64_t sa = 1, sb = 1, sc = 1;
+    for (auto const &it : plan.items) {
+        sa = std::max(sa, span_of(it.A));
+        sb = std::max(sb, span_of(it.B));
+        sc = std::max(sc, span_of(it.C));
+    }
+    void *pa = nullptr, *pb = nullptr, *pc = nullptr;
+    EIN_TRY(ein_malloc(&pa, size_t(sa) * sizeof(T)));
+    EIN_TRY(ein_malloc(&pb, size_t(sb) * sizeof(T)));
+    EIN_TRY(ein_malloc(&pc, size_t(sc) * sizeof(T)));
+    EIN_HIP_CHECK(hipMemsetAsync(pa, 0, size_t(sa) * sizeof(T), stream));
+    EIN_HIP_CHECK(hipMemsetAsync(pb, 0, size_t(sb) * sizeof(T), stream));
+    EIN_HIP_CHECK(hipMemsetAsync(pc, 0, size_t(sc) * sizeof(T), stream));
+    std::vector<ein_contraction_item> scratch = plan.items;
+    for (auto &it : scratch) {
+        it.a = pa;
+        it.b = pb;
+        it.c = pc;
+    }
+    // the scratch items all write one buffer; keep the plan's own wave structure (the
+    // aliasing only affects the values, which are discarded)
+    GroupedPlan tp;
+    int st = plan_create<T>(scratch.data(), int(scratch.size()), tp, &plan.wave_of);
+    if (st == EIN_SUCCESS) {
+        hipEvent_t e0, e1;
+        EIN_HIP_CHECK(hipEventCreate(&e0));
+        EIN_HIP_CHECK(hipEventCreate(&e1));
+        float best = 0.f;
+        int best_v = plan.bm_index;
+        for (int v = 0; v < 2 && st == EIN_SUCCESS; ++v) {
+            if (tp.tiles_total[v] == 0) continue;
+            st = plan_launch<T>(tp, v, stream); // warm-up
+            EIN_HIP_CHECK(hipEventRecord(e0, stream));
+            for (int r = 0; r < 3 && st == EIN_SUCCESS; ++r) st = plan_launch<T>(tp, v, stream);
+            EIN_HIP_CHECK(hipEventRecord(e1, stream));
+            EIN_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            EIN_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (v == 0 || ms < best) {
+                best   = ms;
+                best_v = v;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        plan.bm_index = best_v;
+        plan.tuned    = true;
+    }
+    for (auto &w : tp.waves)
+        for (int v = 0; v < 2; ++v) {
+            (void)ein_free(w.d_items[v]);
+            (void)ein_free(w.d_tiles[v]);
+        }
+    (void)hipStreamSynchronize(stream);
+    (void)ein_free(pa);
+    (void)ein_free(pb);
+    (void)ein_free(pc);
+    return st;
+}
+
+} // namespace
+} // namespace ein
+
+extern "C" {
+
+int ein_grouped_plan_create(int32_t elem_bytes, const ein_contraction_item *items, int32_t count, void **plan) {
+    if (elem_bytes != 8 && elem_bytes != 4) return ein::fail(-1, "grouped plan: elem_bytes must be 8 or 4");
+    if (count < 0 || (count > 0 && items == nullptr)) return ein::fail(-2, "grouped plan: bad item list");
+    if (plan == nullptr) return -4;
+    auto *p = new ein::GroupedPlan();
+    int st = elem_bytes == 8 ? ein::plan_create<double>(items, count, *p) : ein::plan_create<float>(items, count, *p);
+    if (st != EIN_SUCCESS) {
+        for (auto &w : p->waves)
+            for (int v = 0; v < 2; ++v) {
+                (void)ein_free(w.d_items[v]);
+                (void)ein_free(w.d_tiles[v]);
+            }
+        delete p;
+        return st;
+    }
+    *plan = p;
+    return EIN_SUCCESS;
+}
+
+int ein_grouped_plan_execute(void *plan, ein_stream_t stream) {
+    if (plan == nullptr) return -1;
+    auto *p = static_cast<ein::GroupedPlan *>(plan);
+    return p->elem == 8 ? ein::plan_launch<double>(*p, p->bm_index, ein::as_stream(stream))
+                        : ein::plan_launch<float>(*p, p->bm_index, ein::as_stream(stream));
+}
+
+int ein_grouped_plan_autotune(void *plan, ein_stream_t stream) {
+    if (plan == nullptr) return -1;
+    auto *p = static_cast<ein::GroupedPlan *>(plan);
+    return p->elem == 8 ? ein::plan_autotune<double>(*p, ein::as_stream(stream))
+                        : ein::plan_autotune<float>(*p, ein::as_stream(stream));
+}
+
+int ein_grouped_plan_info(void *plan, int32_t *tile_rows, int32_t *waves, int64_t *tiles) {
+    if (plan == nullptr) return -1;
+    auto *p = static_cast<ein::GroupedPlan *>(plan);
+    if (tile_rows) *tile_rows = p->bm_index ? 128 : 64;
+    if (waves) *waves = int32_t(p->waves.size());
+    if (tiles) *tiles = p->tiles_total[p->bm_index];
+    return EIN_SUCCESS;
+}
+
+int ein_grouped_plan_destroy(void *plan) {
+    if (plan == nullptr) return EIN_SUCCESS;
+    auto *p = static_cast<ein::GroupedPlan *>(plan);
+    for (auto &w : p->waves)
+        for (int v = 0; v < 2; ++v) {
+            (void)ein_free(w.d_items[v]);
+            (void)ein_free(w.d_tiles[v]);
+        }
+    delete p;
+    return EIN_SUCCESS;
 }
 
 } // extern "C"

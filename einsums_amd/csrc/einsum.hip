@@ -343,6 +343,26 @@ int einsum_execute(int alg, T cp, ein_tensor_desc const &C, T *c, T abp, ein_ten
     return run_generic<T>(cp, C, c, abp, A, a, B, b, used, stream);
 }
 
+template <typename T>
+int einsum_adjoint(int32_t wrt, T alpha, const ein_tensor_desc *C, const T *dC, const ein_tensor_desc *A, const T *a,
+                          const ein_tensor_desc *B, const T *b, T beta, const ein_tensor_desc *G, T *g,
+                          int32_t *algorithm_used, ein_stream_t stream) {
+    if (wrt != 0 && wrt != 1) return fail(-1, "einsum_adjoint: wrt must be 0 (A) or 1 (B)");
+    if (C == nullptr || A == nullptr || B == nullptr || G == nullptr) return fail(-3, "einsum_adjoint: null descriptor");
+    ein_tensor_desc const &X = wrt == 0 ? *A : *B; // differentiated operand
+    ein_tensor_desc const &Y = wrt == 0 ? *B : *A; // the other operand
+    if (G->rank != X.rank) return fail(EIN_ERROR_RANK, "einsum_adjoint: gradient rank differs from the operand");
+    ein_tensor_desc Gd = *G;
+    for (int d = 0; d < X.rank; ++d) {
+        if (G->dims[d] != X.dims[d]) return fail(EIN_ERROR_DIMENSION, "einsum_adjoint: gradient extents differ");
+        Gd.labels[d] = X.labels[d];
+    }
+    T const *y = wrt == 0 ? b : a;
+    // dX[x] = beta*dX + alpha * sum over labels not in X of dC[c] * Y[y]
+    return einsum_execute<T>(EIN_ALG_INDETERMINATE, beta, Gd, g, alpha, *C, dC, Y, y, algorithm_used,
+                                  as_stream(stream));
+}
+
 template int einsum_execute<double>(int, double, ein_tensor_desc const &, double *, double, ein_tensor_desc const &,
                                     double const *, ein_tensor_desc const &, double const *, int32_t *, hipStream_t);
 template int einsum_execute<float>(int, float, ein_tensor_desc const &, float *, float, ein_tensor_desc const &,
@@ -397,6 +417,18 @@ int ein_einsum_split_f64(int32_t nslices, int32_t slice_type, double c_prefactor
                                     stream);
     ein::t_split_slices = 0;
     return st;
+}
+
+int ein_einsum_adjoint_f64(int32_t wrt, double alpha, const ein_tensor_desc *C, const double *dC, const ein_tensor_desc *A,
+                           const double *a, const ein_tensor_desc *B, const double *b, double beta,
+                           const ein_tensor_desc *G, double *g, int32_t *algorithm_used, ein_stream_t stream) {
+    return ein::einsum_adjoint<double>(wrt, alpha, C, dC, A, a, B, b, beta, G, g, algorithm_used, stream);
+}
+
+int ein_einsum_adjoint_f32(int32_t wrt, float alpha, const ein_tensor_desc *C, const float *dC, const ein_tensor_desc *A,
+                           const float *a, const ein_tensor_desc *B, const float *b, float beta, const ein_tensor_desc *G,
+                           float *g, int32_t *algorithm_used, ein_stream_t stream) {
+    return ein::einsum_adjoint<float>(wrt, alpha, C, dC, A, a, B, b, beta, G, g, algorithm_used, stream);
 }
 
 int ein_einsum_execute_f32(int32_t algorithm, float c_prefactor, const ein_tensor_desc *C, float *c, float ab_prefactor,

@@ -157,6 +157,38 @@ int ein_contract_f64(double c_prefactor, const ein_tensor_desc *C, double *c, do
 int ein_contract_f32(float c_prefactor, const ein_tensor_desc *C, float *c, float ab_prefactor, const ein_tensor_desc *A,
                      const float *a, const ein_tensor_desc *B, const float *b, ein_stream_t stream);
 
+/* Adjoint of C = alpha * contract(A, B): the gradient G of the operand `wrt` (0 = A,
+ * 1 = B) given the cotangent dC, G = beta*G + alpha * contract(dC, other operand), with G
+ * labelled like the operand. Labels repeated in the operand make G a diagonal (written
+ * once per element, no atomics, the rest of G scaled by beta); labels summed in the
+ * operand alone broadcast. No counterpart in the reference (it has no autograd). */
+int ein_einsum_adjoint_f64(int32_t wrt, double alpha, const ein_tensor_desc *C, const double *dC, const ein_tensor_desc *A,
+                           const double *a, const ein_tensor_desc *B, const double *b, double beta,
+                           const ein_tensor_desc *G, double *g, int32_t *algorithm_used, ein_stream_t stream);
+int ein_einsum_adjoint_f32(int32_t wrt, float alpha, const ein_tensor_desc *C, const float *dC, const ein_tensor_desc *A,
+                           const float *a, const ein_tensor_desc *B, const float *b, float beta, const ein_tensor_desc *G,
+                           float *g, int32_t *algorithm_used, ein_stream_t stream);
+
+/* Grouped contractions: many independent generic contractions (blocks of a block-sparse
+ * einsum, batches of uneven GEMMs) planned once and run as persistent launches -- one per
+ * wave of items whose outputs differ (items writing the same C pointer go to later waves,
+ * in list order). Plan creation copies descriptors to device memory (synchronous);
+ * execution is launches only, so it can be captured into a hipGraph. The autotuner times
+ * the candidate tile heights on scratch operands shaped like the plan and keeps the
+ * faster. Replaces the per-block loop of BlockAlgebra.hpp:23-52 / TileAlgebra.hpp:18-93. */
+typedef struct ein_contraction_item {
+    ein_tensor_desc C, A, B;
+    void *c;
+    const void *a;
+    const void *b;
+    double c_prefactor, ab_prefactor;
+} ein_contraction_item;
+int ein_grouped_plan_create(int32_t elem_bytes, const ein_contraction_item *items, int32_t count, void **plan);
+int ein_grouped_plan_execute(void *plan, ein_stream_t stream);
+int ein_grouped_plan_autotune(void *plan, ein_stream_t stream);
+int ein_grouped_plan_info(void *plan, int32_t *tile_rows, int32_t *waves, int64_t *tiles);
+int ein_grouped_plan_destroy(void *plan);
+
 /* Runtime pattern analysis with the precedence of compile_plan
  * (libs/EinsumsPy/TensorAlgebra/src/PyTensorAlgebra.cpp:169-300); writes EIN_ALG_*. */
 int ein_einsum_classify(const ein_tensor_desc *C, const ein_tensor_desc *A, const ein_tensor_desc *B, int32_t *algorithm);

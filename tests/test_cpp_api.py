@@ -35,3 +35,11 @@ def test_device_einsum_program():
     res = _run("test_device_einsum", 120)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "PASS test_device_einsum" in res.stdout
+
+
+@pytest.mark.gpu
+def test_device_blocks_program():
+    """BlockDeviceTensor / TiledDeviceTensor einsums through grouped plans."""
+    res = _run("test_device_blocks", 120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "PASS test_device_blocks" in res.stdout
