@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "contract.hpp"
+#include "tile_engine.hpp"
 
 namespace ein {
 
@@ -138,25 +139,23 @@ int fill_group(IndexGroup &G, std::vector<Digit> const &v, int nops) {
 constexpr int64_t NO_ROW = INT64_MIN;
 
 // One tile (or the slab range [s0, s1) of it) of a gather-GEMM problem. `As`/`Bs` are
-// the workgroup's double-buffered LDS slabs; with `piece` != nullptr the raw accumulators
-// go there ([BN][BM], i fastest) instead of through the epilogue into C.
-template <typename T, int BM, int BN, int BK, bool A_I_FAST, bool B_J_FAST>
+// the workgroup's double-buffered LDS slabs (layouts of tile_engine.hpp; AKG / BKG pick
+// the k-grouped layout for an operand whose unit stride runs along k); with
+// `piece` != nullptr the raw accumulators go there ([BN][BM], i fastest) instead of
+// through the epilogue into C.
+template <typename T, int BM, int BN, int BK, bool AKG, bool BKG>
 __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64_t t, uint64_t s0, uint64_t s1, T *As,
                                                T *Bs, T *piece) {
-    using Cfg            = TileCfg<T, BM, BN, BK>;
-    int const tid        = threadIdx.x;
-    int const lane       = tid & 63;
-    int const wave       = tid >> 6;
-    int const wm         = wave & 1;
-    int const wn         = wave >> 1;
-    constexpr int A_STEP = A_I_FAST ? GEMM_THREADS / BM : GEMM_THREADS / BK;
-    int const a_ii       = A_I_FAST ? tid % BM : tid / BK;
-    int const a_pp       = A_I_FAST ? tid / BM : tid % BK;
-    constexpr int B_STEP = B_J_FAST ? GEMM_THREADS / BN : GEMM_THREADS / BK;
-    int const b_jj       = B_J_FAST ? tid % BN : tid / BK;
-    int const b_pp       = B_J_FAST ? tid / BN : tid % BK;
-    int const r          = lane & 15;
-    int const q          = lane >> 4;
+    using Cfg       = TileCfg<T, BM, BN, BK>;
+    using LA        = SlabLayout<T, BM, BK, AKG, false>;
+    using LB        = SlabLayout<T, BN, BK, BKG, false>;
+    int const tid   = threadIdx.x;
+    int const lane  = tid & 63;
+    int const wave  = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int const wm    = wave & 1;
+    int const wn    = wave >> 1;
+    int const r     = lane & 15;
+    int const q     = lane >> 4;
     uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
 
     uint32_t const bz = uint32_t(t / tiles_mn);
@@ -172,105 +171,60 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
     T const *__restrict__ Ab = P.A + boff[0];
     T const *__restrict__ Bb = P.B + boff[1];
 
-    // Row / column offsets fixed for the whole tile (NO_ROW marks out of range).
-    constexpr int NRA = A_I_FAST ? 1 : Cfg::LA;
-    constexpr int NRB = B_J_FAST ? 1 : Cfg::LB;
-    int64_t rowA[NRA], colB[NRB];
-#pragma unroll
-    for (int rr = 0; rr < NRA; ++rr) {
-        int64_t i = i0 + (A_I_FAST ? a_ii : a_ii + rr * A_STEP);
-        int64_t o[2];
-        group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
-        rowA[rr] = i < P.m ? o[0] : NO_ROW;
-    }
-#pragma unroll
-    for (int rr = 0; rr < NRB; ++rr) {
-        int64_t j = j0 + (B_J_FAST ? b_jj : b_jj + rr * B_STEP);
-        int64_t o[2];
-        group_decode<2>(uint32_t(j < P.n ? j : 0), P.gn, o);
-        colB[rr] = j < P.n ? o[0] : NO_ROW;
-    }
-
-    T ra[Cfg::LA], rb[Cfg::LB];
-    auto load_slab = [&](int64_t k0) {
-        // k offsets for the p-fast operands: one decode serves both A and B when both
-        // walk p with the same lane mapping
-        int64_t kfix[2] = {0, 0};
-        bool kfix_ok    = false;
-        if (!A_I_FAST || !B_J_FAST) {
-            int64_t p = k0 + (A_I_FAST ? b_pp : a_pp);
-            kfix_ok   = p < kend;
-            group_decode<2>(uint32_t(kfix_ok ? p : 0), P.gk, kfix);
-        }
-#pragma unroll
-        for (int rr = 0; rr < Cfg::LA; ++rr) {
-            T v = T(0);
-            if constexpr (A_I_FAST) {
-                int64_t p = k0 + a_pp + rr * A_STEP;
-                if (rowA[0] != NO_ROW && p < kend) {
-                    int64_t o[2];
-                    group_decode<2>(uint32_t(p), P.gk, o);
-                    v = Ab[rowA[0] + o[0]];
-                }
-            } else {
-                if (rowA[rr] != NO_ROW && kfix_ok) v = Ab[rowA[rr] + kfix[0]];
-            }
-            ra[rr] = v;
-        }
-#pragma unroll
-        for (int rr = 0; rr < Cfg::LB; ++rr) {
-            T v = T(0);
-            if constexpr (B_J_FAST) {
-                int64_t p = k0 + b_pp + rr * B_STEP;
-                if (colB[0] != NO_ROW && p < kend) {
-                    int64_t o[2];
-                    group_decode<2>(uint32_t(p), P.gk, o);
-                    v = Bb[colB[0] + o[1]];
-                }
-            } else {
-                if (colB[rr] != NO_ROW && kfix_ok) v = Bb[colB[rr] + kfix[1]];
-            }
-            rb[rr] = v;
-        }
+    // element offsets through the index groups; a lane touches at most two tile rows, so
+    // the row decodes are loop invariant
+    auto offA = [&](int row, int64_t p, bool &valid) -> int64_t {
+        bool const pv = p < kend;
+        int64_t ok[2];
+        group_decode<2>(uint32_t(pv ? p : 0), P.gk, ok);
+        int64_t const i = i0 + row;
+        bool const iv   = i < P.m;
+        int64_t om[2];
+        group_decode<2>(uint32_t(iv ? i : 0), P.gm, om);
+        valid = pv && iv;
+        return om[0] + ok[0];
     };
-    auto store_slab = [&](int buf) {
-#pragma unroll
-        for (int rr = 0; rr < Cfg::LA; ++rr) {
-            int ii = A_I_FAST ? a_ii : a_ii + rr * A_STEP;
-            int pp = A_I_FAST ? a_pp + rr * A_STEP : a_pp;
-            As[buf * (BK * Cfg::LDA) + pp * Cfg::LDA + ii] = ra[rr];
-        }
-#pragma unroll
-        for (int rr = 0; rr < Cfg::LB; ++rr) {
-            int jj = B_J_FAST ? b_jj : b_jj + rr * B_STEP;
-            int pp = B_J_FAST ? b_pp + rr * B_STEP : b_pp;
-            Bs[buf * (BK * Cfg::LDB) + pp * Cfg::LDB + jj] = rb[rr];
-        }
+    auto offB = [&](int row, int64_t p, bool &valid) -> int64_t {
+        bool const pv = p < kend;
+        int64_t ok[2];
+        group_decode<2>(uint32_t(pv ? p : 0), P.gk, ok);
+        int64_t const j = j0 + row;
+        bool const jv   = j < P.n;
+        int64_t on[2];
+        group_decode<2>(uint32_t(jv ? j : 0), P.gn, on);
+        valid = pv && jv;
+        return on[0] + ok[1];
     };
 
     WaveAcc<T, Cfg::FM, Cfg::FN> acc;
     acc.zero();
     int64_t const nslabs = int64_t(s1 - s0);
-    load_slab(kbeg);
-    store_slab(0);
-    __syncthreads();
+    __syncthreads(); // LDS of the previous segment no longer read
+    dma_slab<T, BM, BK, AKG, false>(Ab, As, wave, lane, kbeg, offA);
+    dma_slab<T, BN, BK, BKG, false>(Bb, Bs, wave, lane, kbeg, offB);
     for (int64_t sl = 0; sl < nslabs; ++sl) {
         int const buf = int(sl & 1);
-        if (sl + 1 < nslabs) load_slab(kbeg + (sl + 1) * BK);
-        mfma_slab<T, BM, BN, BK>(As + buf * (BK * Cfg::LDA), Bs + buf * (BK * Cfg::LDB), acc, wm, wn, lane);
-        if (sl + 1 < nslabs) store_slab(buf ^ 1);
-        __syncthreads();
+        dma_wait_all();
+        if (sl + 1 < nslabs) {
+            dma_slab<T, BM, BK, AKG, false>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
+            dma_slab<T, BN, BK, BKG, false>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
+        }
+        mfma_slab_lds<T, BM, BN, BK, AKG, false, BKG, false>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, acc, wm, wn, lane);
     }
 
+    // lane coordinates made opaque so the store addresses are not hoisted out of the
+    // caller's tile loop
+    int ri = wm * Cfg::WM + r, rj = wn * Cfg::WN;
+    asm volatile("" : "+v"(ri), "+v"(rj));
     if (piece != nullptr) {
-        #pragma unroll
+#pragma unroll
         for (int a = 0; a < Cfg::FM; ++a)
 #pragma unroll
             for (int b = 0; b < Cfg::FN; ++b)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    int ii = wm * Cfg::WM + a * 16 + r;
-                    int jj = wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
+                    int ii = ri + a * 16;
+                    int jj = rj + b * 16 + Mfma<T>::row(q, v);
                     piece[jj * BM + ii] = acc.c[a][b][v];
                 }
         return;
@@ -279,7 +233,7 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
     int64_t rowC[Cfg::FM];
 #pragma unroll
     for (int a = 0; a < Cfg::FM; ++a) {
-        int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
+        int64_t i = i0 + ri + a * 16;
         int64_t o[2];
         group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
         rowC[a] = i < P.m ? o[1] : NO_ROW;
@@ -288,7 +242,7 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
     for (int b = 0; b < Cfg::FN; ++b)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
+            int64_t j = j0 + rj + b * 16 + Mfma<T>::row(q, v);
             if (j >= P.n) continue;
             int64_t o[2];
             group_decode<2>(uint32_t(j), P.gn, o);
@@ -303,23 +257,21 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
         }
 }
 
-template <typename T, int BM, int BN, int BK, bool A_I_FAST, bool B_J_FAST>
+template <typename T, int BM, int BN, int BK, bool AKG, bool BKG>
 __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProblem<T> P) {
-    using Cfg = TileCfg<T, BM, BN, BK>;
-    __shared__ T As[2 * BK * Cfg::LDA];
-    __shared__ T Bs[2 * BK * Cfg::LDB];
+    __shared__ __attribute__((aligned(16))) T As[2 * SlabLayout<T, BM, BK, AKG, false>::SIZE];
+    __shared__ __attribute__((aligned(16))) T Bs[2 * SlabLayout<T, BN, BK, BKG, false>::SIZE];
     StreamK const &sk = P.sk;
     uint64_t const S  = sk.slabs;
-    uint64_t u0, u1;
-    sk_range(sk, u0, u1);
-    uint64_t const wlog = sk.units ? u0 / sk.units : 0;
+    uint64_t u0, u1, wlog;
+    sk_range(sk, u0, u1, wlog);
     for (uint64_t u = u0; u < u1;) {
         uint64_t const t  = u / S;
         uint64_t const s0 = u - t * S;
         uint64_t const s1 = (S - s0 < u1 - u) ? S : s0 + (u1 - u);
         u += s1 - s0;
         T *piece = (s0 == 0 && s1 == S) ? nullptr : P.partial + sk_slot(sk, wlog, t) * (BM * BN);
-        gather_segment<T, BM, BN, BK, A_I_FAST, B_J_FAST>(P, t, s0, s1, As, Bs, piece);
+        gather_segment<T, BM, BN, BK, AKG, BKG>(P, t, s0, s1, As, Bs, piece);
     }
 }
 
@@ -328,10 +280,9 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProb
 template <typename T, int BM, int BN>
 __global__ __launch_bounds__(256) void gather_streamk_fixup(GatherProblem<T> P) {
     StreamK const &sk = P.sk;
-    uint64_t const t  = blockIdx.x;
-    uint64_t const S  = sk.slabs;
-    uint64_t const g0 = (t * S) / sk.units;
-    uint64_t const g1 = ((t + 1) * S - 1) / sk.units;
+    uint64_t const t = uint64_t(sk.dp_tiles) + blockIdx.x;
+    uint64_t g0, g1;
+    sk_owners(sk, t, g0, g1);
     if (g0 == g1) return;
     uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
     uint32_t const bz       = uint32_t(t / tiles_mn);
@@ -471,13 +422,13 @@ template <typename T, int BM, int BN>
 void launch_gather(GatherProblem<T> const &P, bool a_i_fast, bool b_j_fast, dim3 grid, hipStream_t stream) {
     constexpr int BK = 16;
     if (a_i_fast && b_j_fast)
-        gemm_gather_kernel<T, BM, BN, BK, true, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
-    else if (a_i_fast)
-        gemm_gather_kernel<T, BM, BN, BK, true, false><<<grid, GEMM_THREADS, 0, stream>>>(P);
-    else if (b_j_fast)
-        gemm_gather_kernel<T, BM, BN, BK, false, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
-    else
         gemm_gather_kernel<T, BM, BN, BK, false, false><<<grid, GEMM_THREADS, 0, stream>>>(P);
+    else if (a_i_fast)
+        gemm_gather_kernel<T, BM, BN, BK, false, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
+    else if (b_j_fast)
+        gemm_gather_kernel<T, BM, BN, BK, true, false><<<grid, GEMM_THREADS, 0, stream>>>(P);
+    else
+        gemm_gather_kernel<T, BM, BN, BK, true, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
 }
 
 } // namespace
@@ -561,7 +512,7 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         P.partial = nullptr;
         if (P.sk.units != 0) {
             void *ws = nullptr;
-            EIN_TRY(workspace(stream, size_t(P.sk.grid) * 2 * size_t(bm) * BN * sizeof(T), &ws));
+            EIN_TRY(workspace(stream, size_t(sk_piece_slots(P.sk)) * size_t(bm) * BN * sizeof(T), &ws));
             P.partial = static_cast<T *>(ws);
         }
         dim3 grid{P.sk.grid};
@@ -572,9 +523,9 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         EIN_LAUNCH_CHECK();
         if (P.sk.units != 0) {
             if (bm == 128)
-                gather_streamk_fixup<T, 128, 64><<<uint32_t(tiles), 256, 0, stream>>>(P);
+                gather_streamk_fixup<T, 128, 64><<<uint32_t(tiles - P.sk.dp_tiles), 256, 0, stream>>>(P);
             else
-                gather_streamk_fixup<T, 64, 64><<<uint32_t(tiles), 256, 0, stream>>>(P);
+                gather_streamk_fixup<T, 64, 64><<<uint32_t(tiles - P.sk.dp_tiles), 256, 0, stream>>>(P);
             EIN_LAUNCH_CHECK();
         }
         return EIN_SUCCESS;
@@ -713,18 +664,21 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_grouped_kernel(GroupedIt
                                                                        GroupedTile const *__restrict__ tiles,
                                                                        uint32_t ntiles) {
     constexpr int BN = 64, BK = 16;
-    using Cfg = TileCfg<T, BM, BN, BK>;
-    __shared__ T As[2 * BK * Cfg::LDA];
-    __shared__ T Bs[2 * BK * Cfg::LDB];
+    // the k-major layouts (padded) are the larger ones
+    __shared__ __attribute__((aligned(16))) T As[2 * SlabLayout<T, BM, BK, false, false>::SIZE];
+    __shared__ __attribute__((aligned(16))) T Bs[2 * SlabLayout<T, BN, BK, false, false>::SIZE];
+    static_assert(SlabLayout<T, BM, BK, false, false>::SIZE >= SlabLayout<T, BM, BK, true, false>::SIZE &&
+                      SlabLayout<T, BN, BK, false, false>::SIZE >= SlabLayout<T, BN, BK, true, false>::SIZE,
+                  "LDS sized for the larger layout");
     for (uint32_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
         GroupedTile const e           = tiles[tt];
         GroupedItem<T> const &it      = items[e.item];
         uint64_t const S              = uint64_t((it.P.k + BK - 1) / BK);
         switch (it.flags) {
-        case 3: gather_segment<T, BM, BN, BK, true, true>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
-        case 1: gather_segment<T, BM, BN, BK, true, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
-        case 2: gather_segment<T, BM, BN, BK, false, true>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
-        default: gather_segment<T, BM, BN, BK, false, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        case 3: gather_segment<T, BM, BN, BK, false, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        case 1: gather_segment<T, BM, BN, BK, false, true>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        case 2: gather_segment<T, BM, BN, BK, true, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        default: gather_segment<T, BM, BN, BK, true, true>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
         }
     }
 }

@@ -18,16 +18,22 @@ struct IndexGroup {
     int64_t s[3][GMAX];
 };
 
+// Digits are walked innermost (d = n-1) to outermost. The loop is unrolled over GMAX with
+// a uniform guard so every array index is a compile-time constant: a runtime index into
+// a kernel-argument struct would force the struct into private memory.
 template <int NOPS>
 __device__ __forceinline__ void group_decode(uint32_t idx, IndexGroup const &G, int64_t (&off)[NOPS]) {
 #pragma unroll
     for (int o = 0; o < NOPS; ++o) off[o] = 0;
-    for (int d = G.n - 1; d >= 0; --d) {
-        uint32_t q = fast_div(idx, G.fd[d]);
-        uint32_t r = idx - q * G.ext[d];
 #pragma unroll
-        for (int o = 0; o < NOPS; ++o) off[o] += int64_t(r) * G.s[o][d];
-        idx = q;
+    for (int d = GMAX - 1; d >= 0; --d) {
+        if (d < G.n) {
+            uint32_t q = fast_div(idx, G.fd[d]);
+            uint32_t r = idx - q * G.ext[d];
+#pragma unroll
+            for (int o = 0; o < NOPS; ++o) off[o] += int64_t(r) * G.s[o][d];
+            idx = q;
+        }
     }
 }
 

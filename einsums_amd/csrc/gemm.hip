@@ -11,14 +11,19 @@
 #include <string>
 
 #include "gemm.hpp"
+#include "tile_engine.hpp"
 
 namespace ein {
 
-template <typename T, int BM, int BN, int BK, bool A_I_FAST, bool B_J_FAST>
+// Operand slabs go global -> LDS by DMA (tile_engine.hpp); AKG / BKG select the
+// k-grouped LDS layout for an operand contiguous along k, AW / BW the 16-byte DMA.
+template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW>
 __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem<T> P) {
     using Cfg = TileCfg<T, BM, BN, BK>;
-    __shared__ T As[2][BK * Cfg::LDA];
-    __shared__ T Bs[2][BK * Cfg::LDB];
+    using LA  = SlabLayout<T, BM, BK, AKG, AW>;
+    using LB  = SlabLayout<T, BN, BK, BKG, BW>;
+    __shared__ __attribute__((aligned(16))) T As[2 * LA::SIZE];
+    __shared__ __attribute__((aligned(16))) T Bs[2 * LB::SIZE];
 
     int const tid  = threadIdx.x;
     int const lane = tid & 63;
@@ -28,20 +33,11 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
     int const r    = lane & 15;
     int const q    = lane >> 4;
 
-    // per-thread load coordinates: A slab BM x BK, B slab BK x BN
-    constexpr int A_STEP = A_I_FAST ? GEMM_THREADS / BM : GEMM_THREADS / BK;
-    int const a_ii       = A_I_FAST ? tid % BM : tid / BK;
-    int const a_pp       = A_I_FAST ? tid / BM : tid % BK;
-    constexpr int B_STEP = B_J_FAST ? GEMM_THREADS / BN : GEMM_THREADS / BK;
-    int const b_jj       = B_J_FAST ? tid % BN : tid / BK;
-    int const b_pp       = B_J_FAST ? tid / BN : tid % BK;
-
     StreamK const &sk       = P.sk;
     uint64_t const S        = sk.slabs;
     uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
-    uint64_t u0, u1;
-    sk_range(sk, u0, u1);
-    uint64_t const wlog = sk.units ? u0 / sk.units : 0;
+    uint64_t u0, u1, wlog;
+    sk_range(sk, u0, u1, wlog);
 
     for (uint64_t u = u0; u < u1;) {
         uint64_t const t  = u / S;
@@ -54,57 +50,40 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
         tile_coords(uint32_t(t % tiles_mn), P.tiles_m, P.tiles_n, tm, tn);
         int64_t const i0   = int64_t(tm) * BM;
         int64_t const j0   = int64_t(tn) * BN;
-        int64_t const kbeg = int64_t(s0) * BK;
         int64_t const kend = int64_t(s1) * BK < P.k ? int64_t(s1) * BK : P.k;
-        T const *__restrict__ Ab = P.A + batch * P.a_sb;
-        T const *__restrict__ Bb = P.B + batch * P.b_sb;
-
-        T ra[Cfg::LA], rb[Cfg::LB];
-        auto load_slab = [&](int64_t k0) {
-#pragma unroll
-            for (int rr = 0; rr < Cfg::LA; ++rr) {
-                int ii    = A_I_FAST ? a_ii : a_ii + rr * A_STEP;
-                int pp    = A_I_FAST ? a_pp + rr * A_STEP : a_pp;
-                int64_t i = i0 + ii, p = k0 + pp;
-                ra[rr]    = (i < P.m && p < kend) ? Ab[i * P.a_sm + p * P.a_sk] : T(0);
-            }
-#pragma unroll
-            for (int rr = 0; rr < Cfg::LB; ++rr) {
-                int jj    = B_J_FAST ? b_jj : b_jj + rr * B_STEP;
-                int pp    = B_J_FAST ? b_pp + rr * B_STEP : b_pp;
-                int64_t j = j0 + jj, p = k0 + pp;
-                rb[rr]    = (j < P.n && p < kend) ? Bb[p * P.b_sk + j * P.b_sn] : T(0);
-            }
+        T const *__restrict__ Ab = P.A + batch * P.a_sb + i0 * P.a_sm;
+        T const *__restrict__ Bb = P.B + batch * P.b_sb + j0 * P.b_sn;
+        int64_t const mrem = P.m - i0, nrem = P.n - j0;
+        auto offA = [&](int row, int64_t p, bool &valid) -> int64_t {
+            valid = row < mrem && p < kend;
+            return row * P.a_sm + p * P.a_sk;
         };
-        auto store_slab = [&](int buf) {
-#pragma unroll
-            for (int rr = 0; rr < Cfg::LA; ++rr) {
-                int ii = A_I_FAST ? a_ii : a_ii + rr * A_STEP;
-                int pp = A_I_FAST ? a_pp + rr * A_STEP : a_pp;
-                As[buf][pp * Cfg::LDA + ii] = ra[rr];
-            }
-#pragma unroll
-            for (int rr = 0; rr < Cfg::LB; ++rr) {
-                int jj = B_J_FAST ? b_jj : b_jj + rr * B_STEP;
-                int pp = B_J_FAST ? b_pp + rr * B_STEP : b_pp;
-                Bs[buf][pp * Cfg::LDB + jj] = rb[rr];
-            }
+        auto offB = [&](int row, int64_t p, bool &valid) -> int64_t {
+            valid = row < nrem && p < kend;
+            return row * P.b_sn + p * P.b_sk;
         };
 
         WaveAcc<T, Cfg::FM, Cfg::FN> acc;
         acc.zero();
         int64_t const nslabs = int64_t(s1 - s0);
-        load_slab(kbeg);
-        store_slab(0);
-        __syncthreads();
+        int64_t const kbeg   = int64_t(s0) * BK;
+        __syncthreads(); // LDS of the previous segment no longer read
+        dma_slab<T, BM, BK, AKG, AW>(Ab, As, wave, lane, kbeg, offA);
+        dma_slab<T, BN, BK, BKG, BW>(Bb, Bs, wave, lane, kbeg, offB);
         for (int64_t sl = 0; sl < nslabs; ++sl) {
             int const buf = int(sl & 1);
-            if (sl + 1 < nslabs) load_slab(kbeg + (sl + 1) * BK);
-            mfma_slab<T, BM, BN, BK>(As[buf], Bs[buf], acc, wm, wn, lane);
-            if (sl + 1 < nslabs) store_slab(buf ^ 1);
-            __syncthreads();
+            dma_wait_all();
+            if (sl + 1 < nslabs) {
+                dma_slab<T, BM, BK, AKG, AW>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
+                dma_slab<T, BN, BK, BKG, BW>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
+            }
+            mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, acc, wm, wn, lane);
         }
 
+        // Lane coordinates made opaque per segment so the compiler does not hoist the
+        // 4*FM*FN store addresses out of the segment loop (they would pin registers).
+        int ri = wm * Cfg::WM + r, rj = wn * Cfg::WN + q;
+        asm volatile("" : "+v"(ri), "+v"(rj));
         if (s0 == 0 && s1 == S) {
             T *__restrict__ Cb = P.C + batch * P.c_sb;
 #pragma unroll
@@ -113,8 +92,8 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
                 for (int b = 0; b < Cfg::FN; ++b)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        int64_t i = i0 + wm * Cfg::WM + a * 16 + r;
-                        int64_t j = j0 + wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
+                        int64_t i = i0 + ri + a * 16;
+                        int64_t j = j0 + rj - q + b * 16 + Mfma<T>::row(q, v);
                         if (i < P.m && j < P.n) {
                             T *cp = Cb + i * P.c_sm + j * P.c_sn;
                             T val = P.alpha * acc.c[a][b][v];
@@ -131,8 +110,8 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
                 for (int b = 0; b < Cfg::FN; ++b)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        int ii = wm * Cfg::WM + a * 16 + r;
-                        int jj = wn * Cfg::WN + b * 16 + Mfma<T>::row(q, v);
+                        int ii = ri + a * 16;
+                        int jj = rj - q + b * 16 + Mfma<T>::row(q, v);
                         piece[jj * BM + ii] = acc.c[a][b][v];
                     }
         }
@@ -144,10 +123,9 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
 template <typename T, int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_streamk_fixup(GemmProblem<T> P) {
     StreamK const &sk = P.sk;
-    uint64_t const t  = blockIdx.x;
-    uint64_t const S  = sk.slabs;
-    uint64_t const g0 = (t * S) / sk.units;
-    uint64_t const g1 = ((t + 1) * S - 1) / sk.units;
+    uint64_t const t = uint64_t(sk.dp_tiles) + blockIdx.x;
+    uint64_t g0, g1;
+    sk_owners(sk, t, g0, g1);
     if (g0 == g1) return;
     uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
     int64_t const batch     = int64_t(t / tiles_mn);
@@ -185,10 +163,18 @@ __global__ __launch_bounds__(256) void scale_2d_kernel(GemmProblem<T> P) {
 
 namespace {
 
+template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW>
+int launch_dense_kernel(GemmProblem<T> const &P, dim3 grid, hipStream_t stream) {
+    if constexpr (SlabLayout<T, BM, BK, AKG, AW>::VALID && SlabLayout<T, BN, BK, BKG, BW>::VALID) {
+        gemm_dense_kernel<T, BM, BN, BK, AKG, AW, BKG, BW><<<grid, GEMM_THREADS, 0, stream>>>(P);
+        return EIN_SUCCESS;
+    } else {
+        return fail(EIN_ERROR_UNSUPPORTED, "gemm: operand staging not realisable for this tile");
+    }
+}
+
 template <typename T, int BM, int BN, int BK>
 int launch_dense(GemmProblem<T> &P, hipStream_t stream) {
-    bool const a_i_fast = (P.a_sm == 1) || (P.a_sk != 1 && P.a_sm <= P.a_sk);
-    bool const b_j_fast = (P.b_sn == 1) || (P.b_sk != 1 && P.b_sn <= P.b_sk);
     P.tiles_m      = uint32_t(ceil_div(P.m, BM));
     P.tiles_n      = uint32_t(ceil_div(P.n, BN));
     uint64_t tiles = uint64_t(P.tiles_m) * P.tiles_n * uint64_t(P.batch);
@@ -199,21 +185,36 @@ int launch_dense(GemmProblem<T> &P, hipStream_t stream) {
     P.partial      = nullptr;
     if (P.sk.units != 0) {
         void *ws = nullptr;
-        EIN_TRY(workspace(stream, size_t(P.sk.grid) * 2 * BM * BN * sizeof(T), &ws));
+        EIN_TRY(workspace(stream, size_t(sk_piece_slots(P.sk)) * BM * BN * sizeof(T), &ws));
         P.partial = static_cast<T *>(ws);
     }
+    OperandStaging const sa = choose_staging<T>(BM, P.a_sm, P.a_sk, P.a_sb, P.batch, P.m, P.k, P.A);
+    OperandStaging const sb = choose_staging<T>(BN, P.b_sn, P.b_sk, P.b_sb, P.batch, P.n, P.k, P.B);
     dim3 grid{P.sk.grid};
-    if (a_i_fast && b_j_fast)
-        gemm_dense_kernel<T, BM, BN, BK, true, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
-    else if (a_i_fast && !b_j_fast)
-        gemm_dense_kernel<T, BM, BN, BK, true, false><<<grid, GEMM_THREADS, 0, stream>>>(P);
-    else if (!a_i_fast && b_j_fast)
-        gemm_dense_kernel<T, BM, BN, BK, false, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
-    else
-        gemm_dense_kernel<T, BM, BN, BK, false, false><<<grid, GEMM_THREADS, 0, stream>>>(P);
+    int const code = (sa.kgroup ? 8 : 0) | (sa.wide ? 4 : 0) | (sb.kgroup ? 2 : 0) | (sb.wide ? 1 : 0);
+    int st = EIN_SUCCESS;
+    switch (code) {
+    case 0: st = launch_dense_kernel<T, BM, BN, BK, false, false, false, false>(P, grid, stream); break;
+    case 1: st = launch_dense_kernel<T, BM, BN, BK, false, false, false, true>(P, grid, stream); break;
+    case 2: st = launch_dense_kernel<T, BM, BN, BK, false, false, true, false>(P, grid, stream); break;
+    case 3: st = launch_dense_kernel<T, BM, BN, BK, false, false, true, true>(P, grid, stream); break;
+    case 4: st = launch_dense_kernel<T, BM, BN, BK, false, true, false, false>(P, grid, stream); break;
+    case 5: st = launch_dense_kernel<T, BM, BN, BK, false, true, false, true>(P, grid, stream); break;
+    case 6: st = launch_dense_kernel<T, BM, BN, BK, false, true, true, false>(P, grid, stream); break;
+    case 7: st = launch_dense_kernel<T, BM, BN, BK, false, true, true, true>(P, grid, stream); break;
+    case 8: st = launch_dense_kernel<T, BM, BN, BK, true, false, false, false>(P, grid, stream); break;
+    case 9: st = launch_dense_kernel<T, BM, BN, BK, true, false, false, true>(P, grid, stream); break;
+    case 10: st = launch_dense_kernel<T, BM, BN, BK, true, false, true, false>(P, grid, stream); break;
+    case 11: st = launch_dense_kernel<T, BM, BN, BK, true, false, true, true>(P, grid, stream); break;
+    case 12: st = launch_dense_kernel<T, BM, BN, BK, true, true, false, false>(P, grid, stream); break;
+    case 13: st = launch_dense_kernel<T, BM, BN, BK, true, true, false, true>(P, grid, stream); break;
+    case 14: st = launch_dense_kernel<T, BM, BN, BK, true, true, true, false>(P, grid, stream); break;
+    default: st = launch_dense_kernel<T, BM, BN, BK, true, true, true, true>(P, grid, stream); break;
+    }
+    EIN_TRY(st);
     EIN_LAUNCH_CHECK();
     if (P.sk.units != 0) {
-        gemm_streamk_fixup<T, BM, BN><<<uint32_t(tiles), 256, 0, stream>>>(P);
+        gemm_streamk_fixup<T, BM, BN><<<uint32_t(tiles - P.sk.dp_tiles), 256, 0, stream>>>(P);
         EIN_LAUNCH_CHECK();
     }
     return EIN_SUCCESS;

@@ -114,62 +114,87 @@ __host__ __device__ __forceinline__ void tile_coords(uint32_t logical, uint32_t 
 }
 
 // ------------------------------------------------------- stream-K schedule
-// The work of a GEMM-like launch is (tile, BK-slab) units, tile-major. In data-parallel
-// mode every workgroup owns one whole tile. In stream-K mode workgroup w owns the
-// contiguous unit range [w*U, min(W, (w+1)*U)): whole tiles inside it are finished in
-// place, and the (at most two) tiles cut by its range ends are written as partial
-// accumulators to piece slots 2w (first tile it touches) and 2w+1 (last tile); a fix-up
-// pass sums the pieces of every cut tile in workgroup order, so results are
-// deterministic and no atomics are used.
+// The work of a GEMM-like launch is (tile, BK-slab) units, tile-major. A hybrid schedule
+// keeps L2 locality and still fills the last round:
+// - tiles [0, dp_tiles) are data-parallel: workgroup b < dp_tiles owns one whole tile
+//   (XCD-remapped, so tiles running at the same time are neighbours on one XCD);
+// - the remaining tiles (the last one to two rounds' worth) are shared stream-K style by
+//   the last grid - dp_tiles workgroups: stream-K workgroup w owns the contiguous unit
+//   range base + [w*U, min(W, (w+1)*U)) with base = dp_tiles*slabs. Whole tiles inside it
+//   are finished in place; the (at most two) tiles cut by its range ends are written as
+//   partial accumulators to piece slots 2w (first tile it touches) and 2w+1 (last tile).
+// A fix-up pass over the stream-K tiles sums the pieces of every cut tile in workgroup
+// order, so results are deterministic and no atomics are used.
 struct StreamK {
-    uint32_t tiles;  // tiles in the launch (batch included)
-    uint32_t slabs;  // BK-slabs per tile
-    uint64_t units;  // units per workgroup; 0 = data-parallel
-    uint32_t grid;   // workgroups launched
+    uint32_t tiles;    // tiles in the launch (batch included)
+    uint32_t slabs;    // BK-slabs per tile
+    uint32_t dp_tiles; // tiles owned whole by one workgroup each
+    uint32_t grid;     // workgroups launched: dp_tiles + stream-K workgroups
+    uint64_t units;    // units per stream-K workgroup; 0 = no stream-K part
 };
 
-// Decides between data-parallel and stream-K for `tiles` tiles of `slabs` slabs with
-// `slots` workgroups resident at once.
+// Decides the schedule for `tiles` tiles of `slabs` slabs with `slots` workgroups
+// resident at once.
 inline StreamK plan_stream_k(uint64_t tiles, uint64_t slabs, uint64_t slots) {
     StreamK sk{};
     sk.tiles = uint32_t(tiles);
     sk.slabs = uint32_t(slabs);
-    uint64_t rounds = (tiles + slots - 1) / slots;
-    double eff      = double(tiles) / double(rounds * slots);
-    if (tiles == 0 || slabs == 0 || eff >= 0.97 || tiles * slabs < 2 * slots) {
-        sk.units = 0;
-        sk.grid  = uint32_t(tiles);
+    uint64_t rounds = slots ? (tiles + slots - 1) / slots : 1;
+    double eff      = slots ? double(tiles) / double(rounds * slots) : 1.0;
+    if (tiles == 0 || slabs == 0 || slots == 0 || eff >= 0.97 || tiles * slabs < 2 * slots) {
+        sk.dp_tiles = uint32_t(tiles);
+        sk.grid     = uint32_t(tiles);
+        sk.units    = 0;
         return sk;
     }
-    uint64_t W = tiles * slabs;
-    uint64_t G = slots < W ? slots : W;
-    sk.units   = (W + G - 1) / G;
-    sk.grid    = uint32_t((W + sk.units - 1) / sk.units);
+    uint64_t full = tiles / slots;
+    uint64_t dp   = full >= 2 ? (full - 1) * slots : 0;
+    dp -= dp % 8; // keep the stream-K workgroups on the same XCD pattern
+    uint64_t W    = (tiles - dp) * slabs;
+    uint64_t G    = slots < W ? slots : W;
+    sk.dp_tiles   = uint32_t(dp);
+    sk.units      = (W + G - 1) / G;
+    sk.grid       = uint32_t(dp + (W + sk.units - 1) / sk.units);
     return sk;
 }
 
-// Unit range of launched workgroup `block` (remapped so that neighbouring ranges share
-// an XCD).
-__host__ __device__ __forceinline__ void sk_range_of(StreamK const &sk, uint32_t block, uint64_t &u0, uint64_t &u1) {
-    uint64_t const W = uint64_t(sk.tiles) * sk.slabs;
-    uint64_t const w = xcd_remap(block, sk.grid);
-    if (sk.units == 0) {
-        u0 = w * sk.slabs;
-        u1 = u0 + sk.slabs;
-    } else {
-        u0 = w * sk.units;
-        u1 = u0 + sk.units < W ? u0 + sk.units : W;
+// Unit range [u0, u1) of launched workgroup `block`, and its stream-K index w (0 for a
+// data-parallel workgroup).
+__host__ __device__ __forceinline__ void sk_range_of(StreamK const &sk, uint32_t block, uint64_t &u0, uint64_t &u1,
+                                                     uint64_t &w) {
+    if (block < sk.dp_tiles) {
+        uint64_t const t = xcd_remap(block, sk.dp_tiles);
+        u0               = t * sk.slabs;
+        u1               = u0 + sk.slabs;
+        w                = 0;
+        return;
     }
+    uint64_t const W    = uint64_t(sk.tiles) * sk.slabs;
+    uint64_t const base = uint64_t(sk.dp_tiles) * sk.slabs;
+    w                   = xcd_remap(block - sk.dp_tiles, sk.grid - sk.dp_tiles);
+    u0                  = base + w * sk.units;
+    u1                  = u0 + sk.units < W ? u0 + sk.units : W;
 }
 
-__device__ __forceinline__ void sk_range(StreamK const &sk, uint64_t &u0, uint64_t &u1) {
-    sk_range_of(sk, blockIdx.x, u0, u1);
+__device__ __forceinline__ void sk_range(StreamK const &sk, uint64_t &u0, uint64_t &u1, uint64_t &w) {
+    sk_range_of(sk, blockIdx.x, u0, u1, w);
 }
 
-// Piece slot of tile t for workgroup w (w being the remapped index).
+// Piece slot of tile t for stream-K workgroup w.
 __host__ __device__ __forceinline__ uint64_t sk_slot(StreamK const &sk, uint64_t w, uint64_t t) {
-    uint64_t first_tile = (w * sk.units) / sk.slabs;
+    uint64_t const first_tile = (uint64_t(sk.dp_tiles) * sk.slabs + w * sk.units) / sk.slabs;
     return 2 * w + (t == first_tile ? 0 : 1);
 }
+
+// Stream-K workgroups [g0, g1] whose unit ranges meet tile t (t >= dp_tiles); the tile was
+// finished in place when g0 == g1.
+__host__ __device__ __forceinline__ void sk_owners(StreamK const &sk, uint64_t t, uint64_t &g0, uint64_t &g1) {
+    uint64_t const off = (t - sk.dp_tiles) * sk.slabs;
+    g0                 = off / sk.units;
+    g1                 = (off + sk.slabs - 1) / sk.units;
+}
+
+// Number of piece slots a launch needs.
+inline uint64_t sk_piece_slots(StreamK const &sk) { return sk.units ? 2ull * (sk.grid - sk.dp_tiles) : 0; }
 
 } // namespace ein

@@ -1,0 +1,166 @@
+// Direct-to-LDS tile staging for the MFMA GEMM kernels (gfx950).
+//
+// A K-slab of one operand (ROWS rows x BK contraction steps) is copied from global memory
+// straight into LDS by `global_load_lds_dword{,x4}`: lane l of one wave instruction writes
+// LDS dwords [M0 + VW*l, M0 + VW*(l+1)), so an instruction fills 64*VW contiguous LDS
+// dwords and no VGPRs or ds_write instructions are spent on staging. Each lane chooses
+// which global dwords land in its LDS slot, which is what the two layouts below are
+// built on. A lane whose element lies outside the problem reads a zero page instead.
+//
+// Layouts (positions in elements inside one slab buffer):
+//  - KMAJOR (source contiguous along the rows, e.g. column-major A and i):
+//      pos(row, k) = k * PITCH + row,  PITCH = ROWS + 16.
+//    The 16-element pad puts consecutive k-rows 32 banks (FP64) / 16 banks (FP32) apart,
+//    so the 16 rows x 2 (FP64) or 4 (FP32) k values one MFMA fragment read touches are
+//    on distinct banks.
+//  - KGROUP (source contiguous along k):
+//      pos(row, k) = (k / KG) * (KG * ROWS) + KG * row + k % KG,  KG = 16 bytes / element.
+//    A lane's 16-byte chunk is KG consecutive k of one row, and a fragment read (16
+//    consecutive rows, KG consecutive k) covers 64 consecutive dwords.
+//
+// Instructions per slab are spread over the 4 waves of the workgroup (n = wave + 4x).
+// The index maps are checked on the host by tests/cpp/test_lds_layout.cpp.
+#pragma once
+
+#include "gemm_core.hpp"
+
+namespace ein {
+
+#define EIN_LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
+
+// Zero dwords read by lanes whose element is outside the problem (never written).
+static __device__ uint32_t ein_zero_page[4];
+
+template <typename T, int ROWS, int BK, bool KGROUP, bool WIDE>
+struct SlabLayout {
+    static constexpr int EW    = int(sizeof(T)) / 4;     // dwords per element
+    static constexpr int VW    = WIDE ? 4 : 1;           // dwords per lane per instruction
+    static constexpr int KG    = 4 / EW;                 // k per 16-byte group (KGROUP)
+    static constexpr int PITCH = ROWS + 16;              // KMAJOR k-row pitch (elements)
+    static constexpr int SIZE  = KGROUP ? ROWS * BK : BK * PITCH; // elements per buffer
+    static constexpr int SLAB_DW = ROWS * BK * EW;       // dwords moved per slab
+    static constexpr int INS   = SLAB_DW / (64 * VW);    // wave instructions per slab
+    static constexpr int INS_PER_WAVE = INS / 4;
+    // KMAJOR: instructions per k-row; KGROUP: instructions per k-group
+    static constexpr int IPR = KGROUP ? (ROWS * 4) / (64 * VW) : (ROWS * EW) / (64 * VW);
+    // a wide instruction must not span two LDS rows; instructions split over four waves
+    static constexpr bool VALID = IPR >= 1 && INS % 4 == 0 && BK % KG == 0 && BK % 4 == 0;
+
+    static __host__ __device__ constexpr int pos(int row, int k) {
+        return KGROUP ? (k / KG) * (KG * ROWS) + KG * row + k % KG : k * PITCH + row;
+    }
+    // Instruction n = w + 4x (wave w, x-th instruction of the wave) covers part n % IPR of
+    // LDS row / k-group g = n / IPR. Written per case so that the compiler sees that a
+    // lane touches only IPR/4 (or one) distinct rows.
+    static __host__ __device__ constexpr int part_of(int w, int x) {
+        return IPR % 4 == 0 ? w + 4 * (x % (IPR / 4)) : w % IPR;
+    }
+    static __host__ __device__ constexpr int group_of(int w, int x) {
+        return IPR % 4 == 0 ? x / (IPR / 4) : w / IPR + (4 / IPR) * x;
+    }
+    // First LDS dword written by instruction (w, x).
+    static __host__ __device__ constexpr int ins_base_dw(int w, int x) {
+        return KGROUP ? group_of(w, x) * (ROWS * 4) + part_of(w, x) * 64 * VW
+                      : group_of(w, x) * PITCH * EW + part_of(w, x) * 64 * VW;
+    }
+    // Element (row, k) and dword half h that lane l of instruction (w, x) copies (for WIDE,
+    // the first element of its 16-byte chunk; the chunk runs along rows for KMAJOR, along
+    // k for KGROUP).
+    static __host__ __device__ constexpr void lane_elem(int w, int x, int l, int &row, int &k, int &h) {
+        int const o = part_of(w, x) * 64 * VW + l * VW; // dword inside the k-row / k-group
+        if (KGROUP) {
+            row = o / 4;
+            k   = group_of(w, x) * KG + (o % 4) / EW;
+            h   = (o % 4) % EW;
+        } else {
+            row = o / EW;
+            k   = group_of(w, x);
+            h   = o % EW;
+        }
+    }
+};
+
+// Issues the DMA of the slab starting at contraction step k0. `off(row, p, valid)`
+// returns the element offset of (tile row, contraction step p) from `base` and whether
+// that element is inside the problem. For a 16-byte chunk, (row, p) is its first element
+// and the host guarantees the chunk is entirely inside or entirely outside.
+template <typename T, int ROWS, int BK, bool KGROUP, bool WIDE, typename OffsetFn>
+__device__ __forceinline__ void dma_slab(T const *__restrict__ base, T *lds, int wave, int lane, int64_t k0,
+                                         OffsetFn const &off) {
+    using L = SlabLayout<T, ROWS, BK, KGROUP, WIDE>;
+    static_assert(L::VALID, "slab layout not realisable");
+    uint32_t const *bdw = reinterpret_cast<uint32_t const *>(base);
+#pragma unroll
+    for (int x = 0; x < L::INS_PER_WAVE; ++x) {
+        int row, k, h;
+        L::lane_elem(wave, x, lane, row, k, h);
+        bool valid;
+        int64_t const e = off(row, k0 + k, valid);
+        void const *src = valid ? static_cast<void const *>(bdw + e * L::EW + h) : static_cast<void const *>(ein_zero_page);
+        uint32_t *dst   = reinterpret_cast<uint32_t *>(lds) + L::ins_base_dw(wave, x);
+        __builtin_amdgcn_global_load_lds(src, EIN_LDS_PTR(dst), 4 * L::VW, 0, 0);
+    }
+}
+
+// MFMA work of one slab from LDS, with the fragments of step kk+4 read while the MFMAs of
+// step kk run. As/Bs are the slab buffers of A (rows along i) and B (rows along j).
+template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW>
+__device__ __forceinline__ void mfma_slab_lds(T const *As, T const *Bs,
+                                              WaveAcc<T, TileCfg<T, BM, BN, BK>::FM, TileCfg<T, BM, BN, BK>::FN> &acc,
+                                              int wm, int wn, int lane) {
+    using Cfg = TileCfg<T, BM, BN, BK>;
+    using LA  = SlabLayout<T, BM, BK, AKG, AW>;
+    using LB  = SlabLayout<T, BN, BK, BKG, BW>;
+    int const r = lane & 15;
+    int const q = lane >> 4;
+    T af[2][Cfg::FM], bf[2][Cfg::FN];
+    auto frag = [&](int kk, int slot) {
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a) af[slot][a] = As[LA::pos(wm * Cfg::WM + a * 16 + r, kk + q)];
+#pragma unroll
+        for (int b = 0; b < Cfg::FN; ++b) bf[slot][b] = Bs[LB::pos(wn * Cfg::WN + b * 16 + r, kk + q)];
+    };
+    frag(0, 0);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+        int const cur = (kk / 4) & 1;
+        if (kk + 4 < BK) frag(kk + 4, cur ^ 1);
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+            for (int b = 0; b < Cfg::FN; ++b) acc.c[a][b] = Mfma<T>::mma(bf[cur][b], af[cur][a], acc.c[a][b]);
+    }
+}
+
+// Staging choice for one operand (host side): k-grouped layout when the source is
+// contiguous along k, 16-byte DMA when every chunk is contiguous, 16-byte aligned and
+// entirely inside or outside the problem.
+struct OperandStaging {
+    bool kgroup;
+    bool wide;
+};
+
+template <typename T>
+inline OperandStaging choose_staging(int rows_tile, int64_t s_row, int64_t s_k, int64_t s_batch, int64_t batch,
+                                     int64_t rows, int64_t K, void const *ptr) {
+    constexpr int EW = int(sizeof(T)) / 4;
+    constexpr int KG = 4 / EW;
+    auto mag         = [](int64_t x) { return x < 0 ? -x : x; };
+    OperandStaging st{};
+    bool const row_fast = s_row == 1 || (s_k != 1 && mag(s_row) <= mag(s_k));
+    st.kgroup           = !row_fast;
+    bool const aligned  = reinterpret_cast<uintptr_t>(ptr) % 16 == 0 && (batch <= 1 || s_batch % KG == 0);
+    if (!st.kgroup)
+        st.wide = aligned && s_row == 1 && s_k % KG == 0 && rows % KG == 0 && rows_tile * EW >= 256;
+    else
+        st.wide = aligned && s_k == 1 && s_row % KG == 0 && K % KG == 0;
+    return st;
+}
+
+// Waits for this wave's outstanding DMA, then for the whole workgroup.
+__device__ __forceinline__ void dma_wait_all() {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+}
+
+} // namespace ein

@@ -43,3 +43,10 @@ def test_device_blocks_program():
     res = _run("test_device_blocks", 120)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "PASS test_device_blocks" in res.stdout
+
+
+def test_lds_layout_program():
+    """Host check of the direct-to-LDS slab layouts: exact coverage, LDS positions the MFMA
+    fragment reads expect, and conflict-free fragment reads."""
+    res = _run("test_lds_layout", 60)
+    assert res.returncode == 0 and "0 failures" in res.stdout
