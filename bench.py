@@ -34,7 +34,9 @@ def parse():
     p.add_argument("--v", type=int, default=96)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--traffic", type=str, default=None, help="JSON file with measured HBM bytes per Wabef launch")
-    p.add_argument("--split", type=int, default=5, help="slices for the split-precision comparison (0: skip)")
+    p.add_argument("--split", type=int, default=1, help="1: add the split-precision comparison, 0: skip it")
+    p.add_argument("--precision-sweep", action="store_true",
+                   help="only run the split-precision vs FP64 GEMM sweep (one JSON line per case) and exit")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES bytes (counter calibration) and exit")
     return p.parse_args()
@@ -83,37 +85,103 @@ def cpu_baseline(I, o, v, flops):
                       f"host BLAS: {os.path.basename(pyoracle.blas_name())}"}
 
 
-def split_comparison(core, plans, W, I, ns, reps=10):
-    """Wabef and Wmnij through the split-precision GEMM vs plain FP64 on the same inputs:
-    speedup of the kernel time and relative Frobenius error of the contraction against
-    the plain FP64 result (itself within 1e-13 of the CPU oracle in the parity tests)."""
+SPLIT_MODES = (("bf16", 5), ("int8", 6))
+
+
+def split_comparison(core, plans, W, reps=10):
+    """Wabef and Wmnij through the split-precision GEMM (BF16 x5 and int8 x6 slices) vs plain
+    FP64 on the same inputs: kernel-time speedup and relative Frobenius error of the
+    contraction against the plain FP64 result (itself within 1e-13 of the CPU oracle in the
+    parity tests)."""
+    import torch
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {}
+    for name in ("Wabef", "Wmnij"):
+        plan, A, B, abp, _ = plans[name]
+        ref = torch.zeros_like(W[name])
+        plan.execute(0.0, ref, abp, A, B)
+        got = torch.zeros_like(ref)
+        fp64_ms = timed(lambda: plan.execute(0.0, got, abp, A, B))
+        row = {"fp64_ms": round(fp64_ms, 4)}
+        for st, ns in SPLIT_MODES:
+            plan.execute(0.0, got, abp, A, B, split=ns, slice_type=st)
+            torch.cuda.synchronize()
+            err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+            ms = timed(lambda: plan.execute(0.0, got, abp, A, B, split=ns, slice_type=st))
+            row[f"{st}x{ns}"] = {"ms": round(ms, 4), "speedup": round(fp64_ms / ms, 3), "rel_err_vs_fp64": float(f"{err:.3e}")}
+        out[name] = row
+    return out
+
+
+def precision_sweep(sizes=(1024, 2048, 4096, 8192, 16384), reps=3):
+    """Split-precision vs plain FP64 GEMM: DGEMM n x n x n with U(-1,1) and stressed
+    U(-1,1)*2^e (e uniform in [-20, 20]) data, and the Wabef GEMM at o=48, v=256
+    (65536 x 65536 x 2304, 34 GB output). Errors: relative Frobenius against the FP64
+    result on the GPU, and against numpy on 64 sampled rows (host BLAS, FP64)."""
     import numpy as np
     import torch
 
-    out = {"nslices": ns, "slice_type": "bf16"}
-    for name in ("Wabef", "Wmnij"):
-        plan, A, B, abp, _ = plans[name]
-        base = W[name].clone()
-        ref = torch.zeros_like(base)
-        plan.execute(0.0, ref, abp, A, B)
-        got = torch.zeros_like(base)
-        plan.execute(0.0, got, abp, A, B, split=ns)
+    import einsums_amd.core as core
+
+    def timed(fn):
+        fn()
         torch.cuda.synchronize()
-        err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
-        times = {}
-        for mode in (0, ns):
-            for _ in range(2):
-                plan.execute(0.0, got, abp, A, B, split=mode)
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(reps):
-                plan.execute(0.0, got, abp, A, B, split=mode)
-            e1.record()
-            torch.cuda.synchronize()
-            times[mode] = e0.elapsed_time(e1) / reps
-        out[name] = {"fp64_ms": round(times[0], 4), "split_ms": round(times[ns], 4),
-                     "speedup": round(times[0] / times[ns], 3), "rel_err_vs_fp64": float(f"{err:.3e}")}
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def case(label, m, n, k, make):
+        g = torch.Generator(device="cuda").manual_seed(1234)
+        A, B = make(m, k, g), make(k, n, g)
+        C64 = torch.empty((m, n), dtype=torch.float64, device="cuda")
+        t64 = timed(lambda: core.gemm("N", "N", 1.0, A, B, 0.0, C64))
+        rows = torch.linspace(0, m - 1, 64, device="cuda").long()
+        ref_rows = A[rows].cpu().numpy() @ B.cpu().numpy()
+        row = {"case": label, "m": m, "n": n, "k": k, "fp64_ms": round(t64, 3),
+               "fp64_tflops": round(2.0 * m * n * k / t64 / 1e9, 2),
+               "fp64_rel_err_rows_vs_host": float(f"{np.linalg.norm(C64[rows].cpu().numpy() - ref_rows) / np.linalg.norm(ref_rows):.3e}")}
+        Cs = torch.empty_like(C64)
+        for st, ns in SPLIT_MODES:
+            ts = timed(lambda: core.gemm_split("N", "N", 1.0, A, B, 0.0, Cs, nslices=ns, slice_type=st))
+            err = float(torch.linalg.norm(Cs - C64) / torch.linalg.norm(C64))
+            err_rows = float(np.linalg.norm(Cs[rows].cpu().numpy() - ref_rows) / np.linalg.norm(ref_rows))
+            row[f"{st}x{ns}"] = {"ms": round(ts, 3), "speedup": round(t64 / ts, 3), "rel_err_vs_fp64": float(f"{err:.3e}"),
+                                 "rel_err_rows_vs_host": float(f"{err_rows:.3e}")}
+        del A, B, C64, Cs
+        torch.cuda.empty_cache()
+        print(json.dumps(row), flush=True)
+        return row
+
+    uniform = lambda r, c, g: torch.rand((r, c), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+
+    def stressed(r, c, g):
+        x = uniform(r, c, g)
+        e = torch.randint(-20, 21, (r, c), device="cuda", generator=g).double()
+        return x * torch.exp2(e)
+
+    out = []
+    for n in sizes:
+        out.append(case(f"dgemm uniform n={n}", n, n, n, uniform))
+        out.append(case(f"dgemm stressed n={n}", n, n, n, stressed))
+    # Wabef at o=48, v=256: C (v^2 x v^2) = g_oovv^T (v^2 x o^2) t_oovv (o^2 x v^2)
+    out.append(case("Wabef o=48 v=256", 65536, 65536, 2304, lambda r, c, g: uniform(r, c, g) * 0.1))
     return out
 
 
@@ -121,6 +189,9 @@ def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.precision_sweep:
+        precision_sweep()
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of this build",
@@ -214,7 +285,7 @@ def main():
         "kernels": kernels,
     }
     if args.split:
-        out["split"] = split_comparison(core, plans, W, I, args.split)
+        out["split"] = split_comparison(core, plans, W)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(I, o, v, flops)
     print(json.dumps(out))

@@ -159,7 +159,7 @@ class EinsumGenericPlan:
         if split:
             if dtype != torch.float64:
                 raise TypeError("split precision applies to float64 operands")
-            st = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16}[slice_type]
+            st = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16, "int8": _lib.SLICE_INT8}[slice_type]
             status = lib.ein_einsum_split_f64(int(split), st, float(C_prefactor), ctypes.byref(cd), _ptr(dC),
                                               float(AB_prefactor), ctypes.byref(ad), _ptr(dA), ctypes.byref(bd), _ptr(dB),
                                               ctypes.byref(used), stream_handle(stream))
@@ -272,7 +272,7 @@ def gemm_split(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch
     m = A.shape[1] if ta else A.shape[0]
     k = A.shape[0] if ta else A.shape[1]
     n = B.shape[0] if tb else B.shape[1]
-    st = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16}[slice_type]
+    st = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16, "int8": _lib.SLICE_INT8}[slice_type]
     status = lib.ein_gemm_split_f64(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0), _ptr(A),
                                     A.stride(0), float(beta), _ptr(C), C.stride(0), int(nslices), st, stream_handle(stream))
     check(status, "gemm_split")

@@ -15,9 +15,11 @@
 // With NS = 5 the representation error is 2^-40 relative to the row/column maximum.
 #include <algorithm>
 #include <cmath>
+#include <tuple>
 
 #include "gemm.hpp"
 #include "ozaki.hpp"
+#include "tile_engine.hpp"
 
 namespace ein {
 
@@ -46,17 +48,26 @@ struct HalfMfma<_Float16> {
 
 // ----------------------------------------------------------- row exponents
 // e[r] = exponent with max_p |X(r,p)| < 2^e (0 for an all-zero row).
-__global__ __launch_bounds__(256) void row_exp_rows_fast(SliceSrc S, int32_t *__restrict__ e) {
+// With `guard`, the exponent is raised by one when the scaled maximum would exceed 127/128,
+// so that a 7-bit slice rounded to nearest stays within int8 (|slice| <= 127).
+__device__ __forceinline__ int row_exponent(double mx, bool guard) {
+    int ex = 0;
+    if (mx > 0.0) {
+        double const f = frexp(mx, &ex);
+        if (guard && f > 127.0 / 128.0) ++ex;
+    }
+    return ex;
+}
+
+__global__ __launch_bounds__(256) void row_exp_rows_fast(SliceSrc S, int32_t *__restrict__ e, bool guard) {
     int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= S.rows) return;
     double mx = 0.0;
     for (int64_t p = 0; p < S.k; ++p) mx = fmax(mx, fabs(S.X[r * S.s_row + p * S.s_k]));
-    int ex = 0;
-    if (mx > 0.0) (void)frexp(mx, &ex);
-    e[r] = ex;
+    e[r] = row_exponent(mx, guard);
 }
 
-__global__ __launch_bounds__(256) void row_exp_k_fast(SliceSrc S, int32_t *__restrict__ e) {
+__global__ __launch_bounds__(256) void row_exp_k_fast(SliceSrc S, int32_t *__restrict__ e, bool guard) {
     int const lane = threadIdx.x & 63;
     int64_t r      = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     if (r >= S.rows) return;
@@ -64,11 +75,7 @@ __global__ __launch_bounds__(256) void row_exp_k_fast(SliceSrc S, int32_t *__res
     for (int64_t p = lane; p < S.k; p += 64) mx = fmax(mx, fabs(S.X[r * S.s_row + p * S.s_k]));
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-    if (lane == 0) {
-        int ex = 0;
-        if (mx > 0.0) (void)frexp(mx, &ex);
-        e[r] = ex;
-    }
+    if (lane == 0) e[r] = row_exponent(mx, guard);
 }
 
 // ----------------------------------------------------------------- slicing
@@ -278,6 +285,281 @@ __global__ __launch_bounds__((BM / 32) * (BN / 32) * 64) void ozaki_gemm_kernel(
     }
 }
 
+// ------------------------------------------------------------ int8 slices
+// Integer variant on v_mfma_i32_16x16x64_i8. The row exponent is chosen so that
+// |A(i,:)| * 2^-e_i <= 127/128; slices carry 7 bits each, rounded to nearest:
+//     a_0 = rint(128 x) in [-127, 127], then a_t = rint(128 r_t) in [-64, 64], r in [-1/2, 1/2],
+//     A(i,p) = 2^e_i * sum_t a_t(i,p) 2^(-7(t+1)) + r,  |r| <= 2^(e_i - 7 NS - 1).
+// A diagonal d = t+u holds at most one product of two leading slices (<= 127^2, d = 0 only)
+// and otherwise products <= 127*64, so its int32 accumulator is exact while
+// (d+1) * 127 * 64 * K < 2^31: over chunks of I8_KC = 32768 contraction steps for NS <= 6.
+// Chunks are folded into FP64 accumulators with weight 2^(-7(d+2)).
+constexpr int I8_BETA = 7;
+constexpr int I8_KC   = 32768;
+constexpr int I8_KS   = 64; // contraction steps per stage (one MFMA depth)
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// 64 x 64 tiles of X staged through LDS; slices written along p (out[t][r][kp]).
+template <bool ROWS_FAST>
+__global__ __launch_bounds__(256) void slice_i8_kernel(SliceSrc S, int32_t const *__restrict__ e, int ns,
+                                                       int8_t *__restrict__ out) {
+    __shared__ double tile[64][65];
+    int64_t const r0 = int64_t(blockIdx.y) * 64;
+    int64_t const p0 = int64_t(blockIdx.x) * 64;
+    int const t      = threadIdx.x;
+    if constexpr (ROWS_FAST) {
+        int const lr = t & 63;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            int lp       = (t >> 6) + 4 * q;
+            int64_t r    = r0 + lr, p = p0 + lp;
+            tile[lr][lp] = (r < S.rows && p < S.k) ? S.X[r * S.s_row + p * S.s_k] : 0.0;
+        }
+    } else {
+        int const lp = t & 63;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            int lr       = (t >> 6) + 4 * q;
+            int64_t r    = r0 + lr, p = p0 + lp;
+            tile[lr][lp] = (r < S.rows && p < S.k) ? S.X[r * S.s_row + p * S.s_k] : 0.0;
+        }
+    }
+    __syncthreads();
+    int const lp    = t & 63;
+    int64_t const p = p0 + lp;
+    if (p >= S.kp) return;
+#pragma unroll 1
+    for (int q = 0; q < 16; ++q) {
+        int lr    = (t >> 6) + 4 * q;
+        int64_t r = r0 + lr;
+        if (r >= S.rows) continue;
+        double x = ldexp(tile[lr][lp], -e[r]);
+        for (int s = 0; s < ns; ++s) {
+            double y = ldexp(x, I8_BETA);
+            double d = rint(y);
+            x        = y - d;
+            out[(int64_t(s) * S.rows + r) * S.kp + p] = static_cast<int8_t>(d);
+        }
+    }
+}
+
+// LDS stage of one operand: NS slices x R rows x 64 int8 as 16-byte chunks; chunk c of
+// (slice s, row) sits at slot (s*R + row)*4 + (c ^ ((row >> 2) & 3)), so the 16 rows of
+// one fragment read fall on 16 distinct 16-byte bank groups.
+template <int R, int NS>
+struct I8Stage {
+    static constexpr int SLOTS = NS * R * 4;
+    static constexpr int INS   = SLOTS / 64;
+    static_assert(INS % 4 == 0, "stage instructions must split over four waves");
+    static __device__ __forceinline__ int slot(int s, int row, int c) { return (s * R + row) * 4 + (c ^ ((row >> 2) & 3)); }
+};
+
+// Stage DMA: lane l of instruction n fills slot 64n + l.
+template <int R, int NS>
+__device__ __forceinline__ void i8_dma(int8_t const *__restrict__ X, int64_t rows_total, int64_t kp, int64_t row0,
+                                       int64_t k0, int8_t *lds, int wave, int lane) {
+    using St = I8Stage<R, NS>;
+#pragma unroll
+    for (int x = 0; x < St::INS / 4; ++x) {
+        int const n    = wave + 4 * x;
+        int const sl   = n * 64 + lane;
+        int const srow = sl / 4; // s*R + row
+        int const s    = srow / R;
+        int const row  = srow % R;
+        int const c    = (sl % 4) ^ ((row >> 2) & 3);
+        int64_t const g = row0 + row;
+        void const *src = g < rows_total ? static_cast<void const *>(X + (int64_t(s) * rows_total + g) * kp + k0 + 16 * c)
+                                         : static_cast<void const *>(ein_zero_page);
+        __builtin_amdgcn_global_load_lds(src, EIN_LDS_PTR(lds + n * 64 * 16), 16, 0, 0);
+    }
+}
+
+// C(i,j) = alpha * 2^(e_i+f_j) * sum_d 2^(-7(d+2)) D_d(i,j) + beta * C(i,j); workgroup tile
+// 128 x 64, waves of 64 x 32 (4 x 2 blocks of 16 x 16 x 64).
+// MULTI: the contraction spans several exact-accumulation chunks (kp > I8_KC), so the int32
+// diagonals are folded into FP64 accumulators after every chunk.
+template <int NS, bool MULTI>
+__global__ __launch_bounds__(256, 1) void ozaki_i8_kernel(OzProblem P) {
+    constexpr int BM = 128, BN = 64, FM = 4, FN = 2;
+    using SA = I8Stage<BM, NS>;
+    using SB = I8Stage<BN, NS>;
+    __shared__ __attribute__((aligned(16))) int8_t lds[2 * (SA::SLOTS + SB::SLOTS) * 16];
+
+    int const tid  = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int const wm   = wave & 1;
+    int const wn   = wave >> 1;
+    int const r    = lane & 15;
+    int const q    = lane >> 4;
+
+    uint32_t const ntiles  = P.tiles_m * P.tiles_n;
+    uint32_t const logical = xcd_remap(blockIdx.x, ntiles);
+    uint32_t tm, tn;
+    tile_coords(logical, P.tiles_m, P.tiles_n, tm, tn);
+    int64_t const i0 = int64_t(tm) * BM;
+    int64_t const j0 = int64_t(tn) * BN;
+    int8_t const *__restrict__ Ag = static_cast<int8_t const *>(P.As_raw);
+    int8_t const *__restrict__ Bg = static_cast<int8_t const *>(P.Bs_raw);
+
+    v4i acc[NS][FM][FN];
+    v4d accd[MULTI ? FM : 1][MULTI ? FN : 1];
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+            if constexpr (MULTI) accd[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int d = 0; d < NS; ++d) acc[d][a][b] = v4i{0, 0, 0, 0};
+        }
+    // weighted sum of the diagonals of block (a, b), element v
+    auto diag_sum = [&](int a, int b, int v) __attribute__((always_inline)) {
+        double x = 0.0;
+#pragma unroll
+        for (int d = NS - 1; d >= 0; --d) x += double(acc[d][a][b][v]) * ldexp(1.0, -I8_BETA * (d + 2));
+        return x;
+    };
+
+    int64_t const nst = P.kp / I8_KS;
+    auto stage_ptr    = [&](int buf) { return lds + buf * (SA::SLOTS + SB::SLOTS) * 16; };
+    if (nst > 0) {
+        i8_dma<BM, NS>(Ag, P.m, P.kp, i0, 0, stage_ptr(0), wave, lane);
+        i8_dma<BN, NS>(Bg, P.n, P.kp, j0, 0, stage_ptr(0) + SA::SLOTS * 16, wave, lane);
+    }
+    for (int64_t st = 0; st < nst; ++st) {
+        int const buf = int(st & 1);
+        dma_wait_all();
+        if (st + 1 < nst) {
+            i8_dma<BM, NS>(Ag, P.m, P.kp, i0, (st + 1) * I8_KS, stage_ptr(buf ^ 1), wave, lane);
+            i8_dma<BN, NS>(Bg, P.n, P.kp, j0, (st + 1) * I8_KS, stage_ptr(buf ^ 1) + SA::SLOTS * 16, wave, lane);
+        }
+        v4i const *As = reinterpret_cast<v4i const *>(stage_ptr(buf));
+        v4i const *Bs = As + SA::SLOTS;
+        v4i bfr[NS][FN];
+#pragma unroll
+        for (int u = 0; u < NS; ++u)
+#pragma unroll
+            for (int b = 0; b < FN; ++b) bfr[u][b] = Bs[SB::slot(u, wn * 32 + b * 16 + r, q)];
+#pragma unroll
+        for (int t = 0; t < NS; ++t) {
+            v4i af[FM];
+#pragma unroll
+            for (int a = 0; a < FM; ++a) af[a] = As[SA::slot(t, wm * 64 + a * 16 + r, q)];
+#pragma unroll
+            for (int u = 0; u < NS - t; ++u)
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b)
+                        acc[t + u][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bfr[u][b], af[a], acc[t + u][a][b], 0, 0, 0);
+        }
+        if constexpr (MULTI) {
+            if (((st + 1) * I8_KS) % I8_KC == 0 && st + 1 < nst) {
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            accd[a][b][v] += diag_sum(a, b, v);
+#pragma unroll
+                            for (int d = 0; d < NS; ++d) acc[d][a][b][v] = 0;
+                        }
+            }
+        }
+    }
+
+    // epilogue: result layout D[4*(lane/16)+v][lane%16]; rows of D run along j
+#pragma unroll
+    for (int a = 0; a < FM; ++a) {
+        int64_t i = i0 + wm * 64 + a * 16 + r;
+        if (i >= P.m) continue;
+        int const ei = P.ea[i];
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int64_t j = j0 + wn * 32 + b * 16 + q * 4 + v;
+                if (j >= P.n) continue;
+                double *cp  = P.C + i * P.c_sm + j * P.c_sn;
+                double tot  = diag_sum(a, b, v);
+                if constexpr (MULTI) tot += accd[a][b][v];
+                double val = P.alpha * ldexp(tot, ei + P.eb[j]);
+                if (P.beta != 0.0) val += P.beta * *cp;
+                *cp = val;
+            }
+    }
+}
+
+template <int NS>
+void launch_i8(OzProblem const &Q, dim3 grid, hipStream_t stream) {
+    if (Q.kp > I8_KC)
+        ozaki_i8_kernel<NS, true><<<grid, 256, 0, stream>>>(Q);
+    else
+        ozaki_i8_kernel<NS, false><<<grid, 256, 0, stream>>>(Q);
+}
+
+int gemm_split_i8(GemmProblem<double> P, int ns, hipStream_t stream) {
+    int64_t const kp      = ceil_div(P.k, I8_KS) * I8_KS;
+    size_t const a_bytes  = size_t(ns) * size_t(P.m) * size_t(kp);
+    size_t const b_bytes  = size_t(ns) * size_t(P.n) * size_t(kp);
+    size_t const b_off    = (a_bytes + 255) / 256 * 256;
+    size_t const ea_off   = b_off + (b_bytes + 255) / 256 * 256;
+    size_t const eb_off   = ea_off + (size_t(P.m) * 4 + 255) / 256 * 256;
+    size_t const total    = eb_off + (size_t(P.n) * 4 + 255) / 256 * 256;
+    void *ws = nullptr;
+    EIN_TRY(workspace(stream, total, &ws));
+    char *base  = static_cast<char *>(ws);
+    int8_t *As  = reinterpret_cast<int8_t *>(base);
+    int8_t *Bs  = reinterpret_cast<int8_t *>(base + b_off);
+    int32_t *ea = reinterpret_cast<int32_t *>(base + ea_off);
+    int32_t *eb = reinterpret_cast<int32_t *>(base + eb_off);
+    SliceSrc SA{P.m, P.k, kp, P.A, P.a_sm, P.a_sk};
+    SliceSrc SB{P.n, P.k, kp, P.B, P.b_sn, P.b_sk};
+    for (auto const &[S, e, out] : {std::tuple<SliceSrc, int32_t *, int8_t *>{SA, ea, As}, {SB, eb, Bs}}) {
+        bool const rows_fast = S.s_row == 1 || (S.s_k != 1 && std::llabs(S.s_row) < std::llabs(S.s_k));
+        if (rows_fast)
+            row_exp_rows_fast<<<uint32_t(ceil_div(S.rows, 256)), 256, 0, stream>>>(S, e, true);
+        else
+            row_exp_k_fast<<<uint32_t(ceil_div(S.rows, 4)), 256, 0, stream>>>(S, e, true);
+        EIN_LAUNCH_CHECK();
+        dim3 grid{uint32_t(ceil_div(S.kp, 64)), uint32_t(ceil_div(S.rows, 64))};
+        if (grid.y > 65535) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: more than 4M rows");
+        if (rows_fast)
+            slice_i8_kernel<true><<<grid, 256, 0, stream>>>(S, e, ns, out);
+        else
+            slice_i8_kernel<false><<<grid, 256, 0, stream>>>(S, e, ns, out);
+        EIN_LAUNCH_CHECK();
+    }
+    OzProblem Q{};
+    Q.m       = P.m;
+    Q.n       = P.n;
+    Q.kp      = kp;
+    Q.As_raw  = As;
+    Q.Bs_raw  = Bs;
+    Q.ea      = ea;
+    Q.eb      = eb;
+    Q.C       = P.C;
+    Q.c_sm    = P.c_sm;
+    Q.c_sn    = P.c_sn;
+    Q.alpha   = P.alpha;
+    Q.beta    = P.beta;
+    Q.tiles_m = uint32_t(ceil_div(P.m, 128));
+    Q.tiles_n = uint32_t(ceil_div(P.n, 64));
+    if (uint64_t(Q.tiles_m) * Q.tiles_n > 0x7fffffffu) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: tile grid too large");
+    dim3 grid{Q.tiles_m * Q.tiles_n};
+    switch (ns) {
+    case 3: launch_i8<3>(Q, grid, stream); break;
+    case 4: launch_i8<4>(Q, grid, stream); break;
+    case 5: launch_i8<5>(Q, grid, stream); break;
+    case 6: launch_i8<6>(Q, grid, stream); break;
+    default: return fail(EIN_ERROR_VALUE, "split gemm: int8 slices must number 3..6");
+    }
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
 template <typename H, int NS>
 void launch_oz(OzProblem const &P, int bm, hipStream_t stream) {
     if constexpr (NS <= 5) {
@@ -297,9 +579,9 @@ template <typename H>
 int slice_operand(SliceSrc const &S, int ns, int32_t *e, H *out, hipStream_t stream) {
     bool const rows_fast = S.s_row == 1 || (S.s_k != 1 && std::llabs(S.s_row) < std::llabs(S.s_k));
     if (rows_fast) {
-        row_exp_rows_fast<<<uint32_t(ceil_div(S.rows, 256)), 256, 0, stream>>>(S, e);
+        row_exp_rows_fast<<<uint32_t(ceil_div(S.rows, 256)), 256, 0, stream>>>(S, e, false);
     } else {
-        row_exp_k_fast<<<uint32_t(ceil_div(S.rows, 4)), 256, 0, stream>>>(S, e);
+        row_exp_k_fast<<<uint32_t(ceil_div(S.rows, 4)), 256, 0, stream>>>(S, e, false);
     }
     EIN_LAUNCH_CHECK();
     dim3 grid{uint32_t(ceil_div(S.kp, 64)), uint32_t(ceil_div(S.rows, 64))};
@@ -392,9 +674,10 @@ int gemm_split(GemmProblem<double> P, int nslices, int slice_type, hipStream_t s
     }
     int64_t const t128 = ceil_div(P.m, 128) * ceil_div(P.n, 64);
     int const bm = t128 >= int64_t(num_cus()) * 2 ? 128 : 64;
+    if (slice_type == EIN_SLICE_INT8) return gemm_split_i8(P, nslices, stream);
     if (slice_type == EIN_SLICE_BF16) return gemm_split_typed<__bf16>(P, nslices, bm, stream);
     if (slice_type == EIN_SLICE_FP16) return gemm_split_typed<_Float16>(P, nslices, bm, stream);
-    return fail(EIN_ERROR_VALUE, "split gemm: slice_type must be EIN_SLICE_BF16 or EIN_SLICE_FP16");
+    return fail(EIN_ERROR_VALUE, "split gemm: slice_type must be EIN_SLICE_BF16, EIN_SLICE_FP16 or EIN_SLICE_INT8");
 }
 
 } // namespace ein

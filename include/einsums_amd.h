@@ -119,13 +119,18 @@ int ein_axpby_f64(int64_t n, double alpha, const double *x, int64_t incx, double
 int ein_axpby_f32(int64_t n, float alpha, const float *x, int64_t incx, float beta, float *y, int64_t incy,
                   ein_stream_t stream);
 
-/* Split-precision FP64 GEMM (Ozaki scheme): op(A), op(B) are cut into `nslices` (3..6)
- * BF16 or FP16 integer slices of 8 bits with per-row / per-column power-of-two scales,
- * multiplied on MFMA with exact FP32 accumulation and recombined in FP64. Relative
- * representation error 2^-(8*nslices). Same argument order and checks as ein_gemm_f64;
- * no counterpart in the reference (its GEMM leaf is hipblasDgemm). */
+/* Split-precision FP64 GEMM (Ozaki scheme): op(A), op(B) are cut into `nslices` integer
+ * slices with per-row / per-column power-of-two scales, multiplied on MFMA with exact
+ * accumulation and recombined in FP64.
+ *   EIN_SLICE_BF16 / EIN_SLICE_FP16: 8-bit slices (nslices 3..6) on BF16/FP16 MFMA with FP32
+ *     accumulation; relative representation error 2^-(8*nslices).
+ *   EIN_SLICE_INT8: 7-bit slices (nslices 3..6) on int8 MFMA with int32 accumulation;
+ *     relative representation error 2^-(7*nslices).
+ * Same argument order and checks as ein_gemm_f64; no counterpart in the reference (its GEMM
+ * leaf is hipblasDgemm). */
 #define EIN_SLICE_BF16 0
 #define EIN_SLICE_FP16 1
+#define EIN_SLICE_INT8 2
 int ein_gemm_split_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
                        const double *B, int64_t ldb, double beta, double *C, int64_t ldc, int32_t nslices,
                        int32_t slice_type, ein_stream_t stream);

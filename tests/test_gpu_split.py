@@ -1,9 +1,10 @@
-"""Split-precision (Ozaki) FP64 GEMM on BF16/FP16 MFMA against the FP64 CPU oracle.
+"""Split-precision (Ozaki) FP64 GEMM on BF16/FP16 and int8 MFMA against the FP64 CPU oracle.
 
 Tolerances (relative Frobenius error against the FP64 oracle):
-  nslices=5: <= 1e-11 for U(-1,1) data and the CCD contractions,
+  BF16/FP16 nslices=5 (40 bits) and int8 nslices=6 (42 bits):
+             <= 1e-11 for U(-1,1) data and the CCD contractions,
              <= 1e-10 for the stressed set U(-1,1)*2^e, e in [-20, 20];
-  nslices=3: <= 1e-6 (24-bit representation);
+  BF16 nslices=3 (24 bits): <= 1e-6; int8 nslices=3 (21 bits): <= 1e-5;
   integer-valued known answers: exact.
 """
 import ctypes
@@ -31,7 +32,7 @@ def seeded(*parts):
     return np.random.default_rng(zlib.crc32("|".join(map(str, parts)).encode()))
 
 
-@pytest.mark.parametrize("slice_type", [0, 1])
+@pytest.mark.parametrize("slice_type", [0, 1, 2])
 @pytest.mark.parametrize("trans", ["NN", "TN", "NT", "TT"])
 def test_split_gemm_3x3_known_answers(slice_type, trans):
     from einsums_amd import _lib
@@ -40,7 +41,7 @@ def test_split_gemm_3x3_known_answers(slice_type, trans):
     A, B = dev(np.array(g["A"], dtype=np.float64)), dev(np.array(g["B"], dtype=np.float64))
     C = dev(np.full(9, np.nan))
     _lib.check(_lib.lib.ein_gemm_split_f64(trans[1].encode(), trans[0].encode(), 3, 3, 3, 1.0, vp(B), 3, vp(A), 3, 0.0,
-                                           vp(C), 3, 5, slice_type, None))
+                                           vp(C), 3, 6 if slice_type == 2 else 5, slice_type, None))
     assert host(C).tolist() == g["expected"][trans]
 
 
@@ -49,8 +50,9 @@ SHAPES = [(1, 1, 1), (5, 7, 3), (64, 64, 32), (100, 130, 257), (257, 129, 1000),
 
 @pytest.mark.parametrize("m,n,k", SHAPES)
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
-@pytest.mark.parametrize("nslices,tol", [(5, 1e-11), (3, 1e-6)])
-def test_split_gemm_vs_oracle(oracle, m, n, k, ta, tb, nslices, tol):
+@pytest.mark.parametrize("slice_type,nslices,tol", [(0, 5, 1e-11), (0, 3, 1e-6), (2, 6, 1e-11), (2, 3, 1e-5)],
+                         ids=["bf16x5", "bf16x3", "int8x6", "int8x3"])
+def test_split_gemm_vs_oracle(oracle, m, n, k, ta, tb, slice_type, nslices, tol):
     from einsums_amd import _lib
 
     rng = seeded(m, n, k, ta, tb)
@@ -64,16 +66,16 @@ def test_split_gemm_vs_oracle(oracle, m, n, k, ta, tb, nslices, tol):
     oracle.gemm_colmajor(ta, tb, m, n, k, 0.75, A, lda, B, ldb, -0.5, ref, ldc)
     dA, dB, dC = dev(A), dev(B), dev(C0)
     _lib.check(_lib.lib.ein_gemm_split_f64(ta.encode(), tb.encode(), m, n, k, 0.75, vp(dA), lda, vp(dB), ldb, -0.5, vp(dC),
-                                           ldc, nslices, 0, None))
+                                           ldc, nslices, slice_type, None))
     got = host(dC).reshape(n, ldc)
     ref = ref.reshape(n, ldc)
     assert np.array_equal(got[:, m:], C0.reshape(n, ldc)[:, m:])
     err = rel_fro(got[:, :m], ref[:, :m])
-    print(f"split{nslices} {m}x{n}x{k} {ta}{tb}: rel err {err:.3e}")
+    print(f"split type {slice_type} x{nslices} {m}x{n}x{k} {ta}{tb}: rel err {err:.3e}")
     assert err <= tol
 
 
-@pytest.mark.parametrize("slice_type", ["bf16", "fp16"])
+@pytest.mark.parametrize("slice_type", ["bf16", "fp16", "int8"])
 @pytest.mark.parametrize("n", [1024, 2048])
 def test_split_dgemm_uniform_and_stressed(core, oracle, slice_type, n):
     rng = seeded("stress", n)
@@ -88,14 +90,16 @@ def test_split_dgemm_uniform_and_stressed(core, oracle, slice_type, n):
         oracle.gemm_colmajor("N", "N", n, n, n, 1.0, np.ascontiguousarray(B).ravel(), n, np.ascontiguousarray(A).ravel(),
                              n, 0.0, ref, n)
         dC = torch.zeros((n, n), dtype=torch.float64, device="cuda")
-        core.gemm_split("N", "N", 1.0, dev(A), dev(B), 0.0, dC, nslices=5, slice_type=slice_type)
+        ns = 6 if slice_type == "int8" else 5
+        core.gemm_split("N", "N", 1.0, dev(A), dev(B), 0.0, dC, nslices=ns, slice_type=slice_type)
         err = rel_fro(host(dC).ravel(), ref)
         print(f"{slice_type} n={n} stressed={stressed}: rel err {err:.3e}")
         assert err <= (1e-10 if stressed else 1e-11)
 
 
+@pytest.mark.parametrize("slice_type,nslices", [("bf16", 5), ("int8", 6)])
 @pytest.mark.parametrize("name", ["Wmnij", "Wabef"])
-def test_split_ccd_contractions(core, oracle, name):
+def test_split_ccd_contractions(core, oracle, name, slice_type, nslices):
     I = workloads.ccd_inputs(24, 96, seed=7)
     spec = {c[0]: c for c in workloads.CCD_CONTRACTIONS}[name]
     _, Ci, An, Ai, Bn, Bi, cp, abp, init = spec
@@ -104,9 +108,10 @@ def test_split_ccd_contractions(core, oracle, name):
     ref = W0.copy()
     oracle.einsum(cp, Ci, ref, abp, Ai, A, Bi, B)
     dW = dev(W0)
-    alg = core.einsum(Ci, dW, Ai, dev(A), Bi, dev(B), C_prefactor=cp, AB_prefactor=abp, split=5)
+    alg = core.einsum(Ci, dW, Ai, dev(A), Bi, dev(B), C_prefactor=cp, AB_prefactor=abp, split=nslices,
+                      slice_type=slice_type)
     assert core.algorithm_name(alg) == "GEMM"
     # error of the contraction part alone (W0 is added exactly in FP64)
     err = rel_fro(host(dW) - W0, ref - W0)
-    print(f"{name} split5 rel err of the contraction {err:.3e}")
+    print(f"{name} {slice_type} x{nslices} rel err of the contraction {err:.3e}")
     assert err <= 1e-11
