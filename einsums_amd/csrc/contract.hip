@@ -140,15 +140,15 @@ constexpr int64_t NO_ROW = INT64_MIN;
 
 // One tile (or the slab range [s0, s1) of it) of a gather-GEMM problem. `As`/`Bs` are
 // the workgroup's double-buffered LDS slabs (layouts of tile_engine.hpp; AKG / BKG pick
-// the k-grouped layout for an operand whose unit stride runs along k); with
+// the row-major (KROW) layout for an operand whose unit stride runs along k); with
 // `piece` != nullptr the raw accumulators go there ([BN][BM], i fastest) instead of
 // through the epilogue into C.
-template <typename T, int BM, int BN, int BK, bool AKG, bool BKG>
+template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW>
 __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64_t t, uint64_t s0, uint64_t s1, T *As,
                                                T *Bs, T *piece) {
     using Cfg       = TileCfg<T, BM, BN, BK>;
-    using LA        = SlabLayout<T, BM, BK, AKG, false>;
-    using LB        = SlabLayout<T, BN, BK, BKG, false>;
+    using LA        = SlabLayout<T, BM, BK, AKG, AW>;
+    using LB        = SlabLayout<T, BN, BK, BKG, BW>;
     int const tid   = threadIdx.x;
     int const lane  = tid & 63;
     int const wave  = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -171,45 +171,32 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
     T const *__restrict__ Ab = P.A + boff[0];
     T const *__restrict__ Bb = P.B + boff[1];
 
-    // element offsets through the index groups; a lane touches at most two tile rows, so
-    // the row decodes are loop invariant
+    // element offsets from the per-call tables (one load each, cached)
     auto offA = [&](int row, int64_t p, bool &valid) -> int64_t {
-        bool const pv = p < kend;
-        int64_t ok[2];
-        group_decode<2>(uint32_t(pv ? p : 0), P.gk, ok);
         int64_t const i = i0 + row;
-        bool const iv   = i < P.m;
-        int64_t om[2];
-        group_decode<2>(uint32_t(iv ? i : 0), P.gm, om);
-        valid = pv && iv;
-        return om[0] + ok[0];
+        valid           = i < P.m && p < kend;
+        return valid ? P.rowA[i] + P.kA[p] : 0;
     };
     auto offB = [&](int row, int64_t p, bool &valid) -> int64_t {
-        bool const pv = p < kend;
-        int64_t ok[2];
-        group_decode<2>(uint32_t(pv ? p : 0), P.gk, ok);
         int64_t const j = j0 + row;
-        bool const jv   = j < P.n;
-        int64_t on[2];
-        group_decode<2>(uint32_t(jv ? j : 0), P.gn, on);
-        valid = pv && jv;
-        return on[0] + ok[1];
+        valid           = j < P.n && p < kend;
+        return valid ? P.colB[j] + P.kB[p] : 0;
     };
 
     WaveAcc<T, Cfg::FM, Cfg::FN> acc;
     acc.zero();
     int64_t const nslabs = int64_t(s1 - s0);
     __syncthreads(); // LDS of the previous segment no longer read
-    dma_slab<T, BM, BK, AKG, false>(Ab, As, wave, lane, kbeg, offA);
-    dma_slab<T, BN, BK, BKG, false>(Bb, Bs, wave, lane, kbeg, offB);
+    dma_slab<T, BM, BK, AKG, AW>(Ab, As, wave, lane, kbeg, offA);
+    dma_slab<T, BN, BK, BKG, BW>(Bb, Bs, wave, lane, kbeg, offB);
     for (int64_t sl = 0; sl < nslabs; ++sl) {
         int const buf = int(sl & 1);
         dma_wait_all();
         if (sl + 1 < nslabs) {
-            dma_slab<T, BM, BK, AKG, false>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
-            dma_slab<T, BN, BK, BKG, false>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
+            dma_slab<T, BM, BK, AKG, AW>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
+            dma_slab<T, BN, BK, BKG, BW>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
         }
-        mfma_slab_lds<T, BM, BN, BK, AKG, false, BKG, false>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, acc, wm, wn, lane);
+        mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, acc, wm, wn, lane);
     }
 
     // lane coordinates made opaque so the store addresses are not hoisted out of the
@@ -234,33 +221,36 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
 #pragma unroll
     for (int a = 0; a < Cfg::FM; ++a) {
         int64_t i = i0 + ri + a * 16;
-        int64_t o[2];
-        group_decode<2>(uint32_t(i < P.m ? i : 0), P.gm, o);
-        rowC[a] = i < P.m ? o[1] : NO_ROW;
+        rowC[a]   = i < P.m ? P.rowC[i] : NO_ROW;
     }
+    // C reads of one column block are issued together (one latency per block)
 #pragma unroll
-    for (int b = 0; b < Cfg::FN; ++b)
+    for (int b = 0; b < Cfg::FN; ++b) {
+        int64_t colj[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             int64_t j = j0 + rj + b * 16 + Mfma<T>::row(q, v);
-            if (j >= P.n) continue;
-            int64_t o[2];
-            group_decode<2>(uint32_t(j), P.gn, o);
-#pragma unroll
-            for (int a = 0; a < Cfg::FM; ++a) {
-                if (rowC[a] == NO_ROW) continue;
-                T *cp = Cb + rowC[a] + o[1];
-                T val = P.alpha * acc.c[a][b][v];
-                if (P.beta != T(0)) val += P.beta * *cp;
-                *cp = val;
-            }
+            colj[v]   = j < P.n ? P.colC[j] : NO_ROW;
         }
+        T cv[Cfg::FM][4];
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                cv[a][v] = (P.beta != T(0) && rowC[a] != NO_ROW && colj[v] != NO_ROW) ? Cb[rowC[a] + colj[v]] : T(0);
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (rowC[a] != NO_ROW && colj[v] != NO_ROW)
+                    Cb[rowC[a] + colj[v]] = P.alpha * acc.c[a][b][v] + P.beta * cv[a][v];
+    }
 }
 
-template <typename T, int BM, int BN, int BK, bool AKG, bool BKG>
+template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW>
 __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProblem<T> P) {
-    __shared__ __attribute__((aligned(16))) T As[2 * SlabLayout<T, BM, BK, AKG, false>::SIZE];
-    __shared__ __attribute__((aligned(16))) T Bs[2 * SlabLayout<T, BN, BK, BKG, false>::SIZE];
+    __shared__ __attribute__((aligned(16))) T As[2 * SlabLayout<T, BM, BK, AKG, AW>::SIZE];
+    __shared__ __attribute__((aligned(16))) T Bs[2 * SlabLayout<T, BN, BK, BKG, BW>::SIZE];
     StreamK const &sk = P.sk;
     uint64_t const S  = sk.slabs;
     uint64_t u0, u1, wlog;
@@ -271,7 +261,7 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProb
         uint64_t const s1 = (S - s0 < u1 - u) ? S : s0 + (u1 - u);
         u += s1 - s0;
         T *piece = (s0 == 0 && s1 == S) ? nullptr : P.partial + sk_slot(sk, wlog, t) * (BM * BN);
-        gather_segment<T, BM, BN, BK, AKG, BKG>(P, t, s0, s1, As, Bs, piece);
+        gather_segment<T, BM, BN, BK, AKG, AW, BKG, BW>(P, t, s0, s1, As, Bs, piece);
     }
 }
 
@@ -297,10 +287,7 @@ __global__ __launch_bounds__(256) void gather_streamk_fixup(GatherProblem<T> P) 
         if (i >= P.m || j >= P.n) continue;
         T sum = T(0);
         for (uint64_t g = g0; g <= g1; ++g) sum += P.partial[sk_slot(sk, g, t) * (BM * BN) + e];
-        int64_t om[2], on[2];
-        group_decode<2>(uint32_t(i), P.gm, om);
-        group_decode<2>(uint32_t(j), P.gn, on);
-        T *cp = P.C + ob[2] + om[1] + on[1];
+        T *cp = P.C + ob[2] + P.rowC[i] + P.colC[j];
         T val = P.alpha * sum;
         if (P.beta != T(0)) val += P.beta * *cp;
         *cp = val;
@@ -418,17 +405,100 @@ __global__ __launch_bounds__(256) void reduce_finish_targets_kernel(ReduceProble
 
 namespace {
 
-template <typename T, int BM, int BN>
-void launch_gather(GatherProblem<T> const &P, bool a_i_fast, bool b_j_fast, dim3 grid, hipStream_t stream) {
+template <typename T, int BM, int BN, bool AKG, bool AW, bool BKG, bool BW>
+void const *gather_kernel_ptr() {
     constexpr int BK = 16;
-    if (a_i_fast && b_j_fast)
-        gemm_gather_kernel<T, BM, BN, BK, false, false><<<grid, GEMM_THREADS, 0, stream>>>(P);
-    else if (a_i_fast)
-        gemm_gather_kernel<T, BM, BN, BK, false, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
-    else if (b_j_fast)
-        gemm_gather_kernel<T, BM, BN, BK, true, false><<<grid, GEMM_THREADS, 0, stream>>>(P);
+    if constexpr (SlabLayout<T, BM, BK, AKG, AW>::VALID && SlabLayout<T, BN, BK, BKG, BW>::VALID)
+        return reinterpret_cast<void const *>(&gemm_gather_kernel<T, BM, BN, BK, AKG, AW, BKG, BW>);
     else
-        gemm_gather_kernel<T, BM, BN, BK, true, true><<<grid, GEMM_THREADS, 0, stream>>>(P);
+        return nullptr;
+}
+
+template <typename T, int BM, int BN>
+void const *gather_kernel_for(OperandStaging sa, OperandStaging sb) {
+    switch ((sa.krow ? 8 : 0) | (sa.wide ? 4 : 0) | (sb.krow ? 2 : 0) | (sb.wide ? 1 : 0)) {
+    case 0: return gather_kernel_ptr<T, BM, BN, false, false, false, false>();
+    case 1: return gather_kernel_ptr<T, BM, BN, false, false, false, true>();
+    case 2: return gather_kernel_ptr<T, BM, BN, false, false, true, false>();
+    case 3: return gather_kernel_ptr<T, BM, BN, false, false, true, true>();
+    case 4: return gather_kernel_ptr<T, BM, BN, false, true, false, false>();
+    case 5: return gather_kernel_ptr<T, BM, BN, false, true, false, true>();
+    case 6: return gather_kernel_ptr<T, BM, BN, false, true, true, false>();
+    case 7: return gather_kernel_ptr<T, BM, BN, false, true, true, true>();
+    case 8: return gather_kernel_ptr<T, BM, BN, true, false, false, false>();
+    case 9: return gather_kernel_ptr<T, BM, BN, true, false, false, true>();
+    case 10: return gather_kernel_ptr<T, BM, BN, true, false, true, false>();
+    case 11: return gather_kernel_ptr<T, BM, BN, true, false, true, true>();
+    case 12: return gather_kernel_ptr<T, BM, BN, true, true, false, false>();
+    case 13: return gather_kernel_ptr<T, BM, BN, true, true, false, true>();
+    case 14: return gather_kernel_ptr<T, BM, BN, true, true, true, false>();
+    default: return gather_kernel_ptr<T, BM, BN, true, true, true, true>();
+    }
+}
+
+// 16-byte DMA for one gathered operand: the chunk runs along the innermost digit of `fast`
+// (operand index fop), which must have stride +1 and an extent that is a multiple of the
+// chunk; every other stride of the operand must keep chunk starts 16-byte aligned.
+template <typename T>
+bool gather_wide_ok(std::vector<Digit> const &fast, int fop, std::vector<Digit> const &other, int oop,
+                    std::vector<Digit> const &batch, int bop, void const *ptr) {
+    constexpr int64_t KG = 16 / int64_t(sizeof(T));
+    if (fast.empty() || fast.back().s[fop] != 1 || fast.back().e % KG != 0) return false;
+    for (size_t d = 0; d + 1 < fast.size(); ++d)
+        if (fast[d].s[fop] % KG != 0) return false;
+    for (auto const &d : other)
+        if (d.s[oop] % KG != 0) return false;
+    for (auto const &d : batch)
+        if (d.s[bop] % KG != 0) return false;
+    return reinterpret_cast<uintptr_t>(ptr) % 16 == 0;
+}
+
+__device__ __forceinline__ void fill_offsets(IndexGroup const &G, int64_t total, int64_t *__restrict__ o0,
+                                             int64_t *__restrict__ o1) {
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < total; i += int64_t(gridDim.x) * 256) {
+        int64_t o[2];
+        group_decode<2>(uint32_t(i), G, o);
+        o0[i] = o[0];
+        o1[i] = o[1];
+    }
+}
+
+// Element offsets of every index of the M, N and K groups (blockIdx.y selects the group),
+// laid out as gather_table_entries describes.
+__global__ __launch_bounds__(256) void group_tables_kernel(IndexGroup gm, IndexGroup gn, IndexGroup gk, int64_t m,
+                                                           int64_t n, int64_t k, int64_t *__restrict__ tab) {
+    if (blockIdx.y == 0)
+        fill_offsets(gm, m, tab, tab + m);
+    else if (blockIdx.y == 1)
+        fill_offsets(gn, n, tab + 2 * m, tab + 2 * m + n);
+    else
+        fill_offsets(gk, k, tab + 2 * (m + n), tab + 2 * (m + n) + k);
+}
+
+// Host version of the same tables (grouped plans build them once at creation).
+void host_offsets(IndexGroup const &G, int64_t total, int64_t *o0, int64_t *o1) {
+    for (int64_t i = 0; i < total; ++i) {
+        uint64_t idx = uint64_t(i);
+        int64_t a = 0, b = 0;
+        for (int d = G.n - 1; d >= 0; --d) {
+            uint64_t const r = idx % G.ext[d];
+            idx /= G.ext[d];
+            a += int64_t(r) * G.s[0][d];
+            b += int64_t(r) * G.s[1][d];
+        }
+        o0[i] = a;
+        o1[i] = b;
+    }
+}
+
+template <typename T>
+void set_tables(GatherProblem<T> &P, int64_t const *tab) {
+    P.rowA = tab;
+    P.rowC = tab + P.m;
+    P.colB = tab + 2 * P.m;
+    P.colC = tab + 2 * P.m + P.n;
+    P.kA   = tab + 2 * (P.m + P.n);
+    P.kB   = tab + 2 * (P.m + P.n) + P.k;
 }
 
 } // namespace
@@ -508,19 +578,32 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         P.tiles_n        = uint32_t(ceil_div(N, BN));
         uint64_t tiles   = uint64_t(P.tiles_m) * P.tiles_n * uint64_t(NB);
         if (tiles > 0x7fffffffu) return fail(EIN_ERROR_UNSUPPORTED, "contraction tile grid too large");
-        P.sk      = plan_stream_k(tiles, uint64_t(ceil_div(K, 16)), uint64_t(num_cus()) * 2);
-        P.partial = nullptr;
-        if (P.sk.units != 0) {
-            void *ws = nullptr;
-            EIN_TRY(workspace(stream, size_t(sk_piece_slots(P.sk)) * size_t(bm) * BN * sizeof(T), &ws));
-            P.partial = static_cast<T *>(ws);
+        OperandStaging sa{!a_fast, false}, sb{!b_j_fast, false};
+        constexpr int EW = int(sizeof(T)) / 4;
+        sa.wide = a_fast ? (bm * EW >= 256 && gather_wide_ok<T>(gm, 0, gk, 0, gbv, 0, a))
+                         : gather_wide_ok<T>(gk, 0, gm, 0, gbv, 0, a);
+        sb.wide = b_j_fast ? (BN * EW >= 256 && gather_wide_ok<T>(gn, 0, gk, 1, gbv, 1, b))
+                           : gather_wide_ok<T>(gk, 1, gn, 0, gbv, 1, b);
+        void const *kern = bm == 128 ? gather_kernel_for<T, 128, 64>(sa, sb) : gather_kernel_for<T, 64, 64>(sa, sb);
+        if (kern == nullptr) return fail(EIN_ERROR_UNSUPPORTED, "contraction: operand staging not realisable for this tile");
+        P.sk = plan_stream_k(tiles, uint64_t(ceil_div(K, 16)),
+                             uint64_t(num_cus()) * uint64_t(resident_blocks(kern, GEMM_THREADS)));
+        // workspace: offset tables, then the stream-K pieces
+        size_t const tab_bytes   = (gather_table_entries(M, N, K) * sizeof(int64_t) + 255) / 256 * 256;
+        size_t const piece_bytes = size_t(sk_piece_slots(P.sk)) * size_t(bm) * BN * sizeof(T);
+        void *ws                 = nullptr;
+        EIN_TRY(workspace(stream, tab_bytes + piece_bytes, &ws));
+        int64_t *tab = static_cast<int64_t *>(ws);
+        set_tables(P, tab);
+        P.partial = P.sk.units != 0 ? reinterpret_cast<T *>(static_cast<char *>(ws) + tab_bytes) : nullptr;
+        {
+            int64_t const mx = std::max(M, std::max(N, K));
+            dim3 tgrid{uint32_t(std::min<int64_t>(ceil_div(mx, 256), 4096)), 3};
+            group_tables_kernel<<<tgrid, 256, 0, stream>>>(P.gm, P.gn, P.gk, M, N, K, tab);
+            EIN_LAUNCH_CHECK();
         }
-        dim3 grid{P.sk.grid};
-        if (bm == 128)
-            launch_gather<T, 128, 64>(P, a_fast, b_j_fast, grid, stream);
-        else
-            launch_gather<T, 64, 64>(P, a_fast, b_j_fast, grid, stream);
-        EIN_LAUNCH_CHECK();
+        void *args[] = {&P};
+        EIN_HIP_CHECK(hipLaunchKernel(kern, dim3(P.sk.grid), dim3(GEMM_THREADS), args, 0, stream));
         if (P.sk.units != 0) {
             if (bm == 128)
                 gather_streamk_fixup<T, 128, 64><<<uint32_t(tiles - P.sk.dp_tiles), 256, 0, stream>>>(P);
@@ -675,10 +758,10 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_grouped_kernel(GroupedIt
         GroupedItem<T> const &it      = items[e.item];
         uint64_t const S              = uint64_t((it.P.k + BK - 1) / BK);
         switch (it.flags) {
-        case 3: gather_segment<T, BM, BN, BK, false, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
-        case 1: gather_segment<T, BM, BN, BK, false, true>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
-        case 2: gather_segment<T, BM, BN, BK, true, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
-        default: gather_segment<T, BM, BN, BK, true, true>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        case 3: gather_segment<T, BM, BN, BK, false, false, false, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        case 1: gather_segment<T, BM, BN, BK, false, false, true, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        case 2: gather_segment<T, BM, BN, BK, true, false, false, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
+        default: gather_segment<T, BM, BN, BK, true, false, true, false>(it.P, e.tile, 0, S, As, Bs, nullptr); break;
         }
     }
 }
@@ -695,8 +778,19 @@ struct GroupedWave {
     std::vector<PreScale> prescale;        // C scales that must precede this wave's tiles
     void *d_items[2] = {nullptr, nullptr}; // per tile height (64, 128)
     void *d_tiles[2] = {nullptr, nullptr};
+    void *d_tables   = nullptr;            // offset tables of the wave's items
     uint32_t ntiles[2] = {0, 0};
 };
+
+void free_wave(GroupedWave &w) {
+    for (int v = 0; v < 2; ++v) {
+        (void)ein_free(w.d_items[v]);
+        (void)ein_free(w.d_tiles[v]);
+        w.d_items[v] = w.d_tiles[v] = nullptr;
+    }
+    (void)ein_free(w.d_tables);
+    w.d_tables = nullptr;
+}
 
 struct GroupedPlan {
     int elem = 8;
@@ -855,6 +949,26 @@ int plan_create(ein_contraction_item const *items, int count, GroupedPlan &plan,
         wave_items[1][w].push_back(gi[1]);
     }
     for (size_t w = 0; w < plan.waves.size(); ++w) {
+        // offset tables of the wave's items (shared by both tile heights)
+        {
+            size_t total = 0;
+            for (auto const &it : wave_items[0][w]) total += gather_table_entries(it.P.m, it.P.n, it.P.k);
+            std::vector<int64_t> host(std::max<size_t>(total, 1));
+            EIN_TRY(ein_malloc(&plan.waves[w].d_tables, std::max<size_t>(total * sizeof(int64_t), 256)));
+            int64_t *dev = static_cast<int64_t *>(plan.waves[w].d_tables);
+            size_t at = 0;
+            for (size_t q = 0; q < wave_items[0][w].size(); ++q) {
+                GatherProblem<T> &P0 = wave_items[0][w][q].P;
+                int64_t *h           = host.data() + at;
+                host_offsets(P0.gm, P0.m, h, h + P0.m);
+                host_offsets(P0.gn, P0.n, h + 2 * P0.m, h + 2 * P0.m + P0.n);
+                host_offsets(P0.gk, P0.k, h + 2 * (P0.m + P0.n), h + 2 * (P0.m + P0.n) + P0.k);
+                set_tables(P0, dev + at);
+                set_tables(wave_items[1][w][q].P, dev + at);
+                at += gather_table_entries(P0.m, P0.n, P0.k);
+            }
+            if (total) EIN_HIP_CHECK(hipMemcpy(dev, host.data(), total * sizeof(int64_t), hipMemcpyHostToDevice));
+        }
         for (int v = 0; v < 2; ++v) {
             auto const &its = wave_items[v][w];
             std::vector<GroupedTile> tl;
@@ -960,12 +1074,8 @@ int plan_autotune(GroupedPlan &plan, hipStream_t stream) {
         plan.bm_index = best_v;
         plan.tuned    = true;
     }
-    for (auto &w : tp.waves)
-        for (int v = 0; v < 2; ++v) {
-            (void)ein_free(w.d_items[v]);
-            (void)ein_free(w.d_tiles[v]);
-        }
     (void)hipStreamSynchronize(stream);
+    for (auto &w : tp.waves) free_wave(w);
     (void)ein_free(pa);
     (void)ein_free(pb);
     (void)ein_free(pc);
@@ -984,11 +1094,7 @@ int ein_grouped_plan_create(int32_t elem_bytes, const ein_contraction_item *item
     auto *p = new ein::GroupedPlan();
     int st = elem_bytes == 8 ? ein::plan_create<double>(items, count, *p) : ein::plan_create<float>(items, count, *p);
     if (st != EIN_SUCCESS) {
-        for (auto &w : p->waves)
-            for (int v = 0; v < 2; ++v) {
-                (void)ein_free(w.d_items[v]);
-                (void)ein_free(w.d_tiles[v]);
-            }
+        for (auto &w : p->waves) ein::free_wave(w);
         delete p;
         return st;
     }
@@ -1022,11 +1128,7 @@ int ein_grouped_plan_info(void *plan, int32_t *tile_rows, int32_t *waves, int64_
 int ein_grouped_plan_destroy(void *plan) {
     if (plan == nullptr) return EIN_SUCCESS;
     auto *p = static_cast<ein::GroupedPlan *>(plan);
-    for (auto &w : p->waves)
-        for (int v = 0; v < 2; ++v) {
-            (void)ein_free(w.d_items[v]);
-            (void)ein_free(w.d_tiles[v]);
-        }
+    for (auto &w : p->waves) ein::free_wave(w);
     delete p;
     return EIN_SUCCESS;
 }

@@ -38,6 +38,8 @@ __device__ __forceinline__ void group_decode(uint32_t idx, IndexGroup const &G, 
 }
 
 // Operand order inside the groups: M = (A, C), N = (B, C), K = (A, B), batch = (A, B, C).
+// The tile kernels read element offsets from tables (built once per call from the
+// groups): rowA/rowC[i] for i < m, colB/colC[j] for j < n, kA/kB[p] for p < k.
 template <typename T>
 struct GatherProblem {
     IndexGroup gm, gn, gk, gb;
@@ -49,7 +51,16 @@ struct GatherProblem {
     uint32_t tiles_m, tiles_n;
     StreamK sk;
     T *partial;
+    int64_t const *rowA;
+    int64_t const *rowC;
+    int64_t const *colB;
+    int64_t const *colC;
+    int64_t const *kA;
+    int64_t const *kB;
 };
+
+// Offset tables of one problem: [rowA | rowC | colB | colC | kA | kB] (2(m+n+k) entries).
+inline size_t gather_table_entries(int64_t m, int64_t n, int64_t k) { return size_t(2 * (m + n + k)); }
 
 // Result of the host-side label analysis.
 struct ContractionAnalysis {

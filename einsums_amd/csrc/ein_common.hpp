@@ -49,6 +49,10 @@ inline hipStream_t as_stream(ein_stream_t s) { return reinterpret_cast<hipStream
 // Number of compute units on the current device (cached).
 int num_cus();
 
+// Workgroups of `kernel` (launched with `threads` threads, no dynamic LDS) resident per
+// CU at once, from the occupancy calculator (cached per kernel; at least 1).
+int resident_blocks(void const *kernel, int threads);
+
 // ------------------------------------------------------ fast 32-bit divmod
 // Division by an invariant divisor d >= 1 for numerators n < 2^31 using the
 // round-up multiply-shift method (Granlund & Montgomery, PLDI 1994).

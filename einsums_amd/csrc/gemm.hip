@@ -16,7 +16,7 @@
 namespace ein {
 
 // Operand slabs go global -> LDS by DMA (tile_engine.hpp); AKG / BKG select the
-// k-grouped LDS layout for an operand contiguous along k, AW / BW the 16-byte DMA.
+// row-major (KROW) LDS layout for an operand contiguous along k, AW / BW the 16-byte DMA.
 template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW>
 __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem<T> P) {
     using Cfg = TileCfg<T, BM, BN, BK>;
@@ -86,21 +86,46 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
         asm volatile("" : "+v"(ri), "+v"(rj));
         if (s0 == 0 && s1 == S) {
             T *__restrict__ Cb = P.C + batch * P.c_sb;
+            if (i0 + BM <= P.m && j0 + BN <= P.n) {
+                // whole tile inside C: the reads of one column block are issued together
+                // so their latency is paid once per block, not once per element
 #pragma unroll
-            for (int a = 0; a < Cfg::FM; ++a)
+                for (int b = 0; b < Cfg::FN; ++b) {
+                    T cv[Cfg::FM][4];
 #pragma unroll
-                for (int b = 0; b < Cfg::FN; ++b)
+                    for (int a = 0; a < Cfg::FM; ++a)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        int64_t i = i0 + ri + a * 16;
-                        int64_t j = j0 + rj - q + b * 16 + Mfma<T>::row(q, v);
-                        if (i < P.m && j < P.n) {
-                            T *cp = Cb + i * P.c_sm + j * P.c_sn;
-                            T val = P.alpha * acc.c[a][b][v];
-                            if (P.beta != T(0)) val += P.beta * *cp;
-                            *cp = val;
+                        for (int v = 0; v < 4; ++v) {
+                            int64_t i = i0 + ri + a * 16;
+                            int64_t j = j0 + rj - q + b * 16 + Mfma<T>::row(q, v);
+                            cv[a][v]  = P.beta != T(0) ? Cb[i * P.c_sm + j * P.c_sn] : T(0);
                         }
-                    }
+#pragma unroll
+                    for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            int64_t i = i0 + ri + a * 16;
+                            int64_t j = j0 + rj - q + b * 16 + Mfma<T>::row(q, v);
+                            Cb[i * P.c_sm + j * P.c_sn] = P.alpha * acc.c[a][b][v] + P.beta * cv[a][v];
+                        }
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < Cfg::FN; ++b)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            int64_t i = i0 + ri + a * 16;
+                            int64_t j = j0 + rj - q + b * 16 + Mfma<T>::row(q, v);
+                            if (i < P.m && j < P.n) {
+                                T *cp = Cb + i * P.c_sm + j * P.c_sn;
+                                T val = P.alpha * acc.c[a][b][v];
+                                if (P.beta != T(0)) val += P.beta * *cp;
+                                *cp = val;
+                            }
+                        }
+            }
         } else {
             // tile cut by the range ends: raw accumulators to this workgroup's slot, [BN][BM]
             T *__restrict__ piece = P.partial + sk_slot(sk, wlog, t) * (BM * BN);
@@ -164,12 +189,33 @@ __global__ __launch_bounds__(256) void scale_2d_kernel(GemmProblem<T> P) {
 namespace {
 
 template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW>
-int launch_dense_kernel(GemmProblem<T> const &P, dim3 grid, hipStream_t stream) {
-    if constexpr (SlabLayout<T, BM, BK, AKG, AW>::VALID && SlabLayout<T, BN, BK, BKG, BW>::VALID) {
-        gemm_dense_kernel<T, BM, BN, BK, AKG, AW, BKG, BW><<<grid, GEMM_THREADS, 0, stream>>>(P);
-        return EIN_SUCCESS;
-    } else {
-        return fail(EIN_ERROR_UNSUPPORTED, "gemm: operand staging not realisable for this tile");
+void const *dense_kernel_ptr() {
+    if constexpr (SlabLayout<T, BM, BK, AKG, AW>::VALID && SlabLayout<T, BN, BK, BKG, BW>::VALID)
+        return reinterpret_cast<void const *>(&gemm_dense_kernel<T, BM, BN, BK, AKG, AW, BKG, BW>);
+    else
+        return nullptr;
+}
+
+// Kernel instance for staging code (A krow, A wide, B krow, B wide) as bits 3..0.
+template <typename T, int BM, int BN, int BK>
+void const *dense_kernel_for(int code) {
+    switch (code) {
+    case 0: return dense_kernel_ptr<T, BM, BN, BK, false, false, false, false>();
+    case 1: return dense_kernel_ptr<T, BM, BN, BK, false, false, false, true>();
+    case 2: return dense_kernel_ptr<T, BM, BN, BK, false, false, true, false>();
+    case 3: return dense_kernel_ptr<T, BM, BN, BK, false, false, true, true>();
+    case 4: return dense_kernel_ptr<T, BM, BN, BK, false, true, false, false>();
+    case 5: return dense_kernel_ptr<T, BM, BN, BK, false, true, false, true>();
+    case 6: return dense_kernel_ptr<T, BM, BN, BK, false, true, true, false>();
+    case 7: return dense_kernel_ptr<T, BM, BN, BK, false, true, true, true>();
+    case 8: return dense_kernel_ptr<T, BM, BN, BK, true, false, false, false>();
+    case 9: return dense_kernel_ptr<T, BM, BN, BK, true, false, false, true>();
+    case 10: return dense_kernel_ptr<T, BM, BN, BK, true, false, true, false>();
+    case 11: return dense_kernel_ptr<T, BM, BN, BK, true, false, true, true>();
+    case 12: return dense_kernel_ptr<T, BM, BN, BK, true, true, false, false>();
+    case 13: return dense_kernel_ptr<T, BM, BN, BK, true, true, false, true>();
+    case 14: return dense_kernel_ptr<T, BM, BN, BK, true, true, true, false>();
+    default: return dense_kernel_ptr<T, BM, BN, BK, true, true, true, true>();
     }
 }
 
@@ -180,7 +226,13 @@ int launch_dense(GemmProblem<T> &P, hipStream_t stream) {
     uint64_t tiles = uint64_t(P.tiles_m) * P.tiles_n * uint64_t(P.batch);
     uint64_t slabs = uint64_t(ceil_div(P.k, BK));
     if (tiles > 0x7fffffffu) return fail(EIN_ERROR_UNSUPPORTED, "gemm tile grid too large");
-    uint64_t slots = P.force_dp ? tiles : uint64_t(num_cus()) * 2;
+    OperandStaging const sa = choose_staging<T>(BM, P.a_sm, P.a_sk, P.a_sb, P.batch, P.m, P.k, P.A);
+    OperandStaging const sb = choose_staging<T>(BN, P.b_sn, P.b_sk, P.b_sb, P.batch, P.n, P.k, P.B);
+    int const code = (sa.krow ? 8 : 0) | (sa.wide ? 4 : 0) | (sb.krow ? 2 : 0) | (sb.wide ? 1 : 0);
+    void const *kern = dense_kernel_for<T, BM, BN, BK>(code);
+    if (kern == nullptr) return fail(EIN_ERROR_UNSUPPORTED, "gemm: operand staging not realisable for this tile");
+    // resident workgroups: the stream-K schedule balances the last round over them
+    uint64_t slots = P.force_dp ? tiles : uint64_t(num_cus()) * uint64_t(resident_blocks(kern, GEMM_THREADS));
     P.sk           = plan_stream_k(tiles, slabs, slots);
     P.partial      = nullptr;
     if (P.sk.units != 0) {
@@ -188,31 +240,8 @@ int launch_dense(GemmProblem<T> &P, hipStream_t stream) {
         EIN_TRY(workspace(stream, size_t(sk_piece_slots(P.sk)) * BM * BN * sizeof(T), &ws));
         P.partial = static_cast<T *>(ws);
     }
-    OperandStaging const sa = choose_staging<T>(BM, P.a_sm, P.a_sk, P.a_sb, P.batch, P.m, P.k, P.A);
-    OperandStaging const sb = choose_staging<T>(BN, P.b_sn, P.b_sk, P.b_sb, P.batch, P.n, P.k, P.B);
-    dim3 grid{P.sk.grid};
-    int const code = (sa.kgroup ? 8 : 0) | (sa.wide ? 4 : 0) | (sb.kgroup ? 2 : 0) | (sb.wide ? 1 : 0);
-    int st = EIN_SUCCESS;
-    switch (code) {
-    case 0: st = launch_dense_kernel<T, BM, BN, BK, false, false, false, false>(P, grid, stream); break;
-    case 1: st = launch_dense_kernel<T, BM, BN, BK, false, false, false, true>(P, grid, stream); break;
-    case 2: st = launch_dense_kernel<T, BM, BN, BK, false, false, true, false>(P, grid, stream); break;
-    case 3: st = launch_dense_kernel<T, BM, BN, BK, false, false, true, true>(P, grid, stream); break;
-    case 4: st = launch_dense_kernel<T, BM, BN, BK, false, true, false, false>(P, grid, stream); break;
-    case 5: st = launch_dense_kernel<T, BM, BN, BK, false, true, false, true>(P, grid, stream); break;
-    case 6: st = launch_dense_kernel<T, BM, BN, BK, false, true, true, false>(P, grid, stream); break;
-    case 7: st = launch_dense_kernel<T, BM, BN, BK, false, true, true, true>(P, grid, stream); break;
-    case 8: st = launch_dense_kernel<T, BM, BN, BK, true, false, false, false>(P, grid, stream); break;
-    case 9: st = launch_dense_kernel<T, BM, BN, BK, true, false, false, true>(P, grid, stream); break;
-    case 10: st = launch_dense_kernel<T, BM, BN, BK, true, false, true, false>(P, grid, stream); break;
-    case 11: st = launch_dense_kernel<T, BM, BN, BK, true, false, true, true>(P, grid, stream); break;
-    case 12: st = launch_dense_kernel<T, BM, BN, BK, true, true, false, false>(P, grid, stream); break;
-    case 13: st = launch_dense_kernel<T, BM, BN, BK, true, true, false, true>(P, grid, stream); break;
-    case 14: st = launch_dense_kernel<T, BM, BN, BK, true, true, true, false>(P, grid, stream); break;
-    default: st = launch_dense_kernel<T, BM, BN, BK, true, true, true, true>(P, grid, stream); break;
-    }
-    EIN_TRY(st);
-    EIN_LAUNCH_CHECK();
+    void *args[] = {&P};
+    EIN_HIP_CHECK(hipLaunchKernel(kern, dim3(P.sk.grid), dim3(GEMM_THREADS), args, 0, stream));
     if (P.sk.units != 0) {
         gemm_streamk_fixup<T, BM, BN><<<uint32_t(tiles - P.sk.dp_tiles), 256, 0, stream>>>(P);
         EIN_LAUNCH_CHECK();

@@ -100,6 +100,18 @@ int num_cus() {
     return cus;
 }
 
+int resident_blocks(void const *kernel, int threads) {
+    static std::mutex mu;
+    static std::unordered_map<void const *, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(kernel);
+    if (it != cache.end()) return it->second;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, threads, 0) != hipSuccess || blocks < 1) blocks = 1;
+    cache.emplace(kernel, blocks);
+    return blocks;
+}
+
 } // namespace ein
 
 extern "C" {

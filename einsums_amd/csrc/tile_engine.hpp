@@ -13,10 +13,13 @@
 //    The 16-element pad puts consecutive k-rows 32 banks (FP64) / 16 banks (FP32) apart,
 //    so the 16 rows x 2 (FP64) or 4 (FP32) k values one MFMA fragment read touches are
 //    on distinct banks.
-//  - KGROUP (source contiguous along k):
-//      pos(row, k) = (k / KG) * (KG * ROWS) + KG * row + k % KG,  KG = 16 bytes / element.
-//    A lane's 16-byte chunk is KG consecutive k of one row, and a fragment read (16
-//    consecutive rows, KG consecutive k) covers 64 consecutive dwords.
+//  - KROW (source contiguous along k): each tile row holds its BK elements as 16-byte
+//    chunks (CE = 16 bytes / element elements each), the chunk index XOR-swizzled by
+//    SW(row) = (row / RB) % NC, RB = rows per 64 banks, NC = chunks per row:
+//      pos(row, k) = row * BK + ((k / CE) ^ SW(row)) * CE + k % CE.
+//    One DMA instruction fills whole rows, so every lane group reads contiguous 128-byte
+//    (FP64) or 64-byte (FP32) row segments; the swizzle keeps the fragment reads (16 rows,
+//    2 or 4 consecutive k) on distinct banks.
 //
 // Instructions per slab are spread over the 4 waves of the workgroup (n = wave + 4x).
 // The index maps are checked on the host by tests/cpp/test_lds_layout.cpp.
@@ -31,27 +34,29 @@ namespace ein {
 // Zero dwords read by lanes whose element is outside the problem (never written).
 static __device__ uint32_t ein_zero_page[4];
 
-template <typename T, int ROWS, int BK, bool KGROUP, bool WIDE>
+template <typename T, int ROWS, int BK, bool KROW, bool WIDE>
 struct SlabLayout {
     static constexpr int EW    = int(sizeof(T)) / 4;     // dwords per element
     static constexpr int VW    = WIDE ? 4 : 1;           // dwords per lane per instruction
-    static constexpr int KG    = 4 / EW;                 // k per 16-byte group (KGROUP)
+    static constexpr int CE    = 4 / EW;                 // elements per 16-byte chunk
+    static constexpr int NC    = BK / CE;                // chunks per KROW row
+    static constexpr int RB    = 64 / (BK * EW);         // KROW rows per 64 banks
     static constexpr int PITCH = ROWS + 16;              // KMAJOR k-row pitch (elements)
-    static constexpr int SIZE  = KGROUP ? ROWS * BK : BK * PITCH; // elements per buffer
+    static constexpr int SIZE  = KROW ? ROWS * BK : BK * PITCH; // elements per buffer
     static constexpr int SLAB_DW = ROWS * BK * EW;       // dwords moved per slab
     static constexpr int INS   = SLAB_DW / (64 * VW);    // wave instructions per slab
     static constexpr int INS_PER_WAVE = INS / 4;
-    // KMAJOR: instructions per k-row; KGROUP: instructions per k-group
-    static constexpr int IPR = KGROUP ? (ROWS * 4) / (64 * VW) : (ROWS * EW) / (64 * VW);
-    // a wide instruction must not span two LDS rows; instructions split over four waves
-    static constexpr bool VALID = IPR >= 1 && INS % 4 == 0 && BK % KG == 0 && BK % 4 == 0;
+    // KMAJOR: instructions per k-row (a wide instruction must not span two padded k-rows)
+    static constexpr int IPR = KROW ? 1 : (ROWS * EW) / (64 * VW);
+    static constexpr bool VALID = IPR >= 1 && INS % 4 == 0 && BK % CE == 0 && RB >= 1 && NC >= 1;
 
+    static __host__ __device__ constexpr int sw(int row) { return (row / RB) % NC; }
     static __host__ __device__ constexpr int pos(int row, int k) {
-        return KGROUP ? (k / KG) * (KG * ROWS) + KG * row + k % KG : k * PITCH + row;
+        return KROW ? row * BK + ((k / CE) ^ sw(row)) * CE + k % CE : k * PITCH + row;
     }
-    // Instruction n = w + 4x (wave w, x-th instruction of the wave) covers part n % IPR of
-    // LDS row / k-group g = n / IPR. Written per case so that the compiler sees that a
-    // lane touches only IPR/4 (or one) distinct rows.
+    // KMAJOR: instruction n = w + 4x (wave w, x-th instruction of the wave) covers part
+    // n % IPR of k-row n / IPR. Written per case so that the compiler sees that a lane
+    // touches only IPR/4 (or one) distinct rows.
     static __host__ __device__ constexpr int part_of(int w, int x) {
         return IPR % 4 == 0 ? w + 4 * (x % (IPR / 4)) : w % IPR;
     }
@@ -60,22 +65,24 @@ struct SlabLayout {
     }
     // First LDS dword written by instruction (w, x).
     static __host__ __device__ constexpr int ins_base_dw(int w, int x) {
-        return KGROUP ? group_of(w, x) * (ROWS * 4) + part_of(w, x) * 64 * VW
-                      : group_of(w, x) * PITCH * EW + part_of(w, x) * 64 * VW;
+        return KROW ? (w + 4 * x) * 64 * VW : group_of(w, x) * PITCH * EW + part_of(w, x) * 64 * VW;
     }
     // Element (row, k) and dword half h that lane l of instruction (w, x) copies (for WIDE,
     // the first element of its 16-byte chunk; the chunk runs along rows for KMAJOR, along
-    // k for KGROUP).
+    // k for KROW).
     static __host__ __device__ constexpr void lane_elem(int w, int x, int l, int &row, int &k, int &h) {
-        int const o = part_of(w, x) * 64 * VW + l * VW; // dword inside the k-row / k-group
-        if (KGROUP) {
-            row = o / 4;
-            k   = group_of(w, x) * KG + (o % 4) / EW;
-            h   = (o % 4) % EW;
+        if (KROW) {
+            int const d  = (w + 4 * x) * 64 * VW + l * VW; // dword inside the slab buffer
+            int const e  = d / EW;
+            row          = e / BK;
+            int const cs = (e % BK) / CE; // chunk slot in the row
+            k            = (cs ^ sw(row)) * CE + e % CE;
+            h            = d % EW;
         } else {
-            row = o / EW;
-            k   = group_of(w, x);
-            h   = o % EW;
+            int const o = part_of(w, x) * 64 * VW + l * VW; // dword inside the k-row
+            row         = o / EW;
+            k           = group_of(w, x);
+            h           = o % EW;
         }
     }
 };
@@ -84,16 +91,19 @@ struct SlabLayout {
 // returns the element offset of (tile row, contraction step p) from `base` and whether
 // that element is inside the problem. For a 16-byte chunk, (row, p) is its first element
 // and the host guarantees the chunk is entirely inside or entirely outside.
-template <typename T, int ROWS, int BK, bool KGROUP, bool WIDE, typename OffsetFn>
+template <typename T, int ROWS, int BK, bool KROW, bool WIDE, typename OffsetFn>
 __device__ __forceinline__ void dma_slab(T const *__restrict__ base, T *lds, int wave, int lane, int64_t k0,
                                          OffsetFn const &off) {
-    using L = SlabLayout<T, ROWS, BK, KGROUP, WIDE>;
+    using L = SlabLayout<T, ROWS, BK, KROW, WIDE>;
     static_assert(L::VALID, "slab layout not realisable");
     uint32_t const *bdw = reinterpret_cast<uint32_t const *>(base);
 #pragma unroll
     for (int x = 0; x < L::INS_PER_WAVE; ++x) {
         int row, k, h;
         L::lane_elem(wave, x, lane, row, k, h);
+        // a KROW lane visits a different row per instruction: keep the row offsets from
+        // being hoisted out of the slab loop (they would pin 2*INS_PER_WAVE registers)
+        if constexpr (KROW) asm volatile("" : "+v"(row));
         bool valid;
         int64_t const e = off(row, k0 + k, valid);
         void const *src = valid ? static_cast<void const *>(bdw + e * L::EW + h) : static_cast<void const *>(ein_zero_page);
@@ -132,11 +142,11 @@ __device__ __forceinline__ void mfma_slab_lds(T const *As, T const *Bs,
     }
 }
 
-// Staging choice for one operand (host side): k-grouped layout when the source is
-// contiguous along k, 16-byte DMA when every chunk is contiguous, 16-byte aligned and
-// entirely inside or outside the problem.
+// Staging choice for one operand (host side): KROW layout when the source is contiguous
+// along k, 16-byte DMA when every chunk is contiguous, 16-byte aligned and entirely
+// inside or outside the problem.
 struct OperandStaging {
-    bool kgroup;
+    bool krow;
     bool wide;
 };
 
@@ -148,9 +158,9 @@ inline OperandStaging choose_staging(int rows_tile, int64_t s_row, int64_t s_k, 
     auto mag         = [](int64_t x) { return x < 0 ? -x : x; };
     OperandStaging st{};
     bool const row_fast = s_row == 1 || (s_k != 1 && mag(s_row) <= mag(s_k));
-    st.kgroup           = !row_fast;
+    st.krow             = !row_fast;
     bool const aligned  = reinterpret_cast<uintptr_t>(ptr) % 16 == 0 && (batch <= 1 || s_batch % KG == 0);
-    if (!st.kgroup)
+    if (!st.krow)
         st.wide = aligned && s_row == 1 && s_k % KG == 0 && rows % KG == 0 && rows_tile * EW >= 256;
     else
         st.wide = aligned && s_k == 1 && s_row % KG == 0 && K % KG == 0;

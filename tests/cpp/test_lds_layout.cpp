@@ -23,9 +23,9 @@ static int failures = 0;
         }                                                                                                              \
     } while (0)
 
-template <typename T, int ROWS, int BK, bool KGROUP, bool WIDE>
+template <typename T, int ROWS, int BK, bool KROW, bool WIDE>
 void check_layout(char const *name) {
-    using L = SlabLayout<T, ROWS, BK, KGROUP, WIDE>;
+    using L = SlabLayout<T, ROWS, BK, KROW, WIDE>;
     if constexpr (!L::VALID) {
         std::printf("  %-34s not realisable (skipped by the launcher)\n", name);
         return;
@@ -44,7 +44,7 @@ void check_layout(char const *name) {
                         if (WIDE) {
                             int e = j / EW;
                             h2    = j % EW;
-                            if (KGROUP) k2 = k + e; else r2 = row + e;
+                            if (KROW) k2 = k + e; else r2 = row + e;
                         }
                         CHECK(r2 >= 0 && r2 < ROWS && k2 >= 0 && k2 < BK && h2 < EW, "%s: element out of the slab", name);
                         int dw = L::ins_base_dw(w, x) + l * L::VW + j;
@@ -80,19 +80,19 @@ void check_layout(char const *name) {
 int main() {
     check_layout<double, 128, 16, false, false>("f64 128 kmajor dword");
     check_layout<double, 128, 16, false, true>("f64 128 kmajor x4");
-    check_layout<double, 128, 16, true, false>("f64 128 kgroup dword");
-    check_layout<double, 128, 16, true, true>("f64 128 kgroup x4");
+    check_layout<double, 128, 16, true, false>("f64 128 krow dword");
+    check_layout<double, 128, 16, true, true>("f64 128 krow x4");
     check_layout<double, 64, 16, false, false>("f64 64 kmajor dword");
     check_layout<double, 64, 16, false, true>("f64 64 kmajor x4");
-    check_layout<double, 64, 16, true, false>("f64 64 kgroup dword");
-    check_layout<double, 64, 16, true, true>("f64 64 kgroup x4");
+    check_layout<double, 64, 16, true, false>("f64 64 krow dword");
+    check_layout<double, 64, 16, true, true>("f64 64 krow x4");
     check_layout<float, 128, 16, false, false>("f32 128 kmajor dword");
     check_layout<float, 128, 16, false, true>("f32 128 kmajor x4");
-    check_layout<float, 128, 16, true, false>("f32 128 kgroup dword");
-    check_layout<float, 128, 16, true, true>("f32 128 kgroup x4");
+    check_layout<float, 128, 16, true, false>("f32 128 krow dword");
+    check_layout<float, 128, 16, true, true>("f32 128 krow x4");
     check_layout<float, 64, 16, false, false>("f32 64 kmajor dword");
-    check_layout<float, 64, 16, true, false>("f32 64 kgroup dword");
-    check_layout<float, 64, 16, true, true>("f32 64 kgroup x4");
+    check_layout<float, 64, 16, true, false>("f32 64 krow dword");
+    check_layout<float, 64, 16, true, true>("f32 64 krow x4");
     std::printf("%d failures\n", failures);
     return failures ? 1 : 0;
 }
