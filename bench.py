@@ -33,7 +33,8 @@ def parse():
     p.add_argument("--o", type=int, default=24)
     p.add_argument("--v", type=int, default=96)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--traffic", type=str, default=None, help="JSON file with measured HBM bytes per Wabef launch")
+    p.add_argument("--traffic", type=str, default=str(ROOT / "profiles" / "gemm_dense_wabef_o24_v96_traffic.txt"),
+                   help="rocprof summary holding the measured HBM bytes per Wabef launch (o=24, v=96 only)")
     p.add_argument("--split", type=int, default=1, help="1: add the split-precision comparison, 0: skip it")
     p.add_argument("--precision-sweep", action="store_true",
                    help="only run the split-precision vs FP64 GEMM sweep (one JSON line per case) and exit")
@@ -254,8 +255,11 @@ def main():
     dom = "Wabef"
     achieved = flops[dom] / (per[dom] * 1e-3) / 1e12
     traffic = None
-    if args.traffic and Path(args.traffic).exists():
-        traffic = json.loads(Path(args.traffic).read_text()).get("Wabef_bytes")
+    if args.traffic and Path(args.traffic).exists() and (o, v) == (24, 96):
+        # FETCH_SIZE + WRITE_SIZE of the Wabef kernel, each scaled by a 1 GiB copy's reading
+        for line in Path(args.traffic).read_text().splitlines():
+            if line.startswith("Wabef_bytes_per_launch:"):
+                traffic = int(line.split(":")[1])
     kernels = {}
     for n in names:
         gf = flops[n] / (per[n] * 1e-3) / 1e9 if flops[n] else 0.0

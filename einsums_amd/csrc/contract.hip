@@ -270,7 +270,8 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProb
 template <typename T, int BM, int BN>
 __global__ __launch_bounds__(256) void gather_streamk_fixup(GatherProblem<T> P) {
     StreamK const &sk = P.sk;
-    uint64_t const t = uint64_t(sk.dp_tiles) + blockIdx.x;
+    uint64_t const t = uint64_t(sk.dp_tiles) + blockIdx.x / FIXUP_SPLIT;
+    int const part   = int(blockIdx.x % FIXUP_SPLIT);
     uint64_t g0, g1;
     sk_owners(sk, t, g0, g1);
     if (g0 == g1) return;
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(256) void gather_streamk_fixup(GatherProblem<T> P) 
     int64_t const i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
     int64_t ob[3];
     group_decode<3>(bz, P.gb, ob);
-    for (int e = threadIdx.x; e < BM * BN; e += 256) {
+    for (int e = part * 256 + threadIdx.x; e < BM * BN; e += 256 * FIXUP_SPLIT) {
         int ii = e % BM, jj = e / BM;
         int64_t i = i0 + ii, j = j0 + jj;
         if (i >= P.m || j >= P.n) continue;
@@ -573,24 +574,26 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         P.beta  = beta;
         GemmConfig cfg = choose_gemm_config(M, N, K, NB, int(sizeof(T)));
         int const bm   = cfg.bm == 128 ? 128 : 64;
-        constexpr int BN = 64;
-        P.tiles_m        = uint32_t(ceil_div(M, bm));
-        P.tiles_n        = uint32_t(ceil_div(N, BN));
+        int const bn   = cfg.bn == 128 ? 128 : 64;
+        P.tiles_m      = uint32_t(ceil_div(M, bm));
+        P.tiles_n      = uint32_t(ceil_div(N, bn));
         uint64_t tiles   = uint64_t(P.tiles_m) * P.tiles_n * uint64_t(NB);
         if (tiles > 0x7fffffffu) return fail(EIN_ERROR_UNSUPPORTED, "contraction tile grid too large");
         OperandStaging sa{!a_fast, false}, sb{!b_j_fast, false};
         constexpr int EW = int(sizeof(T)) / 4;
         sa.wide = a_fast ? (bm * EW >= 256 && gather_wide_ok<T>(gm, 0, gk, 0, gbv, 0, a))
                          : gather_wide_ok<T>(gk, 0, gm, 0, gbv, 0, a);
-        sb.wide = b_j_fast ? (BN * EW >= 256 && gather_wide_ok<T>(gn, 0, gk, 1, gbv, 1, b))
+        sb.wide = b_j_fast ? (bn * EW >= 256 && gather_wide_ok<T>(gn, 0, gk, 1, gbv, 1, b))
                            : gather_wide_ok<T>(gk, 1, gn, 0, gbv, 1, b);
-        void const *kern = bm == 128 ? gather_kernel_for<T, 128, 64>(sa, sb) : gather_kernel_for<T, 64, 64>(sa, sb);
+        void const *kern = bm == 128 && bn == 128 ? gather_kernel_for<T, 128, 128>(sa, sb)
+                           : bm == 128            ? gather_kernel_for<T, 128, 64>(sa, sb)
+                                                  : gather_kernel_for<T, 64, 64>(sa, sb);
         if (kern == nullptr) return fail(EIN_ERROR_UNSUPPORTED, "contraction: operand staging not realisable for this tile");
         P.sk = plan_stream_k(tiles, uint64_t(ceil_div(K, 16)),
                              uint64_t(num_cus()) * uint64_t(resident_blocks(kern, GEMM_THREADS)));
         // workspace: offset tables, then the stream-K pieces
         size_t const tab_bytes   = (gather_table_entries(M, N, K) * sizeof(int64_t) + 255) / 256 * 256;
-        size_t const piece_bytes = size_t(sk_piece_slots(P.sk)) * size_t(bm) * BN * sizeof(T);
+        size_t const piece_bytes = size_t(sk_piece_slots(P.sk)) * size_t(bm) * size_t(bn) * sizeof(T);
         void *ws                 = nullptr;
         EIN_TRY(workspace(stream, tab_bytes + piece_bytes, &ws));
         int64_t *tab = static_cast<int64_t *>(ws);
@@ -605,10 +608,12 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         void *args[] = {&P};
         EIN_HIP_CHECK(hipLaunchKernel(kern, dim3(P.sk.grid), dim3(GEMM_THREADS), args, 0, stream));
         if (P.sk.units != 0) {
-            if (bm == 128)
-                gather_streamk_fixup<T, 128, 64><<<uint32_t(tiles - P.sk.dp_tiles), 256, 0, stream>>>(P);
+            if (bm == 128 && bn == 128)
+                gather_streamk_fixup<T, 128, 128><<<uint32_t(tiles - P.sk.dp_tiles) * FIXUP_SPLIT, 256, 0, stream>>>(P);
+            else if (bm == 128)
+                gather_streamk_fixup<T, 128, 64><<<uint32_t(tiles - P.sk.dp_tiles) * FIXUP_SPLIT, 256, 0, stream>>>(P);
             else
-                gather_streamk_fixup<T, 64, 64><<<uint32_t(tiles - P.sk.dp_tiles), 256, 0, stream>>>(P);
+                gather_streamk_fixup<T, 64, 64><<<uint32_t(tiles - P.sk.dp_tiles) * FIXUP_SPLIT, 256, 0, stream>>>(P);
             EIN_LAUNCH_CHECK();
         }
         return EIN_SUCCESS;

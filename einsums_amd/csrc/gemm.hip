@@ -148,7 +148,8 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
 template <typename T, int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_streamk_fixup(GemmProblem<T> P) {
     StreamK const &sk = P.sk;
-    uint64_t const t = uint64_t(sk.dp_tiles) + blockIdx.x;
+    uint64_t const t = uint64_t(sk.dp_tiles) + blockIdx.x / FIXUP_SPLIT;
+    int const part   = int(blockIdx.x % FIXUP_SPLIT);
     uint64_t g0, g1;
     sk_owners(sk, t, g0, g1);
     if (g0 == g1) return;
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(256) void gemm_streamk_fixup(GemmProblem<T> P) {
     tile_coords(uint32_t(t % tiles_mn), P.tiles_m, P.tiles_n, tm, tn);
     int64_t const i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
     T *__restrict__ Cb = P.C + batch * P.c_sb;
-    for (int e = threadIdx.x; e < BM * BN; e += 256) {
+    for (int e = part * 256 + threadIdx.x; e < BM * BN; e += 256 * FIXUP_SPLIT) {
         int ii = e % BM, jj = e / BM;
         int64_t i = i0 + ii, j = j0 + jj;
         if (i >= P.m || j >= P.n) continue;
@@ -243,7 +244,7 @@ int launch_dense(GemmProblem<T> &P, hipStream_t stream) {
     void *args[] = {&P};
     EIN_HIP_CHECK(hipLaunchKernel(kern, dim3(P.sk.grid), dim3(GEMM_THREADS), args, 0, stream));
     if (P.sk.units != 0) {
-        gemm_streamk_fixup<T, BM, BN><<<uint32_t(tiles - P.sk.dp_tiles), 256, 0, stream>>>(P);
+        gemm_streamk_fixup<T, BM, BN><<<uint32_t(tiles - P.sk.dp_tiles) * FIXUP_SPLIT, 256, 0, stream>>>(P);
         EIN_LAUNCH_CHECK();
     }
     return EIN_SUCCESS;
@@ -254,11 +255,11 @@ int launch_dense(GemmProblem<T> &P, hipStream_t stream) {
 GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, int elem_bytes) {
     (void)elem_bytes;
     (void)k;
+    (void)batch;
     GemmConfig cfg;
-    int64_t const slots = int64_t(num_cus()) * 2;
-    int64_t const t128  = ceil_div(m, 128) * ceil_div(n, 128) * batch;
-    // big tiles when they fill the machine; stream-K spreads the remainder
-    if (m >= 128 && n >= 128 && t128 >= slots) {
+    // 128 x 128 tiles do four times the MFMA work of 64 x 64 per slab barrier; the
+    // stream-K schedule balances however few of them there are
+    if (m >= 128 && n >= 128) {
         cfg.bm = 128;
         cfg.bn = 128;
     } else {

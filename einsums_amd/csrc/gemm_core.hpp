@@ -118,13 +118,13 @@ __host__ __device__ __forceinline__ void tile_coords(uint32_t logical, uint32_t 
 // keeps L2 locality and still fills the last round:
 // - tiles [0, dp_tiles) are data-parallel: workgroup b < dp_tiles owns one whole tile
 //   (XCD-remapped, so tiles running at the same time are neighbours on one XCD);
-// - the remaining tiles (the last one to two rounds' worth) are shared stream-K style by
+// - the remaining tiles (those of the last, partial round) are shared stream-K style by
 //   the last grid - dp_tiles workgroups: stream-K workgroup w owns the contiguous unit
 //   range base + [w*U, min(W, (w+1)*U)) with base = dp_tiles*slabs. Whole tiles inside it
 //   are finished in place; the (at most two) tiles cut by its range ends are written as
 //   partial accumulators to piece slots 2w (first tile it touches) and 2w+1 (last tile).
-// A fix-up pass over the stream-K tiles sums the pieces of every cut tile in workgroup
-// order, so results are deterministic and no atomics are used.
+// A fix-up pass over the stream-K tiles (FIXUP_SPLIT workgroups per tile) sums the pieces
+// of every cut tile in workgroup order, so results are deterministic and no atomics are used.
 struct StreamK {
     uint32_t tiles;    // tiles in the launch (batch included)
     uint32_t slabs;    // BK-slabs per tile
@@ -147,8 +147,10 @@ inline StreamK plan_stream_k(uint64_t tiles, uint64_t slabs, uint64_t slots) {
         sk.units    = 0;
         return sk;
     }
+    // every full round runs data-parallel; only the tiles of the last, partial round are
+    // shared (each by several workgroups), which keeps the fix-up small
     uint64_t full = tiles / slots;
-    uint64_t dp   = full >= 2 ? (full - 1) * slots : 0;
+    uint64_t dp   = full * slots;
     dp -= dp % 8; // keep the stream-K workgroups on the same XCD pattern
     uint64_t W    = (tiles - dp) * slabs;
     uint64_t G    = slots < W ? slots : W;
@@ -193,6 +195,8 @@ __host__ __device__ __forceinline__ void sk_owners(StreamK const &sk, uint64_t t
     g0                 = off / sk.units;
     g1                 = (off + sk.slabs - 1) / sk.units;
 }
+
+constexpr int FIXUP_SPLIT = 8;
 
 // Number of piece slots a launch needs.
 inline uint64_t sk_piece_slots(StreamK const &sk) { return sk.units ? 2ull * (sk.grid - sk.dp_tiles) : 0; }
