@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--traffic", type=str, default=str(ROOT / "profiles" / "gemm_dense_wabef_o24_v96_traffic.txt"),
                    help="rocprof summary holding the measured HBM bytes per Wabef launch (o=24, v=96 only)")
     p.add_argument("--split", type=int, default=1, help="1: add the split-precision comparison, 0: skip it")
+    p.add_argument("--grouped", action="store_true",
+                   help="only run the batched uneven-block benchmarks (one JSON line) and exit")
     p.add_argument("--precision-sweep", action="store_true",
                    help="only run the split-precision vs FP64 GEMM sweep (one JSON line per case) and exit")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -159,6 +161,12 @@ def precision_sweep(sizes=(1024, 2048, 4096, 8192, 16384), reps=3):
                "fp64_tflops": round(2.0 * m * n * k / t64 / 1e9, 2),
                "fp64_rel_err_rows_vs_host": float(f"{np.linalg.norm(C64[rows].cpu().numpy() - ref_rows) / np.linalg.norm(ref_rows):.3e}")}
         Cs = torch.empty_like(C64)
+        A32, B32 = A.float(), B.float()
+        C32 = torch.empty((m, n), dtype=torch.float32, device="cuda")
+        t32 = timed(lambda: core.gemm("N", "N", 1.0, A32, B32, 0.0, C32))
+        row["fp32"] = {"ms": round(t32, 3), "speedup": round(t64 / t32, 3),
+                       "rel_err_vs_fp64": float(f"{float(torch.linalg.norm(C32.double() - C64) / torch.linalg.norm(C64)):.3e}")}
+        del A32, B32, C32
         for st, ns in SPLIT_MODES:
             ts = timed(lambda: core.gemm_split("N", "N", 1.0, A, B, 0.0, Cs, nslices=ns, slice_type=st))
             err = float(torch.linalg.norm(Cs - C64) / torch.linalg.norm(C64))
@@ -181,9 +189,103 @@ def precision_sweep(sizes=(1024, 2048, 4096, 8192, 16384), reps=3):
     for n in sizes:
         out.append(case(f"dgemm uniform n={n}", n, n, n, uniform))
         out.append(case(f"dgemm stressed n={n}", n, n, n, stressed))
-    # Wabef at o=48, v=256: C (v^2 x v^2) = g_oovv^T (v^2 x o^2) t_oovv (o^2 x v^2)
+    # o=48, v=256: Wmnij (o^2 x o^2, k = v^2) and Wabef (v^2 x v^2, k = o^2) as GEMMs
+    out.append(case("Wmnij o=48 v=256", 2304, 2304, 65536, lambda r, c, g: uniform(r, c, g) * 0.1))
     out.append(case("Wabef o=48 v=256", 65536, 65536, 2304, lambda r, c, g: uniform(r, c, g) * 0.1))
     return out
+
+
+def grouped_bench(reps=5):
+    """Batched uneven blocks: (1) per-irrep CCD W intermediates of 8 irreps (o_h in [2,24],
+    v_h in [8,96]) as one grouped plan vs one einsum call per block (the reference's block
+    loop); (2) 4096 GEMMs with m, n, k log-uniform in [4,512], FP64 and FP32, autotuned
+    grouped plan vs per-GEMM calls; (3) a CCD iteration (grouped W intermediates, energy,
+    permute) captured in a hipGraph and replayed."""
+    import numpy as np
+    import torch
+
+    import einsums_amd.core as core
+    from einsums_amd import blocks
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    rng = np.random.default_rng(2024)
+    o = [int(x) for x in rng.integers(2, 25, 8)]
+    v = [int(x) for x in rng.integers(8, 97, 8)]
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    terms, flop = [], 0.0
+    for h in range(8):
+        oo, vv = o[h], v[h]
+        t = dev(rng.uniform(-0.01, 0.01, (oo, oo, vv, vv)))
+        g = dev(rng.uniform(-1, 1, (oo, oo, vv, vv)))
+        terms += [("mnij", dev(rng.uniform(-1, 1, (oo,) * 4)), "ijef", t, "mnef", g, 1.0, 0.25),
+                  ("abef", dev(rng.uniform(-1, 1, (vv,) * 4)), "mnef", g, "mnab", t, 1.0, 0.25),
+                  ("mbej", dev(rng.uniform(-1, 1, (oo, vv, vv, oo))), "jnfb", t, "mnef", g, 1.0, -0.5)]
+        flop += 2.0 * (oo ** 4 * vv ** 2 + vv ** 4 * oo ** 2 + oo ** 3 * vv ** 3)
+    plan = blocks.GroupedEinsum(terms, autotune=True)
+    t_grouped = timed(plan.execute)
+
+    def loop():
+        for ci, C, ai, A, bi, B, cp, abp in terms:
+            core.einsum(ci, C, ai, A, bi, B, C_prefactor=cp, AB_prefactor=abp)
+
+    t_loop = timed(loop)
+    res = {"ccd_irreps": {"o_h": o, "v_h": v, "gflop": round(flop / 1e9, 3), "grouped_ms": round(t_grouped, 4),
+                          "per_block_loop_ms": round(t_loop, 4), "speedup": round(t_loop / t_grouped, 2),
+                          "grouped_tflops": round(flop / t_grouped / 1e9, 2), "info": plan.info()}}
+
+    # CCD iteration in a hipGraph: grouped W intermediates + energy + permute of irrep 0
+    s = torch.cuda.Stream()
+    E = torch.zeros((), dtype=torch.float64, device="cuda")
+    t0, g0 = terms[0][3], terms[0][5]
+    P = torch.empty((t0.shape[0], t0.shape[2], t0.shape[1], t0.shape[3]), dtype=torch.float64, device="cuda")
+
+    def iteration():
+        plan.execute(s)
+        core.einsum("", E, "ijab", t0, "ijab", g0, AB_prefactor=0.25, stream=s)
+        core.permute(0.0, "iajb", P, 1.0, "ijab", t0, stream=s)
+
+    with torch.cuda.stream(s):
+        iteration()
+    s.synchronize()
+    graph = blocks.Graph(iteration, s)
+    with torch.cuda.stream(s):
+        t_eager = timed(iteration)
+        t_graph = timed(graph.replay)
+    res["ccd_iteration_graph"] = {"eager_ms": round(t_eager, 4), "graph_replay_ms": round(t_graph, 4)}
+
+    for dtype in (torch.float64, torch.float32):
+        rng = np.random.default_rng(7)
+        shapes = np.exp(rng.uniform(np.log(4), np.log(512), (4096, 3))).astype(int)
+        gt, gflop = [], 0.0
+        for (m, n, k) in shapes:
+            A = torch.rand((m, k), dtype=dtype, device="cuda")
+            B = torch.rand((k, n), dtype=dtype, device="cuda")
+            C = torch.empty((m, n), dtype=dtype, device="cuda")
+            gt.append(("ij", C, "ik", A, "kj", B, 0.0, 1.0))
+            gflop += 2.0 * m * n * k
+        gp = blocks.GroupedEinsum(gt, autotune=True)
+        tg = timed(gp.execute)
+
+        def loop2():
+            for ci, C, ai, A, bi, B, cp, abp in gt:
+                core.einsum(ci, C, ai, A, bi, B, C_prefactor=cp, AB_prefactor=abp)
+
+        tl = timed(loop2)
+        res[f"gemm4096_{'f64' if dtype == torch.float64 else 'f32'}"] = {
+            "gflop": round(gflop / 1e9, 3), "grouped_ms": round(tg, 3), "per_gemm_loop_ms": round(tl, 3),
+            "speedup": round(tl / tg, 2), "grouped_tflops": round(gflop / tg / 1e9, 2), "info": gp.info()}
+    return res
 
 
 def main():
@@ -193,6 +295,9 @@ def main():
         return
     if args.precision_sweep:
         precision_sweep()
+        return
+    if args.grouped:
+        print(json.dumps({"grouped": grouped_bench()}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of this build",

@@ -34,4 +34,9 @@ int scale_strided(int64_t n, T alpha, T *x, int64_t incx, hipStream_t stream);
 template <typename T>
 int tensor_scale(ein_tensor_desc const &C, T *c, T alpha, hipStream_t stream);
 
+// C = beta*C + alpha*perm(A): C axis i is A axis perm[i] (permute.hip).
+template <typename T>
+int permute_tensor(int32_t rank, int64_t const *dims_A, int32_t const *perm, T alpha, T const *A,
+                   int64_t const *strides_A, T beta, T *C, int64_t const *strides_C, hipStream_t stream);
+
 } // namespace ein

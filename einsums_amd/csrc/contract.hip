@@ -192,11 +192,14 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
     for (int64_t sl = 0; sl < nslabs; ++sl) {
         int const buf = int(sl & 1);
         dma_wait_all();
-        if (sl + 1 < nslabs) {
-            dma_slab<T, BM, BK, AKG, AW>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
-            dma_slab<T, BN, BK, BKG, BW>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
-        }
-        mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, acc, wm, wn, lane);
+        auto next_dma = [&]() {
+            if (sl + 1 < nslabs) {
+                dma_slab<T, BM, BK, AKG, AW>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
+                dma_slab<T, BN, BK, BKG, BW>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
+            }
+        };
+        mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, acc, wm, wn, lane,
+                                                       next_dma);
     }
 
     // lane coordinates made opaque so the store addresses are not hoisted out of the
@@ -558,6 +561,27 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         bool const b_j_fast   = b_j_fast_n || !b_p_unit;
         bool const a_fast     = a_i_fast || !(!gk.empty() && std::llabs(gk.back().s[0]) == 1);
         auto gbv              = order_and_fold(vb, 3, 2, 0);
+        // The epilogue's lanes run along the innermost M digit. When that digit is far from
+        // unit stride in C, the tile is written to a dense buffer X (i fastest, then j, then
+        // the batch) and added into C by the LDS-tiled permute, whose reads and writes are
+        // both coalesced.
+        bool const stage = std::llabs(gm.back().s[1]) >= 16 && int(gm.size() + gn.size() + gbv.size()) <= EIN_MAX_RANK;
+        std::vector<Digit> const gm_c = gm, gn_c = gn, gb_c = gbv; // C strides
+        if (stage) {
+            int64_t pv = 1;
+            for (size_t d = gm.size(); d-- > 0;) {
+                gm[d].s[1] = pv;
+                pv *= gm[d].e;
+            }
+            for (size_t d = gn.size(); d-- > 0;) {
+                gn[d].s[1] = pv;
+                pv *= gn[d].e;
+            }
+            for (size_t d = gbv.size(); d-- > 0;) {
+                gbv[d].s[2] = pv;
+                pv *= gbv[d].e;
+            }
+        }
         GatherProblem<T> P{};
         EIN_TRY(fill_group(P.gm, gm, 2));
         EIN_TRY(fill_group(P.gn, gn, 2));
@@ -593,12 +617,18 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
                              uint64_t(num_cus()) * uint64_t(resident_blocks(kern, GEMM_THREADS)));
         // workspace: offset tables, then the stream-K pieces
         size_t const tab_bytes   = (gather_table_entries(M, N, K) * sizeof(int64_t) + 255) / 256 * 256;
-        size_t const piece_bytes = size_t(sk_piece_slots(P.sk)) * size_t(bm) * size_t(bn) * sizeof(T);
+        size_t const piece_bytes = (size_t(sk_piece_slots(P.sk)) * size_t(bm) * size_t(bn) * sizeof(T) + 255) / 256 * 256;
+        size_t const x_bytes     = stage ? size_t(M) * size_t(N) * size_t(NB) * sizeof(T) : 0;
         void *ws                 = nullptr;
-        EIN_TRY(workspace(stream, tab_bytes + piece_bytes, &ws));
+        EIN_TRY(workspace(stream, tab_bytes + piece_bytes + x_bytes, &ws));
         int64_t *tab = static_cast<int64_t *>(ws);
         set_tables(P, tab);
         P.partial = P.sk.units != 0 ? reinterpret_cast<T *>(static_cast<char *>(ws) + tab_bytes) : nullptr;
+        T *X      = stage ? reinterpret_cast<T *>(static_cast<char *>(ws) + tab_bytes + piece_bytes) : nullptr;
+        if (stage) {
+            P.C    = X;
+            P.beta = T(0);
+        }
         {
             int64_t const mx = std::max(M, std::max(N, K));
             dim3 tgrid{uint32_t(std::min<int64_t>(ceil_div(mx, 256), 4096)), 3};
@@ -615,6 +645,24 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
             else
                 gather_streamk_fixup<T, 64, 64><<<uint32_t(tiles - P.sk.dp_tiles) * FIXUP_SPLIT, 256, 0, stream>>>(P);
             EIN_LAUNCH_CHECK();
+        }
+        if (stage) {
+            // C = beta*C + X over the (M, N, batch) digits
+            std::vector<int64_t> dims, sx, sc;
+            std::vector<int32_t> perm;
+            auto add = [&](std::vector<Digit> const &dx, std::vector<Digit> const &dc, int o) {
+                for (size_t d = 0; d < dx.size(); ++d) {
+                    perm.push_back(int32_t(dims.size()));
+                    dims.push_back(dx[d].e);
+                    sx.push_back(dx[d].s[o]);
+                    sc.push_back(dc[d].s[o]);
+                }
+            };
+            add(gbv, gb_c, 2);
+            add(gn, gn_c, 1);
+            add(gm, gm_c, 1);
+            EIN_TRY(permute_tensor<T>(int32_t(dims.size()), dims.data(), perm.data(), T(1), X, sx.data(), beta, c,
+                                      sc.data(), stream));
         }
         return EIN_SUCCESS;
     }

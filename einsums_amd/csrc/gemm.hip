@@ -73,11 +73,14 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
         for (int64_t sl = 0; sl < nslabs; ++sl) {
             int const buf = int(sl & 1);
             dma_wait_all();
-            if (sl + 1 < nslabs) {
-                dma_slab<T, BM, BK, AKG, AW>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
-                dma_slab<T, BN, BK, BKG, BW>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
-            }
-            mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, acc, wm, wn, lane);
+            auto next_dma = [&]() {
+                if (sl + 1 < nslabs) {
+                    dma_slab<T, BM, BK, AKG, AW>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
+                    dma_slab<T, BN, BK, BKG, BW>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
+                }
+            };
+            mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, acc, wm, wn, lane,
+                                                           next_dma);
         }
 
         // Lane coordinates made opaque per segment so the compiler does not hoist the

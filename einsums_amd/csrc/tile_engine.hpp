@@ -114,10 +114,13 @@ __device__ __forceinline__ void dma_slab(T const *__restrict__ base, T *lds, int
 
 // MFMA work of one slab from LDS, with the fragments of step kk+4 read while the MFMAs of
 // step kk run. As/Bs are the slab buffers of A (rows along i) and B (rows along j).
-template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW>
+// `between()` runs after the first fragments are requested and before the first MFMA:
+// the caller issues the next slab's DMA there, so the fragment reads are not queued
+// behind it (LDS DMA and ds_read stay in program order).
+template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW, typename Between>
 __device__ __forceinline__ void mfma_slab_lds(T const *As, T const *Bs,
                                               WaveAcc<T, TileCfg<T, BM, BN, BK>::FM, TileCfg<T, BM, BN, BK>::FN> &acc,
-                                              int wm, int wn, int lane) {
+                                              int wm, int wn, int lane, Between const &between) {
     using Cfg = TileCfg<T, BM, BN, BK>;
     using LA  = SlabLayout<T, BM, BK, AKG, AW>;
     using LB  = SlabLayout<T, BN, BK, BKG, BW>;
@@ -131,6 +134,7 @@ __device__ __forceinline__ void mfma_slab_lds(T const *As, T const *Bs,
         for (int b = 0; b < Cfg::FN; ++b) bf[slot][b] = Bs[LB::pos(wn * Cfg::WN + b * 16 + r, kk + q)];
     };
     frag(0, 0);
+    between();
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
         int const cur = (kk / 4) & 1;

@@ -159,3 +159,21 @@ def test_gpuview_copy_and_map(core):
     C2 = np.zeros((20, 20))
     core.compile_plan("ij", "ik", "kj").execute(0.0, core.GPUView(C2, core.MAP), 1.0, vA, vB)
     assert rel_fro(C2, A @ B) < 1e-14
+
+
+# Patterns whose innermost row digit (ordered by A) is far from unit stride in C, so the
+# generic kernel writes a dense buffer and adds it into C with the tiled permute.
+STAGED = [
+    ("zija", "zaik", "zkj", dict(z=3, i=40, j=36, a=24, k=48)),     # batch group, C unit stride on a
+    ("mbej", "jnfb", "mnef", dict(m=20, b=40, e=40, j=20, n=20, f=40)),  # the Wmbej layout
+    ("jiij", "ika", "akj", dict(i=32, j=48, k=24, a=20)),            # repeated C labels (diagonal)
+]
+
+
+@pytest.mark.parametrize("pat", STAGED, ids=lambda p: f"{p[0]}<-{p[1]},{p[2]}")
+@pytest.mark.parametrize("cp", [0.0, 0.75])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_staged_epilogue_vs_oracle(core, oracle, pat, cp, dtype):
+    Ci, Ai, Bi, ext = pat
+    got, ref, _ = run_both(core, oracle, cp, Ci, Ai, Bi, ext, dtype=dtype, generic_only=True)
+    assert rel_fro(got, ref) < (1e-13 if dtype == np.float64 else 1e-5)
