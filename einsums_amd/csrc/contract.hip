@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "contract.hpp"
+#include "einsum.hpp"
 #include "tile_engine.hpp"
 
 namespace ein {
@@ -171,22 +172,34 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
     T const *__restrict__ Ab = P.A + boff[0];
     T const *__restrict__ Bb = P.B + boff[1];
 
-    // element offsets from the per-call tables (one load each, cached)
+    // element offsets from the per-call tables; with k_affine one (wave-uniform) table
+    // read per slab gives all 16 contraction offsets
+    int64_t kslab = 0, kbA = 0, kbB = 0;
+    auto set_slab = [&](int64_t k0) {
+        kslab = k0;
+        if (P.k_affine && k0 < kend) {
+            kbA = P.kA[k0];
+            kbB = P.kB[k0];
+        }
+    };
     auto offA = [&](int row, int64_t p, bool &valid) -> int64_t {
         int64_t const i = i0 + row;
         valid           = i < P.m && p < kend;
-        return valid ? P.rowA[i] + P.kA[p] : 0;
+        if (!valid) return 0;
+        return P.rowA[i] + (P.k_affine ? kbA + (p - kslab) * P.kA_in : P.kA[p]);
     };
     auto offB = [&](int row, int64_t p, bool &valid) -> int64_t {
         int64_t const j = j0 + row;
         valid           = j < P.n && p < kend;
-        return valid ? P.colB[j] + P.kB[p] : 0;
+        if (!valid) return 0;
+        return P.colB[j] + (P.k_affine ? kbB + (p - kslab) * P.kB_in : P.kB[p]);
     };
 
     WaveAcc<T, Cfg::FM, Cfg::FN> acc;
     acc.zero();
     int64_t const nslabs = int64_t(s1 - s0);
     __syncthreads(); // LDS of the previous segment no longer read
+    set_slab(kbeg);
     dma_slab<T, BM, BK, AKG, AW>(Ab, As, wave, lane, kbeg, offA);
     dma_slab<T, BN, BK, BKG, BW>(Bb, Bs, wave, lane, kbeg, offB);
     for (int64_t sl = 0; sl < nslabs; ++sl) {
@@ -194,6 +207,7 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
         dma_wait_all();
         auto next_dma = [&]() {
             if (sl + 1 < nslabs) {
+                set_slab(kbeg + (sl + 1) * BK);
                 dma_slab<T, BM, BK, AKG, AW>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
                 dma_slab<T, BN, BK, BKG, BW>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
             }
@@ -495,6 +509,15 @@ void host_offsets(IndexGroup const &G, int64_t total, int64_t *o0, int64_t *o1) 
     }
 }
 
+// Within a 16-step slab the contraction offsets are affine when the innermost K digit's
+// extent is a multiple of 16 (slabs start at multiples of 16 and never cross that digit).
+template <typename T>
+void set_k_affine(GatherProblem<T> &P, std::vector<Digit> const &gk) {
+    P.k_affine = (!gk.empty() && gk.back().e % 16 == 0) ? 1 : 0;
+    P.kA_in    = gk.empty() ? 0 : gk.back().s[0];
+    P.kB_in    = gk.empty() ? 0 : gk.back().s[1];
+}
+
 template <typename T>
 void set_tables(GatherProblem<T> &P, int64_t const *tab) {
     P.rowA = tab;
@@ -587,6 +610,7 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         EIN_TRY(fill_group(P.gn, gn, 2));
         EIN_TRY(fill_group(P.gk, gk, 2));
         EIN_TRY(fill_group(P.gb, gbv, 3));
+        set_k_affine(P, gk);
         P.m     = M;
         P.n     = N;
         P.k     = K;
@@ -829,6 +853,7 @@ struct PreScale {
 
 struct GroupedWave {
     std::vector<PreScale> prescale;        // C scales that must precede this wave's tiles
+    std::vector<ein_contraction_item> large; // items that fill the GPU alone: regular dispatch
     void *d_items[2] = {nullptr, nullptr}; // per tile height (64, 128)
     void *d_tiles[2] = {nullptr, nullptr};
     void *d_tables   = nullptr;            // offset tables of the wave's items
@@ -932,6 +957,7 @@ int build_item(ein_contraction_item const &in, GroupedItem<T> (&out)[2], int &ki
     EIN_TRY(fill_group(P.gn, gn, 2));
     EIN_TRY(fill_group(P.gk, gk, 2));
     EIN_TRY(fill_group(P.gb, gbv, 3));
+    set_k_affine(P, gk);
     P.m = M;
     P.n = N;
     P.k = K;
@@ -996,6 +1022,13 @@ int plan_create(ein_contraction_item const *items, int count, GroupedPlan &plan,
         placed.push_back(me);
         plan.items.push_back(items[i]);
         plan.wave_of.push_back(me.wave);
+        // an item with at least one 128 x 128 tile per CU keeps its own tuned kernels (wide
+        // DMA, 128 x 128 tiles, stream-K); the persistent kernel takes the many small ones
+        if (kind == 0 && gi[0].P.m >= 128 && gi[0].P.n >= 128 &&
+            ceil_div(gi[0].P.m, 128) * ceil_div(gi[0].P.n, 128) * gi[0].P.batch >= int64_t(num_cus())) {
+            plan.waves[w].large.push_back(items[i]);
+            continue;
+        }
         if (kind == 1 || rep) plan.waves[w].prescale.push_back({items[i].C, items[i].c, items[i].c_prefactor});
         if (kind == 1) continue;
         wave_items[0][w].push_back(gi[0]);
@@ -1056,20 +1089,29 @@ int plan_create(ein_contraction_item const *items, int count, GroupedPlan &plan,
 
 template <typename T>
 int plan_launch(GroupedPlan const &plan, int bm_index, hipStream_t stream) {
-    uint32_t const slots = uint32_t(num_cus()) * 2;
+    void const *k128 = reinterpret_cast<void const *>(&gemm_grouped_kernel<T, 128>);
+    void const *k64  = reinterpret_cast<void const *>(&gemm_grouped_kernel<T, 64>);
+    uint32_t const slots =
+        uint32_t(num_cus()) * uint32_t(resident_blocks(bm_index == 1 ? k128 : k64, GEMM_THREADS));
     for (auto const &w : plan.waves) {
         for (auto const &ps : w.prescale)
             EIN_TRY(tensor_scale<T>(ps.C, static_cast<T *>(ps.c), T(ps.factor), stream));
         uint32_t nt = w.ntiles[bm_index];
-        if (nt == 0) continue;
-        uint32_t grid = nt < slots ? nt : slots;
-        if (bm_index == 1)
-            gemm_grouped_kernel<T, 128><<<grid, GEMM_THREADS, 0, stream>>>(
-                static_cast<GroupedItem<T> const *>(w.d_items[1]), static_cast<GroupedTile const *>(w.d_tiles[1]), nt);
-        else
-            gemm_grouped_kernel<T, 64><<<grid, GEMM_THREADS, 0, stream>>>(
-                static_cast<GroupedItem<T> const *>(w.d_items[0]), static_cast<GroupedTile const *>(w.d_tiles[0]), nt);
-        EIN_LAUNCH_CHECK();
+        if (nt != 0) {
+            uint32_t grid = nt < slots ? nt : slots;
+            if (bm_index == 1)
+                gemm_grouped_kernel<T, 128><<<grid, GEMM_THREADS, 0, stream>>>(
+                    static_cast<GroupedItem<T> const *>(w.d_items[1]), static_cast<GroupedTile const *>(w.d_tiles[1]),
+                    nt);
+            else
+                gemm_grouped_kernel<T, 64><<<grid, GEMM_THREADS, 0, stream>>>(
+                    static_cast<GroupedItem<T> const *>(w.d_items[0]), static_cast<GroupedTile const *>(w.d_tiles[0]), nt);
+            EIN_LAUNCH_CHECK();
+        }
+        for (auto const &it : w.large)
+            EIN_TRY(einsum_execute<T>(EIN_ALG_INDETERMINATE, T(it.c_prefactor), it.C, static_cast<T *>(it.c),
+                                      T(it.ab_prefactor), it.A, static_cast<T const *>(it.a), it.B,
+                                      static_cast<T const *>(it.b), nullptr, stream));
     }
     return EIN_SUCCESS;
 }

@@ -57,6 +57,11 @@ struct GatherProblem {
     int64_t const *colC;
     int64_t const *kA;
     int64_t const *kB;
+    // k_affine: the innermost K digit's extent is a multiple of the 16-step slab, so within a
+    // slab kA[p] = kA[p0] + (p - p0) * kA_in (same for B)
+    int64_t kA_in, kB_in;
+    int32_t k_affine;
+    int32_t pad_;
 };
 
 // Offset tables of one problem: [rowA | rowC | colB | colC | kA | kB] (2(m+n+k) entries).

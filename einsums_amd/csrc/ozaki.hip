@@ -292,8 +292,8 @@ __global__ __launch_bounds__((BM / 32) * (BN / 32) * 64) void ozaki_gemm_kernel(
 //     A(i,p) = 2^e_i * sum_t a_t(i,p) 2^(-7(t+1)) + r,  |r| <= 2^(e_i - 7 NS - 1).
 // A diagonal d = t+u holds at most one product of two leading slices (<= 127^2, d = 0 only)
 // and otherwise products <= 127*64, so its int32 accumulator is exact while
-// (d+1) * 127 * 64 * K < 2^31: over chunks of I8_KC = 32768 contraction steps for NS <= 6.
-// Chunks are folded into FP64 accumulators with weight 2^(-7(d+2)).
+// (d+1) * 127 * 64 * K < 2^31: over I8_KC = 32768 contraction steps for NS <= 6. Longer
+// contractions are split (split-K) and the partial results summed in FP64.
 constexpr int I8_BETA = 7;
 constexpr int I8_KC   = 32768;
 constexpr int I8_KS   = 64; // contraction steps per stage (one MFMA depth)
@@ -377,9 +377,9 @@ __device__ __forceinline__ void i8_dma(int8_t const *__restrict__ X, int64_t row
 
 // C(i,j) = alpha * 2^(e_i+f_j) * sum_d 2^(-7(d+2)) D_d(i,j) + beta * C(i,j); workgroup tile
 // 128 x 64, waves of 64 x 32 (4 x 2 blocks of 16 x 16 x 64).
-// MULTI: the contraction spans several exact-accumulation chunks (kp > I8_KC), so the int32
-// diagonals are folded into FP64 accumulators after every chunk.
-template <int NS, bool MULTI>
+// Split-K: workgroup (tile, split) accumulates contraction steps [split*kchunk, ...) with
+// kchunk <= I8_KC, so every int32 diagonal stays exact without intermediate folding.
+template <int NS>
 __global__ __launch_bounds__(256, 1) void ozaki_i8_kernel(OzProblem P) {
     constexpr int BM = 128, BN = 64, FM = 4, FN = 2;
     using SA = I8Stage<BM, NS>;
@@ -395,44 +395,38 @@ __global__ __launch_bounds__(256, 1) void ozaki_i8_kernel(OzProblem P) {
     int const q    = lane >> 4;
 
     uint32_t const ntiles  = P.tiles_m * P.tiles_n;
-    uint32_t const logical = xcd_remap(blockIdx.x, ntiles);
+    uint32_t const logical = xcd_remap(blockIdx.x, gridDim.x);
+    uint32_t const split   = logical / ntiles;
+    uint32_t const tile    = logical - split * ntiles;
     uint32_t tm, tn;
-    tile_coords(logical, P.tiles_m, P.tiles_n, tm, tn);
+    tile_coords(tile, P.tiles_m, P.tiles_n, tm, tn);
     int64_t const i0 = int64_t(tm) * BM;
     int64_t const j0 = int64_t(tn) * BN;
+    int64_t const kb = int64_t(split) * P.kchunk;
+    int64_t const ke = kb + P.kchunk < P.kp ? kb + P.kchunk : P.kp;
     int8_t const *__restrict__ Ag = static_cast<int8_t const *>(P.As_raw);
     int8_t const *__restrict__ Bg = static_cast<int8_t const *>(P.Bs_raw);
 
     v4i acc[NS][FM][FN];
-    v4d accd[MULTI ? FM : 1][MULTI ? FN : 1];
 #pragma unroll
     for (int a = 0; a < FM; ++a)
 #pragma unroll
-        for (int b = 0; b < FN; ++b) {
-            if constexpr (MULTI) accd[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < FN; ++b)
 #pragma unroll
             for (int d = 0; d < NS; ++d) acc[d][a][b] = v4i{0, 0, 0, 0};
-        }
-    // weighted sum of the diagonals of block (a, b), element v
-    auto diag_sum = [&](int a, int b, int v) __attribute__((always_inline)) {
-        double x = 0.0;
-#pragma unroll
-        for (int d = NS - 1; d >= 0; --d) x += double(acc[d][a][b][v]) * ldexp(1.0, -I8_BETA * (d + 2));
-        return x;
-    };
 
-    int64_t const nst = P.kp / I8_KS;
+    int64_t const nst = (ke - kb) / I8_KS;
     auto stage_ptr    = [&](int buf) { return lds + buf * (SA::SLOTS + SB::SLOTS) * 16; };
     if (nst > 0) {
-        i8_dma<BM, NS>(Ag, P.m, P.kp, i0, 0, stage_ptr(0), wave, lane);
-        i8_dma<BN, NS>(Bg, P.n, P.kp, j0, 0, stage_ptr(0) + SA::SLOTS * 16, wave, lane);
+        i8_dma<BM, NS>(Ag, P.m, P.kp, i0, kb, stage_ptr(0), wave, lane);
+        i8_dma<BN, NS>(Bg, P.n, P.kp, j0, kb, stage_ptr(0) + SA::SLOTS * 16, wave, lane);
     }
     for (int64_t st = 0; st < nst; ++st) {
         int const buf = int(st & 1);
         dma_wait_all();
         if (st + 1 < nst) {
-            i8_dma<BM, NS>(Ag, P.m, P.kp, i0, (st + 1) * I8_KS, stage_ptr(buf ^ 1), wave, lane);
-            i8_dma<BN, NS>(Bg, P.n, P.kp, j0, (st + 1) * I8_KS, stage_ptr(buf ^ 1) + SA::SLOTS * 16, wave, lane);
+            i8_dma<BM, NS>(Ag, P.m, P.kp, i0, kb + (st + 1) * I8_KS, stage_ptr(buf ^ 1), wave, lane);
+            i8_dma<BN, NS>(Bg, P.n, P.kp, j0, kb + (st + 1) * I8_KS, stage_ptr(buf ^ 1) + SA::SLOTS * 16, wave, lane);
         }
         v4i const *As = reinterpret_cast<v4i const *>(stage_ptr(buf));
         v4i const *Bs = As + SA::SLOTS;
@@ -454,23 +448,28 @@ __global__ __launch_bounds__(256, 1) void ozaki_i8_kernel(OzProblem P) {
                     for (int b = 0; b < FN; ++b)
                         acc[t + u][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bfr[u][b], af[a], acc[t + u][a][b], 0, 0, 0);
         }
-        if constexpr (MULTI) {
-            if (((st + 1) * I8_KS) % I8_KC == 0 && st + 1 < nst) {
-#pragma unroll
-                for (int a = 0; a < FM; ++a)
-#pragma unroll
-                    for (int b = 0; b < FN; ++b)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            accd[a][b][v] += diag_sum(a, b, v);
-#pragma unroll
-                            for (int d = 0; d < NS; ++d) acc[d][a][b][v] = 0;
-                        }
-            }
-        }
     }
 
-    // epilogue: result layout D[4*(lane/16)+v][lane%16]; rows of D run along j
+    // weighted sum of the diagonals of block (a, b), element v
+    auto diag_sum = [&](int a, int b, int v) __attribute__((always_inline)) {
+        double x = 0.0;
+#pragma unroll
+        for (int d = NS - 1; d >= 0; --d) x += double(acc[d][a][b][v]) * ldexp(1.0, -I8_BETA * (d + 2));
+        return x;
+    };
+
+    // result layout D[4*(lane/16)+v][lane%16]; rows of D run along j
+    if (P.splits > 1) {
+        double *__restrict__ part = P.partial + (int64_t(split) * ntiles + tile) * (BM * BN);
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+            for (int b = 0; b < FN; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    part[(wn * 32 + b * 16 + q * 4 + v) * BM + wm * 64 + a * 16 + r] = diag_sum(a, b, v);
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < FM; ++a) {
         int64_t i = i0 + wm * 64 + a * 16 + r;
@@ -482,34 +481,58 @@ __global__ __launch_bounds__(256, 1) void ozaki_i8_kernel(OzProblem P) {
             for (int v = 0; v < 4; ++v) {
                 int64_t j = j0 + wn * 32 + b * 16 + q * 4 + v;
                 if (j >= P.n) continue;
-                double *cp  = P.C + i * P.c_sm + j * P.c_sn;
-                double tot  = diag_sum(a, b, v);
-                if constexpr (MULTI) tot += accd[a][b][v];
-                double val = P.alpha * ldexp(tot, ei + P.eb[j]);
+                double *cp = P.C + i * P.c_sm + j * P.c_sn;
+                double val = P.alpha * ldexp(diag_sum(a, b, v), ei + P.eb[j]);
                 if (P.beta != 0.0) val += P.beta * *cp;
                 *cp = val;
             }
     }
 }
 
-template <int NS>
-void launch_i8(OzProblem const &Q, dim3 grid, hipStream_t stream) {
-    if (Q.kp > I8_KC)
-        ozaki_i8_kernel<NS, true><<<grid, 256, 0, stream>>>(Q);
-    else
-        ozaki_i8_kernel<NS, false><<<grid, 256, 0, stream>>>(Q);
+// Sums the split partial tiles in split order and applies the epilogue; 256 elements of
+// one 128 x 64 tile per workgroup.
+__global__ __launch_bounds__(256) void ozaki_i8_reduce(OzProblem P) {
+    constexpr int BM = 128, BN = 64;
+    uint32_t const ntiles = P.tiles_m * P.tiles_n;
+    uint32_t const tile   = blockIdx.x / (BM * BN / 256);
+    int const e           = int(blockIdx.x % (BM * BN / 256)) * 256 + int(threadIdx.x);
+    int const ii = e % BM, jj = e / BM;
+    uint32_t tm, tn;
+    tile_coords(tile, P.tiles_m, P.tiles_n, tm, tn);
+    int64_t const i = int64_t(tm) * BM + ii, j = int64_t(tn) * BN + jj;
+    if (i >= P.m || j >= P.n) return;
+    double sum = 0.0;
+    for (int s = 0; s < P.splits; ++s) sum += P.partial[(int64_t(s) * ntiles + tile) * (BM * BN) + e];
+    double *cp = P.C + i * P.c_sm + j * P.c_sn;
+    double val = P.alpha * ldexp(sum, P.ea[i] + P.eb[j]);
+    if (P.beta != 0.0) val += P.beta * *cp;
+    *cp = val;
 }
 
 int gemm_split_i8(GemmProblem<double> P, int ns, hipStream_t stream) {
-    int64_t const kp      = ceil_div(P.k, I8_KS) * I8_KS;
+    int64_t const kp        = ceil_div(P.k, I8_KS) * I8_KS;
+    uint32_t const tiles_m  = uint32_t(ceil_div(P.m, 128));
+    uint32_t const tiles_n  = uint32_t(ceil_div(P.n, 64));
+    uint64_t const ntiles   = uint64_t(tiles_m) * tiles_n;
+    // splits: enough for exact int32 accumulation (chunk <= I8_KC), and enough workgroups
+    // to fill the GPU (one per CU resident) while every chunk keeps >= 1024 steps
+    int64_t splits = ceil_div(kp, I8_KC);
+    int64_t const want = ceil_div(int64_t(num_cus()) * 2, int64_t(ntiles));
+    int64_t const cap  = std::max<int64_t>(1, kp / 1024);
+    splits             = std::max(splits, std::min(want, cap));
+    int64_t const kchunk = ceil_div(ceil_div(kp, splits), I8_KS) * I8_KS;
+    splits               = ceil_div(kp, kchunk);
+    if (ntiles * uint64_t(splits) > 0x7fffffffu) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: tile grid too large");
+
     size_t const a_bytes  = size_t(ns) * size_t(P.m) * size_t(kp);
     size_t const b_bytes  = size_t(ns) * size_t(P.n) * size_t(kp);
     size_t const b_off    = (a_bytes + 255) / 256 * 256;
     size_t const ea_off   = b_off + (b_bytes + 255) / 256 * 256;
     size_t const eb_off   = ea_off + (size_t(P.m) * 4 + 255) / 256 * 256;
-    size_t const total    = eb_off + (size_t(P.n) * 4 + 255) / 256 * 256;
+    size_t const p_off    = eb_off + (size_t(P.n) * 4 + 255) / 256 * 256;
+    size_t const p_bytes  = splits > 1 ? size_t(splits) * size_t(ntiles) * 128 * 64 * sizeof(double) : 0;
     void *ws = nullptr;
-    EIN_TRY(workspace(stream, total, &ws));
+    EIN_TRY(workspace(stream, p_off + p_bytes, &ws));
     char *base  = static_cast<char *>(ws);
     int8_t *As  = reinterpret_cast<int8_t *>(base);
     int8_t *Bs  = reinterpret_cast<int8_t *>(base + b_off);
@@ -545,18 +568,24 @@ int gemm_split_i8(GemmProblem<double> P, int ns, hipStream_t stream) {
     Q.c_sn    = P.c_sn;
     Q.alpha   = P.alpha;
     Q.beta    = P.beta;
-    Q.tiles_m = uint32_t(ceil_div(P.m, 128));
-    Q.tiles_n = uint32_t(ceil_div(P.n, 64));
-    if (uint64_t(Q.tiles_m) * Q.tiles_n > 0x7fffffffu) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: tile grid too large");
-    dim3 grid{Q.tiles_m * Q.tiles_n};
+    Q.tiles_m = tiles_m;
+    Q.tiles_n = tiles_n;
+    Q.kchunk  = kchunk;
+    Q.splits  = int32_t(splits);
+    Q.partial = splits > 1 ? reinterpret_cast<double *>(base + p_off) : nullptr;
+    dim3 grid{uint32_t(ntiles * uint64_t(splits))};
     switch (ns) {
-    case 3: launch_i8<3>(Q, grid, stream); break;
-    case 4: launch_i8<4>(Q, grid, stream); break;
-    case 5: launch_i8<5>(Q, grid, stream); break;
-    case 6: launch_i8<6>(Q, grid, stream); break;
+    case 3: ozaki_i8_kernel<3><<<grid, 256, 0, stream>>>(Q); break;
+    case 4: ozaki_i8_kernel<4><<<grid, 256, 0, stream>>>(Q); break;
+    case 5: ozaki_i8_kernel<5><<<grid, 256, 0, stream>>>(Q); break;
+    case 6: ozaki_i8_kernel<6><<<grid, 256, 0, stream>>>(Q); break;
     default: return fail(EIN_ERROR_VALUE, "split gemm: int8 slices must number 3..6");
     }
     EIN_LAUNCH_CHECK();
+    if (splits > 1) {
+        ozaki_i8_reduce<<<uint32_t(ntiles * (128 * 64 / 256)), 256, 0, stream>>>(Q);
+        EIN_LAUNCH_CHECK();
+    }
     return EIN_SUCCESS;
 }
 

@@ -22,6 +22,12 @@ struct OzProblem {
     int64_t c_sm, c_sn;
     double alpha, beta;
     uint32_t tiles_m, tiles_n;
+    // split-K (int8 path): split s covers contraction steps [s*kchunk, min(kp, (s+1)*kchunk));
+    // with splits > 1 the partial tiles go to `partial` and a reduction applies the epilogue
+    int64_t kchunk;
+    int32_t splits;
+    int32_t pad_;
+    double *partial;
 };
 
 int gemm_split(GemmProblem<double> P, int nslices, int slice_type, hipStream_t stream);
