@@ -1022,10 +1022,14 @@ int plan_create(ein_contraction_item const *items, int count, GroupedPlan &plan,
         placed.push_back(me);
         plan.items.push_back(items[i]);
         plan.wave_of.push_back(me.wave);
-        // an item with at least one 128 x 128 tile per CU keeps its own tuned kernels (wide
-        // DMA, 128 x 128 tiles, stream-K); the persistent kernel takes the many small ones
-        if (kind == 0 && gi[0].P.m >= 128 && gi[0].P.n >= 128 &&
-            ceil_div(gi[0].P.m, 128) * ceil_div(gi[0].P.n, 128) * gi[0].P.batch >= int64_t(num_cus())) {
+        // Items with 16 or more 128 x 128 tiles keep their own tuned kernels (wide DMA,
+        // 128 x 128 tiles, stream-K over all CUs), and so do items with a long contraction
+        // (k >= 2048), whose few tiles would each run k/16 slabs on one CU here while
+        // stream-K splits them over the GPU. The persistent kernel takes the many small,
+        // short items, for which one launch each would cost more than the work.
+        bool const big_tiles = gi[0].P.m >= 128 && gi[0].P.n >= 128 &&
+                               ceil_div(gi[0].P.m, 128) * ceil_div(gi[0].P.n, 128) * gi[0].P.batch >= 16;
+        if (kind == 0 && (big_tiles || gi[0].P.k >= 2048)) {
             plan.waves[w].large.push_back(items[i]);
             continue;
         }

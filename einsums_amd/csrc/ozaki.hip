@@ -59,11 +59,31 @@ __device__ __forceinline__ int row_exponent(double mx, bool guard) {
     return ex;
 }
 
-__global__ __launch_bounds__(256) void row_exp_rows_fast(SliceSrc S, int32_t *__restrict__ e, bool guard) {
-    int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+
+// Rows-fast operands: the maximum of each row over chunk `blockIdx.y` of kch contraction
+// steps (64 rows x 4 k-lanes per workgroup), then a second pass over the chunks. A single
+// thread per row walking all of k would leave the GPU idle for long k.
+__global__ __launch_bounds__(256) void row_max_partial(SliceSrc S, int64_t kch, double *__restrict__ part) {
+    __shared__ double red[4][64];
+    int const t      = threadIdx.x;
+    int64_t const r  = int64_t(blockIdx.x) * 64 + (t & 63);
+    int64_t const p0 = int64_t(blockIdx.y) * kch;
+    int64_t const p1 = p0 + kch < S.k ? p0 + kch : S.k;
+    double mx        = 0.0;
+    if (r < S.rows)
+        for (int64_t p = p0 + (t >> 6); p < p1; p += 4) mx = fmax(mx, fabs(S.X[r * S.s_row + p * S.s_k]));
+    red[t >> 6][t & 63] = mx;
+    __syncthreads();
+    if (t < 64 && r < S.rows)
+        part[int64_t(blockIdx.y) * S.rows + r] = fmax(fmax(red[0][t], red[1][t]), fmax(red[2][t], red[3][t]));
+}
+
+__global__ __launch_bounds__(256) void row_exp_finish(SliceSrc S, double const *__restrict__ part, int nck,
+                                                      int32_t *__restrict__ e, bool guard) {
+    int64_t const r = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (r >= S.rows) return;
     double mx = 0.0;
-    for (int64_t p = 0; p < S.k; ++p) mx = fmax(mx, fabs(S.X[r * S.s_row + p * S.s_k]));
+    for (int c = 0; c < nck; ++c) mx = fmax(mx, part[int64_t(c) * S.rows + r]);
     e[r] = row_exponent(mx, guard);
 }
 
@@ -76,6 +96,30 @@ __global__ __launch_bounds__(256) void row_exp_k_fast(SliceSrc S, int32_t *__res
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
     if (lane == 0) e[r] = row_exponent(mx, guard);
+}
+
+// Scratch doubles row_exponents needs for `rows` rows.
+inline size_t row_exp_scratch(int64_t rows) { return size_t(rows) + 65536; }
+
+// Per-row exponents of a strided operand (e[r] with max_p |X(r,p)| < 2^e[r]).
+int row_exponents(SliceSrc const &S, int32_t *e, bool guard, double *scratch, hipStream_t stream) {
+    bool const rows_fast = S.s_row == 1 || (S.s_k != 1 && std::llabs(S.s_row) < std::llabs(S.s_k));
+    if (!rows_fast) {
+        row_exp_k_fast<<<uint32_t(ceil_div(S.rows, 4)), 256, 0, stream>>>(S, e, guard);
+        EIN_LAUNCH_CHECK();
+        return EIN_SUCCESS;
+    }
+    int64_t const rb = ceil_div(S.rows, 64);
+    int64_t nck      = std::max<int64_t>(1, std::min<int64_t>(ceil_div(int64_t(num_cus()) * 4, rb), 64));
+    nck              = std::min<int64_t>(nck, std::max<int64_t>(1, ceil_div(S.k, 256)));
+    while (nck > 1 && size_t(nck) * size_t(S.rows) > row_exp_scratch(S.rows)) --nck;
+    int64_t const kch = ceil_div(std::max<int64_t>(S.k, 1), nck);
+    if (rb > 0x7fffffff) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: too many rows");
+    row_max_partial<<<dim3(uint32_t(rb), uint32_t(nck)), 256, 0, stream>>>(S, kch, scratch);
+    EIN_LAUNCH_CHECK();
+    row_exp_finish<<<uint32_t(ceil_div(S.rows, 256)), 256, 0, stream>>>(S, scratch, int(nck), e, guard);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
 }
 
 // ----------------------------------------------------------------- slicing
@@ -529,7 +573,8 @@ int gemm_split_i8(GemmProblem<double> P, int ns, hipStream_t stream) {
     size_t const b_off    = (a_bytes + 255) / 256 * 256;
     size_t const ea_off   = b_off + (b_bytes + 255) / 256 * 256;
     size_t const eb_off   = ea_off + (size_t(P.m) * 4 + 255) / 256 * 256;
-    size_t const p_off    = eb_off + (size_t(P.n) * 4 + 255) / 256 * 256;
+    size_t const x_off    = eb_off + (size_t(P.n) * 4 + 255) / 256 * 256;
+    size_t const p_off    = x_off + (row_exp_scratch(std::max(P.m, P.n)) * sizeof(double) + 255) / 256 * 256;
     size_t const p_bytes  = splits > 1 ? size_t(splits) * size_t(ntiles) * 128 * 64 * sizeof(double) : 0;
     void *ws = nullptr;
     EIN_TRY(workspace(stream, p_off + p_bytes, &ws));
@@ -540,13 +585,10 @@ int gemm_split_i8(GemmProblem<double> P, int ns, hipStream_t stream) {
     int32_t *eb = reinterpret_cast<int32_t *>(base + eb_off);
     SliceSrc SA{P.m, P.k, kp, P.A, P.a_sm, P.a_sk};
     SliceSrc SB{P.n, P.k, kp, P.B, P.b_sn, P.b_sk};
+    double *scratch = reinterpret_cast<double *>(base + x_off);
     for (auto const &[S, e, out] : {std::tuple<SliceSrc, int32_t *, int8_t *>{SA, ea, As}, {SB, eb, Bs}}) {
         bool const rows_fast = S.s_row == 1 || (S.s_k != 1 && std::llabs(S.s_row) < std::llabs(S.s_k));
-        if (rows_fast)
-            row_exp_rows_fast<<<uint32_t(ceil_div(S.rows, 256)), 256, 0, stream>>>(S, e, true);
-        else
-            row_exp_k_fast<<<uint32_t(ceil_div(S.rows, 4)), 256, 0, stream>>>(S, e, true);
-        EIN_LAUNCH_CHECK();
+        EIN_TRY(row_exponents(S, e, true, scratch, stream));
         dim3 grid{uint32_t(ceil_div(S.kp, 64)), uint32_t(ceil_div(S.rows, 64))};
         if (grid.y > 65535) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: more than 4M rows");
         if (rows_fast)
@@ -605,14 +647,9 @@ void launch_oz(OzProblem const &P, int bm, hipStream_t stream) {
 }
 
 template <typename H>
-int slice_operand(SliceSrc const &S, int ns, int32_t *e, H *out, hipStream_t stream) {
+int slice_operand(SliceSrc const &S, int ns, int32_t *e, H *out, double *scratch, hipStream_t stream) {
     bool const rows_fast = S.s_row == 1 || (S.s_k != 1 && std::llabs(S.s_row) < std::llabs(S.s_k));
-    if (rows_fast) {
-        row_exp_rows_fast<<<uint32_t(ceil_div(S.rows, 256)), 256, 0, stream>>>(S, e, false);
-    } else {
-        row_exp_k_fast<<<uint32_t(ceil_div(S.rows, 4)), 256, 0, stream>>>(S, e, false);
-    }
-    EIN_LAUNCH_CHECK();
+    EIN_TRY(row_exponents(S, e, false, scratch, stream));
     dim3 grid{uint32_t(ceil_div(S.kp, 64)), uint32_t(ceil_div(S.rows, 64))};
     if (grid.y > 65535) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: more than 4M rows");
     if (rows_fast)
@@ -632,7 +669,8 @@ int gemm_split_typed(GemmProblem<double> P, int ns, int bm, hipStream_t stream) 
     size_t const b_off = (a_bytes + 255) / 256 * 256;
     size_t const ea_off = b_off + (b_bytes + 255) / 256 * 256;
     size_t const eb_off = ea_off + (size_t(P.m) * 4 + 255) / 256 * 256;
-    size_t const total = eb_off + (size_t(P.n) * 4 + 255) / 256 * 256;
+    size_t const x_off = eb_off + (size_t(P.n) * 4 + 255) / 256 * 256;
+    size_t const total = x_off + row_exp_scratch(std::max(P.m, P.n)) * sizeof(double);
     void *ws = nullptr;
     EIN_TRY(workspace(stream, total, &ws));
     char *base = static_cast<char *>(ws);
@@ -640,11 +678,12 @@ int gemm_split_typed(GemmProblem<double> P, int ns, int bm, hipStream_t stream) 
     H *Bs = reinterpret_cast<H *>(base + b_off);
     int32_t *ea = reinterpret_cast<int32_t *>(base + ea_off);
     int32_t *eb = reinterpret_cast<int32_t *>(base + eb_off);
+    double *scratch = reinterpret_cast<double *>(base + x_off);
 
     SliceSrc SA{P.m, P.k, kp, P.A, P.a_sm, P.a_sk};
     SliceSrc SB{P.n, P.k, kp, P.B, P.b_sn, P.b_sk};
-    EIN_TRY(slice_operand<H>(SA, ns, ea, As, stream));
-    EIN_TRY(slice_operand<H>(SB, ns, eb, Bs, stream));
+    EIN_TRY(slice_operand<H>(SA, ns, ea, As, scratch, stream));
+    EIN_TRY(slice_operand<H>(SB, ns, eb, Bs, scratch, stream));
 
     OzProblem Q{};
     Q.m = P.m;
