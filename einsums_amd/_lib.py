@@ -26,6 +26,7 @@ EIN_ERROR_VALUE = 1007
 # detail::AlgorithmChoice numbering (TensorAlgebra/Detail/Utilities.hpp:26-34)
 GENERIC, DOT, DIRECT, GER, GEMV, GEMM, INDETERMINATE = range(7)
 SLICE_BF16, SLICE_FP16, SLICE_INT8 = 0, 1, 2
+ROUTE_AUTO, ROUTE_GATHER, ROUTE_TTGT = 0, 1, 2
 ALGORITHM_NAMES = {GENERIC: "GENERIC", DOT: "DOT", DIRECT: "DIRECT", GER: "GER", GEMV: "GEMV", GEMM: "GEMM",
                    INDETERMINATE: "INDETERMINATE"}
 
@@ -145,6 +146,7 @@ def _load():
     sig["ein_grouped_plan_info"] = (_c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int64)])
     sig["ein_grouped_plan_destroy"] = (_c_int, [_vp])
+    sig["ein_set_contraction_route"] = (ctypes.c_int32, [ctypes.c_int32])
     sig["ein_einsum_classify"] = (_c_int, [_desc_p, _desc_p, _desc_p, ctypes.POINTER(ctypes.c_int32)])
     for name, (res, args) in sig.items():
         _proto(lib, name, res, args)

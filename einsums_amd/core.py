@@ -12,6 +12,7 @@ There is no CPU path: every execute runs HIP kernels from libeinsums_amd.so.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 from typing import Any, Sequence
 
@@ -25,7 +26,7 @@ from ._lib import (DOT, DIRECT, GEMM, GEMV, GENERIC, GER, INDETERMINATE, ALGORIT
 __all__ = [
     "compile_plan", "EinsumGenericPlan", "EinsumDotPlan", "EinsumDirectProductPlan", "EinsumGerPlan",
     "EinsumGemvPlan", "EinsumGemmPlan", "GPUView", "COPY", "MAP", "DEVICE_TENSOR", "gpu_enabled", "einsum",
-    "permute", "stream_handle", "DimensionError", "RankError", "EinsumsError",
+    "permute", "stream_handle", "DimensionError", "RankError", "EinsumsError", "contraction_route",
 ]
 
 # GPUView modes (libs/EinsumsPy/GPU/include/EinsumsPy/GPU/PyGPUView.hpp:27-46)
@@ -276,6 +277,23 @@ def gemm_split(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch
     status = lib.ein_gemm_split_f64(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0), _ptr(A),
                                     A.stride(0), float(beta), _ptr(C), C.stride(0), int(nslices), st, stream_handle(stream))
     check(status, "gemm_split")
+
+
+_ROUTES = {"auto": _lib.ROUTE_AUTO, "gather": _lib.ROUTE_GATHER, "ttgt": _lib.ROUTE_TTGT}
+
+
+@contextlib.contextmanager
+def contraction_route(route: str):
+    """Within the block, generic contractions of GEMM size on this thread take ``route``:
+    "auto" (cost model), "gather" (MFMA strided-gather kernel) or "ttgt" (operands packed
+    by the permute, then the dense MFMA GEMM). The previous route is restored after."""
+    if route not in _ROUTES:
+        raise ValueError(f"contraction_route: unknown route {route!r}")
+    prev = lib.ein_set_contraction_route(_ROUTES[route])
+    try:
+        yield
+    finally:
+        lib.ein_set_contraction_route(prev)
 
 
 def algorithm_name(alg: int) -> str:

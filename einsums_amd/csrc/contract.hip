@@ -282,16 +282,25 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_gather_kernel(GatherProb
     }
 }
 
-// Stream-K fix-up for the gather GEMM: sums the pieces of every cut tile in workgroup
-// order and writes C through the decoded row/column offsets.
+// Stream-K fix-up for the gather GEMM (launch shape as the dense fix-up): sums the pieces
+// of every cut tile in workgroup order and writes C through the row/column offset tables.
 template <typename T, int BM, int BN>
 __global__ __launch_bounds__(256) void gather_streamk_fixup(GatherProblem<T> P) {
     StreamK const &sk = P.sk;
-    uint64_t const t = uint64_t(sk.dp_tiles) + blockIdx.x / FIXUP_SPLIT;
-    int const part   = int(blockIdx.x % FIXUP_SPLIT);
+    uint64_t const t  = uint64_t(sk.dp_tiles) + blockIdx.y;
     uint64_t g0, g1;
     sk_owners(sk, t, g0, g1);
     if (g0 == g1) return;
+    int const e0     = int(blockIdx.x) * 256 + int(threadIdx.x);
+    int const stride = int(gridDim.x) * 256;
+    T sum[FIXUP_EPT];
+#pragma unroll
+    for (int x = 0; x < FIXUP_EPT; ++x) sum[x] = T(0);
+    for (uint64_t g = g0; g <= g1; ++g) {
+        T const *__restrict__ pc = P.partial + sk_slot(sk, g, t) * (BM * BN);
+#pragma unroll
+        for (int x = 0; x < FIXUP_EPT; ++x) sum[x] += pc[e0 + x * stride];
+    }
     uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
     uint32_t const bz       = uint32_t(t / tiles_mn);
     uint32_t tm, tn;
@@ -299,14 +308,13 @@ __global__ __launch_bounds__(256) void gather_streamk_fixup(GatherProblem<T> P) 
     int64_t const i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
     int64_t ob[3];
     group_decode<3>(bz, P.gb, ob);
-    for (int e = part * 256 + threadIdx.x; e < BM * BN; e += 256 * FIXUP_SPLIT) {
-        int ii = e % BM, jj = e / BM;
-        int64_t i = i0 + ii, j = j0 + jj;
+#pragma unroll
+    for (int x = 0; x < FIXUP_EPT; ++x) {
+        int const e = e0 + x * stride;
+        int64_t i = i0 + e % BM, j = j0 + e / BM;
         if (i >= P.m || j >= P.n) continue;
-        T sum = T(0);
-        for (uint64_t g = g0; g <= g1; ++g) sum += P.partial[sk_slot(sk, g, t) * (BM * BN) + e];
         T *cp = P.C + ob[2] + P.rowC[i] + P.colC[j];
-        T val = P.alpha * sum;
+        T val = P.alpha * sum[x];
         if (P.beta != T(0)) val += P.beta * *cp;
         *cp = val;
     }
@@ -530,6 +538,194 @@ void set_tables(GatherProblem<T> &P, int64_t const *tab) {
 
 } // namespace
 
+namespace {
+
+// ====================================================== transpose-transpose-GEMM route
+
+thread_local int32_t t_route = EIN_ROUTE_AUTO;
+
+// Outer -> inner order of the digits by descending |stride| in operand slot o.
+std::vector<Digit> sorted_by(std::vector<Digit> v, int o) {
+    std::stable_sort(v.begin(), v.end(),
+                     [&](Digit const &x, Digit const &y) { return std::llabs(x.s[o]) > std::llabs(y.s[o]); });
+    return v;
+}
+
+// The digits (outer -> inner) address one axis of operand slot o with a single positive
+// stride: each stride is the next one's times the next extent. No digits: extent 1.
+bool folds_to_axis(std::vector<Digit> const &v, int o, int64_t &stride) {
+    stride = 0;
+    if (v.empty()) return true;
+    for (size_t d = 0; d + 1 < v.size(); ++d)
+        if (v[d].s[o] != v[d + 1].s[o] * v[d + 1].e) return false;
+    stride = v.back().s[o];
+    return stride > 0;
+}
+
+// dst (dense, the listed digits outermost first) = src, src addressed through slot o of
+// each group's digits.
+template <typename T>
+int pack_digits(T const *src, std::vector<std::pair<std::vector<Digit> const *, int>> const &groups, T *dst,
+                hipStream_t stream) {
+    std::vector<int64_t> dims, ss, sd;
+    for (auto const &g : groups)
+        for (auto const &d : *g.first) {
+            dims.push_back(d.e);
+            ss.push_back(d.s[g.second]);
+        }
+    if (dims.empty()) {
+        dims.push_back(1);
+        ss.push_back(1);
+    }
+    sd.resize(dims.size());
+    int64_t pv = 1;
+    for (size_t i = dims.size(); i-- > 0;) {
+        sd[i] = pv;
+        pv *= dims[i];
+    }
+    std::vector<int32_t> perm(dims.size());
+    std::iota(perm.begin(), perm.end(), 0);
+    return permute_tensor<T>(int32_t(dims.size()), dims.data(), perm.data(), T(1), src, ss.data(), T(0), dst, sd.data(),
+                             stream);
+}
+
+// Transpose-transpose-GEMM: an operand whose digits do not fold into one strided matrix is
+// packed by the LDS-tiled permute into [batch][rows][k] (k fastest), the product runs on
+// the dense MFMA GEMM (or the split-precision GEMM when that mode is on), and an output
+// that does not fold is formed in a dense buffer X ([batch][N][M], M fastest) and added
+// into C by the permute. Digit orders are chosen to fold as much as possible. The route
+// is taken (`taken` = true) when its extra traffic and launches cost less than the
+// measured gap between the gather GEMM and the dense GEMM (the gather kernel takes about
+// 1.45x the dense kernel's time at 2304^3 FP64 on MI355X).
+template <typename T>
+int contract_ttgt(std::vector<Digit> const &vb, std::vector<Digit> const &vm, std::vector<Digit> const &vn,
+                  std::vector<Digit> const &vk, T alpha, T const *a, T const *b, T beta, T *c, hipStream_t stream,
+                  bool &taken) {
+    taken = false;
+    if (t_route == EIN_ROUTE_GATHER) return EIN_SUCCESS;
+    bool const forced = t_route == EIN_ROUTE_TTGT;
+    if (vb.size() + vm.size() + vk.size() > size_t(EIN_MAX_RANK) || vb.size() + vn.size() + vk.size() > size_t(EIN_MAX_RANK) ||
+        vb.size() + vm.size() + vn.size() > size_t(EIN_MAX_RANK))
+        return EIN_SUCCESS;
+    int64_t const M = total_of(vm), N = total_of(vn), K = total_of(vk), NB = total_of(vb);
+    struct Choice {
+        std::vector<Digit> m, n, k, bt;
+        bool a_ok, b_ok, c_ok;
+        int64_t as[3], bs[3], cs[3];
+        double moved; // elements written and read again by packing / staging
+    };
+    Choice best{};
+    best.moved = -1.0;
+    std::vector<Digit> const bt = sorted_by(vb, 2);
+    for (int om = 0; om < 2; ++om)         // M digits ordered by A (0) or C (1) strides
+        for (int on = 0; on < 2; ++on)     // N digits by B (0) or C (1)
+            for (int ok = 0; ok < 2; ++ok) { // K digits by A (0) or B (1)
+                Choice ch;
+                ch.bt   = bt;
+                ch.m    = sorted_by(vm, om);
+                ch.n    = sorted_by(vn, on);
+                ch.k    = sorted_by(vk, ok);
+                ch.a_ok = folds_to_axis(ch.bt, 0, ch.as[0]) && folds_to_axis(ch.m, 0, ch.as[1]) &&
+                          folds_to_axis(ch.k, 0, ch.as[2]);
+                ch.b_ok = folds_to_axis(ch.bt, 1, ch.bs[0]) && folds_to_axis(ch.n, 0, ch.bs[1]) &&
+                          folds_to_axis(ch.k, 1, ch.bs[2]);
+                ch.c_ok = folds_to_axis(ch.bt, 2, ch.cs[0]) && folds_to_axis(ch.m, 1, ch.cs[1]) &&
+                          folds_to_axis(ch.n, 1, ch.cs[2]);
+                ch.moved = (ch.a_ok ? 0.0 : double(NB) * M * K) + (ch.b_ok ? 0.0 : double(NB) * N * K) +
+                           (ch.c_ok ? 0.0 : double(NB) * M * N);
+                if (best.moved < 0.0 || ch.moved < best.moved) best = ch;
+            }
+    int const launches   = int(!best.a_ok) + int(!best.b_ok) + int(!best.c_ok);
+    double const t_gemm  = 2.0 * double(NB) * M * N * K / (sizeof(T) == 8 ? 55e12 : 100e12);
+    double const t_extra = 2.0 * sizeof(T) * best.moved / 4e12 + 5e-6 * launches;
+    if (!forced && t_extra >= 0.45 * t_gemm) return EIN_SUCCESS;
+
+    auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+    size_t const a_bytes = best.a_ok ? 0 : align(size_t(NB * M * K) * sizeof(T));
+    size_t const b_bytes = best.b_ok ? 0 : align(size_t(NB * N * K) * sizeof(T));
+    size_t const x_bytes = best.c_ok ? 0 : align(size_t(NB * M * N) * sizeof(T));
+    size_t const total   = a_bytes + b_bytes + x_bytes;
+    char *ws             = nullptr;
+    if (total > 0) {
+        // under stream capture the buffer must already exist; otherwise the gather runs
+        if (!forced && stream_capturing(stream) && !workspace_fits(stream, 1, total)) return EIN_SUCCESS;
+        void *p = nullptr;
+        EIN_TRY(workspace_slot(stream, 1, total, &p));
+        ws = static_cast<char *>(p);
+    }
+    taken = true;
+    GemmProblem<T> P{};
+    P.m     = M;
+    P.n     = N;
+    P.k     = K;
+    P.batch = NB;
+    P.alpha = alpha;
+    if (best.a_ok) {
+        P.A    = a;
+        P.a_sb = best.as[0];
+        P.a_sm = best.as[1];
+        P.a_sk = best.as[2];
+    } else {
+        T *pa = reinterpret_cast<T *>(ws);
+        EIN_TRY(pack_digits<T>(a, {{&best.bt, 0}, {&best.m, 0}, {&best.k, 0}}, pa, stream));
+        P.A    = pa;
+        P.a_sb = M * K;
+        P.a_sm = K;
+        P.a_sk = 1;
+    }
+    if (best.b_ok) {
+        P.B    = b;
+        P.b_sb = best.bs[0];
+        P.b_sn = best.bs[1];
+        P.b_sk = best.bs[2];
+    } else {
+        T *pb = reinterpret_cast<T *>(ws + a_bytes);
+        EIN_TRY(pack_digits<T>(b, {{&best.bt, 1}, {&best.n, 0}, {&best.k, 1}}, pb, stream));
+        P.B    = pb;
+        P.b_sb = N * K;
+        P.b_sn = K;
+        P.b_sk = 1;
+    }
+    T *X = best.c_ok ? nullptr : reinterpret_cast<T *>(ws + a_bytes + b_bytes);
+    if (best.c_ok) {
+        P.C    = c;
+        P.c_sb = best.cs[0];
+        P.c_sm = best.cs[1];
+        P.c_sn = best.cs[2];
+        P.beta = beta;
+    } else {
+        P.C    = X;
+        P.c_sb = M * N;
+        P.c_sm = 1;
+        P.c_sn = M;
+        P.beta = T(0);
+    }
+    EIN_TRY(gemm_dispatch<T>(P, stream));
+    if (!best.c_ok) {
+        // C = beta*C + X over the (batch, N, M) digits
+        std::vector<int64_t> dims, sx, sc;
+        int64_t pv = 1;
+        std::vector<std::pair<std::vector<Digit> const *, int>> const order = {{&best.bt, 2}, {&best.n, 1}, {&best.m, 1}};
+        for (auto const &g : order)
+            for (auto const &d : *g.first) {
+                dims.push_back(d.e);
+                sc.push_back(d.s[g.second]);
+            }
+        sx.resize(dims.size());
+        for (size_t i = dims.size(); i-- > 0;) {
+            sx[i] = pv;
+            pv *= dims[i];
+        }
+        std::vector<int32_t> perm(dims.size());
+        std::iota(perm.begin(), perm.end(), 0);
+        EIN_TRY(permute_tensor<T>(int32_t(dims.size()), dims.data(), perm.data(), T(1), X, sx.data(), beta, c, sc.data(),
+                                  stream));
+    }
+    return EIN_SUCCESS;
+}
+
+} // namespace
+
 template <typename T>
 int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefactor, ein_tensor_desc const &A, T const *a,
                      ein_tensor_desc const &B, T const *b, hipStream_t stream) {
@@ -574,6 +770,9 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
                           K < (int64_t{1} << 31) && NB <= 65535;
 
     if (use_mfma) {
+        bool taken = false;
+        EIN_TRY(contract_ttgt<T>(vb, vm, vn, vk, ab_prefactor, a, b, beta, c, stream, taken));
+        if (taken) return EIN_SUCCESS;
         auto gm = order_and_fold(vm, 2, 0, 1);
         auto gn = order_and_fold(vn, 2, 0, 1);
         bool const a_i_fast = !gm.empty() && std::llabs(gm.back().s[0]) == 1;
@@ -663,11 +862,11 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         EIN_HIP_CHECK(hipLaunchKernel(kern, dim3(P.sk.grid), dim3(GEMM_THREADS), args, 0, stream));
         if (P.sk.units != 0) {
             if (bm == 128 && bn == 128)
-                gather_streamk_fixup<T, 128, 128><<<uint32_t(tiles - P.sk.dp_tiles) * FIXUP_SPLIT, 256, 0, stream>>>(P);
+                gather_streamk_fixup<T, 128, 128><<<sk_fixup_grid(P.sk, 128 * 128), 256, 0, stream>>>(P);
             else if (bm == 128)
-                gather_streamk_fixup<T, 128, 64><<<uint32_t(tiles - P.sk.dp_tiles) * FIXUP_SPLIT, 256, 0, stream>>>(P);
+                gather_streamk_fixup<T, 128, 64><<<sk_fixup_grid(P.sk, 128 * 64), 256, 0, stream>>>(P);
             else
-                gather_streamk_fixup<T, 64, 64><<<uint32_t(tiles - P.sk.dp_tiles) * FIXUP_SPLIT, 256, 0, stream>>>(P);
+                gather_streamk_fixup<T, 64, 64><<<sk_fixup_grid(P.sk, 64 * 64), 256, 0, stream>>>(P);
             EIN_LAUNCH_CHECK();
         }
         if (stage) {
@@ -777,6 +976,16 @@ template int contract_generic<float>(float, ein_tensor_desc const &, float *, fl
 } // namespace ein
 
 extern "C" {
+
+int32_t ein_set_contraction_route(int32_t route) {
+    if (route != EIN_ROUTE_AUTO && route != EIN_ROUTE_GATHER && route != EIN_ROUTE_TTGT) {
+        ein::set_error("set_contraction_route: unknown route");
+        return -1;
+    }
+    int32_t const prev = ein::t_route;
+    ein::t_route       = route;
+    return prev;
+}
 
 int ein_contract_f64(double c_prefactor, const ein_tensor_desc *C, double *c, double ab_prefactor, const ein_tensor_desc *A,
                      const double *a, const ein_tensor_desc *B, const double *b, ein_stream_t stream) {

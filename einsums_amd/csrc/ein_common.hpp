@@ -43,6 +43,13 @@ int fail(int code, std::string const &msg);
 // hipMalloc outside stream capture; inside capture it fails with EIN_ERROR_WORKSPACE
 // unless ein_workspace_reserve() sized it beforehand.
 int workspace(hipStream_t stream, size_t bytes, void **out);
+// Same for a numbered buffer of the stream (slot 0 is the one above); a call that keeps
+// data in slot 1 can run nested calls that use slot 0.
+int workspace_slot(hipStream_t stream, int slot, size_t bytes, void **out);
+// Whether `stream` is being captured into a graph.
+bool stream_capturing(hipStream_t stream);
+// Whether slot `slot` of `stream` already holds `bytes` (no allocation needed).
+bool workspace_fits(hipStream_t stream, int slot, size_t bytes);
 
 inline hipStream_t as_stream(ein_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 

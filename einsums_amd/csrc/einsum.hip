@@ -25,12 +25,23 @@ namespace {
 thread_local int t_split_slices = 0;
 thread_local int t_split_type   = EIN_SLICE_BF16;
 
+} // namespace
+
 template <typename T>
-int run_gemm(GemmProblem<T> const &P, hipStream_t stream) {
+int gemm_dispatch(GemmProblem<T> const &P, hipStream_t stream) {
     if constexpr (std::is_same_v<T, double>) {
-        if (t_split_slices > 0) return gemm_split(P, t_split_slices, t_split_type, stream);
+        if (t_split_slices > 0 && P.batch == 1) return gemm_split(P, t_split_slices, t_split_type, stream);
     }
     return gemm_strided<T>(P, nullptr, stream);
+}
+template int gemm_dispatch<double>(GemmProblem<double> const &, hipStream_t);
+template int gemm_dispatch<float>(GemmProblem<float> const &, hipStream_t);
+
+namespace {
+
+template <typename T>
+int run_gemm(GemmProblem<T> const &P, hipStream_t stream) {
+    return gemm_dispatch<T>(P, stream);
 }
 
 using Labels = std::vector<int>;

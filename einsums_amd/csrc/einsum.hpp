@@ -13,4 +13,11 @@ template <typename T>
 int einsum_execute(int alg, T cp, ein_tensor_desc const &C, T *c, T abp, ein_tensor_desc const &A, T const *a,
                    ein_tensor_desc const &B, T const *b, int32_t *used, hipStream_t stream);
 
+// GEMM of a GEMM-mapped einsum: the split-precision GEMM when this thread's split mode is
+// on (FP64, unbatched), else the MFMA GEMM.
+template <typename T>
+struct GemmProblem;
+template <typename T>
+int gemm_dispatch(GemmProblem<T> const &P, hipStream_t stream);
+
 } // namespace ein

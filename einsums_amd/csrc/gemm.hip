@@ -146,30 +146,39 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
     }
 }
 
-// Stream-K fix-up: one workgroup per tile; tiles finished in place return at once. The
-// pieces of a cut tile are summed in workgroup order, then alpha and beta are applied.
+// Stream-K fix-up: grid y runs over the stream-K tiles, grid x over parts of one tile;
+// tiles finished in place return at once. Each thread sums the pieces of FIXUP_EPT
+// elements in workgroup order (deterministic), then applies alpha and beta.
 template <typename T, int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_streamk_fixup(GemmProblem<T> P) {
     StreamK const &sk = P.sk;
-    uint64_t const t = uint64_t(sk.dp_tiles) + blockIdx.x / FIXUP_SPLIT;
-    int const part   = int(blockIdx.x % FIXUP_SPLIT);
+    uint64_t const t  = uint64_t(sk.dp_tiles) + blockIdx.y;
     uint64_t g0, g1;
     sk_owners(sk, t, g0, g1);
     if (g0 == g1) return;
+    int const e0     = int(blockIdx.x) * 256 + int(threadIdx.x);
+    int const stride = int(gridDim.x) * 256;
+    T sum[FIXUP_EPT];
+#pragma unroll
+    for (int x = 0; x < FIXUP_EPT; ++x) sum[x] = T(0);
+    for (uint64_t g = g0; g <= g1; ++g) {
+        T const *__restrict__ pc = P.partial + sk_slot(sk, g, t) * (BM * BN);
+#pragma unroll
+        for (int x = 0; x < FIXUP_EPT; ++x) sum[x] += pc[e0 + x * stride];
+    }
     uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
     int64_t const batch     = int64_t(t / tiles_mn);
     uint32_t tm, tn;
     tile_coords(uint32_t(t % tiles_mn), P.tiles_m, P.tiles_n, tm, tn);
     int64_t const i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
     T *__restrict__ Cb = P.C + batch * P.c_sb;
-    for (int e = part * 256 + threadIdx.x; e < BM * BN; e += 256 * FIXUP_SPLIT) {
-        int ii = e % BM, jj = e / BM;
-        int64_t i = i0 + ii, j = j0 + jj;
+#pragma unroll
+    for (int x = 0; x < FIXUP_EPT; ++x) {
+        int const e = e0 + x * stride;
+        int64_t i = i0 + e % BM, j = j0 + e / BM;
         if (i >= P.m || j >= P.n) continue;
-        T sum = T(0);
-        for (uint64_t g = g0; g <= g1; ++g) sum += P.partial[sk_slot(sk, g, t) * (BM * BN) + e];
         T *cp = Cb + i * P.c_sm + j * P.c_sn;
-        T val = P.alpha * sum;
+        T val = P.alpha * sum[x];
         if (P.beta != T(0)) val += P.beta * *cp;
         *cp = val;
     }
@@ -223,8 +232,9 @@ void const *dense_kernel_for(int code) {
     }
 }
 
+// With `ws_bytes` set, only reports the workspace the launch would take (stream-K pieces).
 template <typename T, int BM, int BN, int BK>
-int launch_dense(GemmProblem<T> &P, hipStream_t stream) {
+int launch_dense(GemmProblem<T> &P, int blocks_per_cu, hipStream_t stream, size_t *ws_bytes) {
     P.tiles_m      = uint32_t(ceil_div(P.m, BM));
     P.tiles_n      = uint32_t(ceil_div(P.n, BN));
     uint64_t tiles = uint64_t(P.tiles_m) * P.tiles_n * uint64_t(P.batch);
@@ -236,18 +246,25 @@ int launch_dense(GemmProblem<T> &P, hipStream_t stream) {
     void const *kern = dense_kernel_for<T, BM, BN, BK>(code);
     if (kern == nullptr) return fail(EIN_ERROR_UNSUPPORTED, "gemm: operand staging not realisable for this tile");
     // resident workgroups: the stream-K schedule balances the last round over them
-    uint64_t slots = P.force_dp ? tiles : uint64_t(num_cus()) * uint64_t(resident_blocks(kern, GEMM_THREADS));
+    int const fit  = resident_blocks(kern, GEMM_THREADS);
+    int const bpc  = blocks_per_cu > 0 && blocks_per_cu < fit ? blocks_per_cu : fit;
+    uint64_t slots = P.force_dp ? tiles : uint64_t(num_cus()) * uint64_t(bpc);
     P.sk           = plan_stream_k(tiles, slabs, slots);
     P.partial      = nullptr;
+    size_t const piece_bytes = size_t(sk_piece_slots(P.sk)) * BM * BN * sizeof(T);
+    if (ws_bytes != nullptr) {
+        *ws_bytes = piece_bytes;
+        return EIN_SUCCESS;
+    }
     if (P.sk.units != 0) {
         void *ws = nullptr;
-        EIN_TRY(workspace(stream, size_t(sk_piece_slots(P.sk)) * BM * BN * sizeof(T), &ws));
+        EIN_TRY(workspace(stream, piece_bytes, &ws));
         P.partial = static_cast<T *>(ws);
     }
     void *args[] = {&P};
     EIN_HIP_CHECK(hipLaunchKernel(kern, dim3(P.sk.grid), dim3(GEMM_THREADS), args, 0, stream));
     if (P.sk.units != 0) {
-        gemm_streamk_fixup<T, BM, BN><<<uint32_t(tiles - P.sk.dp_tiles) * FIXUP_SPLIT, 256, 0, stream>>>(P);
+        gemm_streamk_fixup<T, BM, BN><<<sk_fixup_grid(P.sk, BM * BN), 256, 0, stream>>>(P);
         EIN_LAUNCH_CHECK();
     }
     return EIN_SUCCESS;
@@ -256,24 +273,28 @@ int launch_dense(GemmProblem<T> &P, hipStream_t stream) {
 } // namespace
 
 GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, int elem_bytes) {
-    (void)elem_bytes;
-    (void)k;
-    (void)batch;
     GemmConfig cfg;
-    // 128 x 128 tiles do four times the MFMA work of 64 x 64 per slab barrier; the
-    // stream-K schedule balances however few of them there are
-    if (m >= 128 && n >= 128) {
-        cfg.bm = 128;
-        cfg.bn = 128;
-    } else {
+    if (m < 128 || n < 128) {
         cfg.bm = 64;
         cfg.bn = 64;
+        return cfg;
     }
+    // 128 x 128 tiles do four times the MFMA work of 64 x 64 per slab barrier. Measured on
+    // MI355X (FP64): with fewer than a quarter round of them (576 x 576 x 9216) 128 x 64
+    // tiles balance better and halve the stream-K pieces; with many rounds of short-k tiles
+    // (9216 x 9216 x 576, beta = 1) one workgroup per CU is ~5% faster than two.
+    int64_t const cus = num_cus();
+    int64_t const t128 = ceil_div(m, 128) * ceil_div(n, 128) * batch;
+    if (t128 < cus / 4) cfg.bn = 64;
+    else if (elem_bytes == 8 && t128 >= 8 * cus && k <= 2048) cfg.blocks_per_cu = 1;
     return cfg;
 }
 
+namespace {
+
 template <typename T>
-int gemm_strided(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream) {
+int gemm_strided_impl(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream, size_t *ws_bytes) {
+    if (ws_bytes != nullptr) *ws_bytes = 0;
     if (P.m <= 0 || P.n <= 0 || P.batch <= 0) return EIN_SUCCESS;
     // Normalise so that C's unit-stride direction is i (lane-contiguous stores):
     // C^T = B^T A^T swaps the roles of the operands.
@@ -296,6 +317,7 @@ int gemm_strided(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream) 
     }
     int64_t const total = P.m * P.n * P.batch;
     if (P.k <= 0 || P.alpha == T(0)) {
+        if (ws_bytes != nullptr) return EIN_SUCCESS;
         int blocks = int(std::min<int64_t>(ceil_div(total, 256), 65536));
         scale_2d_kernel<T><<<blocks, 256, 0, stream>>>(P);
         EIN_LAUNCH_CHECK();
@@ -304,14 +326,28 @@ int gemm_strided(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream) 
     GemmConfig cfg = force ? *force : choose_gemm_config(P.m, P.n, P.k, P.batch, int(sizeof(T)));
     P.force_dp     = cfg.data_parallel;
     constexpr int BK = 16;
-    if (cfg.bm == 128 && cfg.bn == 128) return launch_dense<T, 128, 128, BK>(P, stream);
-    if (cfg.bm == 128 && cfg.bn == 64) return launch_dense<T, 128, 64, BK>(P, stream);
-    if (cfg.bm == 64 && cfg.bn == 128) return launch_dense<T, 64, 128, BK>(P, stream);
-    return launch_dense<T, 64, 64, BK>(P, stream);
+    if (cfg.bm == 128 && cfg.bn == 128) return launch_dense<T, 128, 128, BK>(P, cfg.blocks_per_cu, stream, ws_bytes);
+    if (cfg.bm == 128 && cfg.bn == 64) return launch_dense<T, 128, 64, BK>(P, cfg.blocks_per_cu, stream, ws_bytes);
+    if (cfg.bm == 64 && cfg.bn == 128) return launch_dense<T, 64, 128, BK>(P, cfg.blocks_per_cu, stream, ws_bytes);
+    return launch_dense<T, 64, 64, BK>(P, cfg.blocks_per_cu, stream, ws_bytes);
+}
+
+} // namespace
+
+template <typename T>
+int gemm_strided(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream) {
+    return gemm_strided_impl<T>(P, force, stream, nullptr);
+}
+
+template <typename T>
+int gemm_workspace_bytes(GemmProblem<T> const &P, size_t *bytes) {
+    return gemm_strided_impl<T>(P, nullptr, nullptr, bytes);
 }
 
 template int gemm_strided<double>(GemmProblem<double>, GemmConfig const *, hipStream_t);
 template int gemm_strided<float>(GemmProblem<float>, GemmConfig const *, hipStream_t);
+template int gemm_workspace_bytes<double>(GemmProblem<double> const &, size_t *);
+template int gemm_workspace_bytes<float>(GemmProblem<float> const &, size_t *);
 
 namespace {
 

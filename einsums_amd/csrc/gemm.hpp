@@ -26,6 +26,7 @@ struct GemmConfig {
     int bm            = 128;
     int bn            = 128;
     int data_parallel = 0; // 1: one tile per workgroup even where stream-K would balance better
+    int blocks_per_cu = 0; // workgroups per CU the schedule plans for; 0 = as many as fit
 };
 
 GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, int elem_bytes);
@@ -33,5 +34,11 @@ GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, in
 // Strided (optionally batched) GEMM; `force` overrides the tile heuristic (autotuner).
 template <typename T>
 int gemm_strided(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream);
+
+// Workspace bytes (from the start of the per-stream workspace) that gemm_strided(P) with
+// the default configuration takes; callers that keep their own data in the same workspace
+// place it after this many bytes.
+template <typename T>
+int gemm_workspace_bytes(GemmProblem<T> const &P, size_t *bytes);
 
 } // namespace ein

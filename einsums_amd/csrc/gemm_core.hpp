@@ -123,7 +123,7 @@ __host__ __device__ __forceinline__ void tile_coords(uint32_t logical, uint32_t 
 //   range base + [w*U, min(W, (w+1)*U)) with base = dp_tiles*slabs. Whole tiles inside it
 //   are finished in place; the (at most two) tiles cut by its range ends are written as
 //   partial accumulators to piece slots 2w (first tile it touches) and 2w+1 (last tile).
-// A fix-up pass over the stream-K tiles (FIXUP_SPLIT workgroups per tile) sums the pieces
+// A fix-up pass over the stream-K tiles (sk_fixup_grid workgroups per tile) sums the pieces
 // of every cut tile in workgroup order, so results are deterministic and no atomics are used.
 struct StreamK {
     uint32_t tiles;    // tiles in the launch (batch included)
@@ -182,10 +182,12 @@ __device__ __forceinline__ void sk_range(StreamK const &sk, uint64_t &u0, uint64
     sk_range_of(sk, blockIdx.x, u0, u1, w);
 }
 
-// Piece slot of tile t for stream-K workgroup w.
+// Piece slot of tile t for stream-K workgroup w, where w's unit range meets t: slot 2w when
+// the range starts inside t (t is w's first tile), 2w+1 otherwise. No division, so the
+// fix-up can evaluate it per piece.
 __host__ __device__ __forceinline__ uint64_t sk_slot(StreamK const &sk, uint64_t w, uint64_t t) {
-    uint64_t const first_tile = (uint64_t(sk.dp_tiles) * sk.slabs + w * sk.units) / sk.slabs;
-    return 2 * w + (t == first_tile ? 0 : 1);
+    uint64_t const start = uint64_t(sk.dp_tiles) * sk.slabs + w * sk.units;
+    return 2 * w + (start >= t * sk.slabs ? 0 : 1);
 }
 
 // Stream-K workgroups [g0, g1] whose unit ranges meet tile t (t >= dp_tiles); the tile was
@@ -196,7 +198,13 @@ __host__ __device__ __forceinline__ void sk_owners(StreamK const &sk, uint64_t t
     g1                 = (off + sk.slabs - 1) / sk.units;
 }
 
-constexpr int FIXUP_SPLIT = 8;
+// Fix-up launch shape: each thread sums FIXUP_EPT elements of one cut tile, so a BM x BN
+// tile takes BM*BN / (256*FIXUP_EPT) workgroups (grid x) and the grid's y runs over the
+// stream-K tiles; every thread issues FIXUP_EPT independent loads per piece.
+constexpr int FIXUP_EPT = 4;
+inline dim3 sk_fixup_grid(StreamK const &sk, int tile_elems) {
+    return dim3(uint32_t(tile_elems / (256 * FIXUP_EPT)), sk.tiles - sk.dp_tiles, 1);
+}
 
 // Number of piece slots a launch needs.
 inline uint64_t sk_piece_slots(StreamK const &sk) { return sk.units ? 2ull * (sk.grid - sk.dp_tiles) : 0; }

@@ -22,7 +22,9 @@ struct Workspace {
     void *ptr    = nullptr;
     size_t bytes = 0;
 };
-std::map<hipStream_t, Workspace> g_ws;
+// slot 0 serves the kernels of one call; slot 1 holds a call's packed operands while a
+// nested call uses slot 0
+std::map<std::pair<hipStream_t, int>, Workspace> g_ws;
 
 // Size-class pool: freed blocks are kept for reuse (a stream-ordered arena would need
 // per-stream bookkeeping; callers of ein_malloc own their tensors for long periods, so
@@ -54,9 +56,19 @@ int fail(int code, std::string const &msg) {
     return code;
 }
 
-int workspace(hipStream_t stream, size_t bytes, void **out) {
+int workspace(hipStream_t stream, size_t bytes, void **out) { return workspace_slot(stream, 0, bytes, out); }
+
+bool stream_capturing(hipStream_t stream) { return stream_is_capturing(stream); }
+
+bool workspace_fits(hipStream_t stream, int slot, size_t bytes) {
     std::lock_guard<std::mutex> lock(g_ws_mutex);
-    Workspace &w = g_ws[stream];
+    auto it = g_ws.find({stream, slot});
+    return it != g_ws.end() && it->second.bytes >= bytes;
+}
+
+int workspace_slot(hipStream_t stream, int slot, size_t bytes, void **out) {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    Workspace &w = g_ws[{stream, slot}];
     if (w.bytes >= bytes) {
         *out = w.ptr;
         return EIN_SUCCESS;
@@ -178,7 +190,7 @@ int ein_workspace_reserve(ein_stream_t stream, size_t bytes) {
 
 size_t ein_workspace_size(ein_stream_t stream) {
     std::lock_guard<std::mutex> lock(ein::g_ws_mutex);
-    auto it = ein::g_ws.find(ein::as_stream(stream));
+    auto it = ein::g_ws.find({ein::as_stream(stream), 0});
     return it == ein::g_ws.end() ? 0 : it->second.bytes;
 }
 

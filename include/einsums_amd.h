@@ -162,6 +162,17 @@ int ein_contract_f64(double c_prefactor, const ein_tensor_desc *C, double *c, do
 int ein_contract_f32(float c_prefactor, const ein_tensor_desc *C, float *c, float ab_prefactor, const ein_tensor_desc *A,
                      const float *a, const ein_tensor_desc *B, const float *b, ein_stream_t stream);
 
+/* Route of the generic contraction's GEMM-sized cases on the calling thread (it holds
+ * until changed): EIN_ROUTE_AUTO picks by a cost model, EIN_ROUTE_GATHER takes the MFMA
+ * strided-gather kernel, EIN_ROUTE_TTGT packs the operands that do not fold into a matrix
+ * with the permute and runs the dense MFMA GEMM (transpose-transpose-GEMM). The reference
+ * has one generic route only (GPUTensorAlgebra.hpp:200-342); this is for autotuners and
+ * tests. Returns the previous route, or -1 for an unknown value (route unchanged). */
+#define EIN_ROUTE_AUTO 0
+#define EIN_ROUTE_GATHER 1
+#define EIN_ROUTE_TTGT 2
+int32_t ein_set_contraction_route(int32_t route);
+
 /* Adjoint of C = alpha * contract(A, B): the gradient G of the operand `wrt` (0 = A,
  * 1 = B) given the cotangent dC, G = beta*G + alpha * contract(dC, other operand), with G
  * labelled like the operand. Labels repeated in the operand make G a diagonal (written

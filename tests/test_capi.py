@@ -108,3 +108,18 @@ def test_rank_mismatch_in_python_layer(core):
 
     with pytest.raises(core.RankError):
         core._desc(torch.zeros(2, 2), "ijk")
+
+
+def test_contraction_route_setter_without_gpu(core):
+    from einsums_amd import _lib
+
+    lib = _lib.lib
+    assert lib.ein_set_contraction_route(_lib.ROUTE_TTGT) == _lib.ROUTE_AUTO
+    assert lib.ein_set_contraction_route(7) == -1            # unknown: rejected, route kept
+    assert lib.ein_set_contraction_route(_lib.ROUTE_AUTO) == _lib.ROUTE_TTGT
+    with core.contraction_route("gather"):
+        assert lib.ein_set_contraction_route(_lib.ROUTE_GATHER) == _lib.ROUTE_GATHER
+    assert lib.ein_set_contraction_route(_lib.ROUTE_AUTO) == _lib.ROUTE_AUTO
+    with pytest.raises(ValueError):
+        with core.contraction_route("fast"):
+            pass
