@@ -205,15 +205,15 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
     for (int64_t sl = 0; sl < nslabs; ++sl) {
         int const buf = int(sl & 1);
         dma_wait_all();
-        auto next_dma = [&]() {
+        auto next_dma = [&](T *An, T *Bn) {
             if (sl + 1 < nslabs) {
                 set_slab(kbeg + (sl + 1) * BK);
-                dma_slab<T, BM, BK, AKG, AW>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
-                dma_slab<T, BN, BK, BKG, BW>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
+                dma_slab<T, BM, BK, AKG, AW>(Ab, An, wave, lane, kbeg + (sl + 1) * BK, offA);
+                dma_slab<T, BN, BK, BKG, BW>(Bb, Bn, wave, lane, kbeg + (sl + 1) * BK, offB);
             }
         };
-        mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, acc, wm, wn, lane,
-                                                       next_dma);
+        mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, As + (buf ^ 1) * LA::SIZE,
+                                                       Bs + (buf ^ 1) * LB::SIZE, acc, wm, wn, lane, next_dma);
     }
 
     // lane coordinates made opaque so the store addresses are not hoisted out of the

@@ -27,7 +27,10 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
 
     int const tid  = threadIdx.x;
     int const lane = tid & 63;
-    int const wave = tid >> 6;
+    // wave-uniform, so per-instruction contraction offsets of the DMA stay scalar (not for
+    // the dword row-major staging, where it makes the compiler keep more lane offsets live)
+    constexpr bool dword_krow = (AKG && !AW) || (BKG && !BW);
+    int const wave = dword_krow ? tid >> 6 : __builtin_amdgcn_readfirstlane(tid >> 6);
     int const wm   = wave & 1;
     int const wn   = wave >> 1;
     int const r    = lane & 15;
@@ -73,14 +76,15 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
         for (int64_t sl = 0; sl < nslabs; ++sl) {
             int const buf = int(sl & 1);
             dma_wait_all();
-            auto next_dma = [&]() {
+            auto next_dma = [&](T *An, T *Bn) {
                 if (sl + 1 < nslabs) {
-                    dma_slab<T, BM, BK, AKG, AW>(Ab, As + (buf ^ 1) * LA::SIZE, wave, lane, kbeg + (sl + 1) * BK, offA);
-                    dma_slab<T, BN, BK, BKG, BW>(Bb, Bs + (buf ^ 1) * LB::SIZE, wave, lane, kbeg + (sl + 1) * BK, offB);
+                    dma_slab<T, BM, BK, AKG, AW>(Ab, An, wave, lane, kbeg + (sl + 1) * BK, offA);
+                    dma_slab<T, BN, BK, BKG, BW>(Bb, Bn, wave, lane, kbeg + (sl + 1) * BK, offB);
                 }
             };
-            mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, acc, wm, wn, lane,
-                                                           next_dma);
+            mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE,
+                                                           As + (buf ^ 1) * LA::SIZE, Bs + (buf ^ 1) * LB::SIZE, acc, wm,
+                                                           wn, lane, next_dma);
         }
 
         // Lane coordinates made opaque per segment so the compiler does not hoist the

@@ -419,6 +419,39 @@ __device__ __forceinline__ void i8_dma(int8_t const *__restrict__ X, int64_t row
     }
 }
 
+// MFMA work of one contraction stage from the LDS stage `cur` (A slots, then B slots);
+// `dma(nxt)` fills the other stage buffer after the B fragments are requested. cur and nxt
+// are restrict parameters so that the compiler does not wait for that DMA before the A
+// fragment reads of this stage.
+template <int NS, typename Dma>
+__device__ __forceinline__ void i8_stage(int8_t const *__restrict__ cur, int8_t *__restrict__ nxt, v4i (&acc)[NS][4][2],
+                                         int wm, int wn, int r, int q, Dma const &dma) {
+    constexpr int FM = 4, FN = 2;
+    using SA = I8Stage<128, NS>;
+    using SB = I8Stage<64, NS>;
+    v4i const *As = reinterpret_cast<v4i const *>(cur);
+    v4i const *Bs = As + SA::SLOTS;
+    v4i bfr[NS][FN];
+#pragma unroll
+    for (int u = 0; u < NS; ++u)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) bfr[u][b] = Bs[SB::slot(u, wn * 32 + b * 16 + r, q)];
+    dma(nxt);
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        v4i af[FM];
+#pragma unroll
+        for (int a = 0; a < FM; ++a) af[a] = As[SA::slot(t, wm * 64 + a * 16 + r, q)];
+#pragma unroll
+        for (int u = 0; u < NS - t; ++u)
+#pragma unroll
+            for (int a = 0; a < FM; ++a)
+#pragma unroll
+                for (int b = 0; b < FN; ++b)
+                    acc[t + u][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bfr[u][b], af[a], acc[t + u][a][b], 0, 0, 0);
+    }
+}
+
 // C(i,j) = alpha * 2^(e_i+f_j) * sum_d 2^(-7(d+2)) D_d(i,j) + beta * C(i,j); workgroup tile
 // 128 x 64, waves of 64 x 32 (4 x 2 blocks of 16 x 16 x 64).
 // Split-K: workgroup (tile, split) accumulates contraction steps [split*kchunk, ...) with
@@ -468,30 +501,13 @@ __global__ __launch_bounds__(256, 1) void ozaki_i8_kernel(OzProblem P) {
     for (int64_t st = 0; st < nst; ++st) {
         int const buf = int(st & 1);
         dma_wait_all();
-        if (st + 1 < nst) {
-            i8_dma<BM, NS>(Ag, P.m, P.kp, i0, kb + (st + 1) * I8_KS, stage_ptr(buf ^ 1), wave, lane);
-            i8_dma<BN, NS>(Bg, P.n, P.kp, j0, kb + (st + 1) * I8_KS, stage_ptr(buf ^ 1) + SA::SLOTS * 16, wave, lane);
-        }
-        v4i const *As = reinterpret_cast<v4i const *>(stage_ptr(buf));
-        v4i const *Bs = As + SA::SLOTS;
-        v4i bfr[NS][FN];
-#pragma unroll
-        for (int u = 0; u < NS; ++u)
-#pragma unroll
-            for (int b = 0; b < FN; ++b) bfr[u][b] = Bs[SB::slot(u, wn * 32 + b * 16 + r, q)];
-#pragma unroll
-        for (int t = 0; t < NS; ++t) {
-            v4i af[FM];
-#pragma unroll
-            for (int a = 0; a < FM; ++a) af[a] = As[SA::slot(t, wm * 64 + a * 16 + r, q)];
-#pragma unroll
-            for (int u = 0; u < NS - t; ++u)
-#pragma unroll
-                for (int a = 0; a < FM; ++a)
-#pragma unroll
-                    for (int b = 0; b < FN; ++b)
-                        acc[t + u][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bfr[u][b], af[a], acc[t + u][a][b], 0, 0, 0);
-        }
+        auto dma = [&](int8_t *nx) {
+            if (st + 1 < nst) {
+                i8_dma<BM, NS>(Ag, P.m, P.kp, i0, kb + (st + 1) * I8_KS, nx, wave, lane);
+                i8_dma<BN, NS>(Bg, P.n, P.kp, j0, kb + (st + 1) * I8_KS, nx + SA::SLOTS * 16, wave, lane);
+            }
+        };
+        i8_stage<NS>(stage_ptr(buf), stage_ptr(buf ^ 1), acc, wm, wn, r, q, dma);
     }
 
     // weighted sum of the diagonals of block (a, b), element v

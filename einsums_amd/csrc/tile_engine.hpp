@@ -32,7 +32,7 @@ namespace ein {
 #define EIN_LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
 
 // Zero dwords read by lanes whose element is outside the problem (never written).
-static __device__ uint32_t ein_zero_page[4];
+static __device__ __attribute__((aligned(16))) uint32_t ein_zero_page[4];
 
 template <typename T, int ROWS, int BK, bool KROW, bool WIDE>
 struct SlabLayout {
@@ -113,14 +113,17 @@ __device__ __forceinline__ void dma_slab(T const *__restrict__ base, T *lds, int
 }
 
 // MFMA work of one slab from LDS, with the fragments of step kk+4 read while the MFMAs of
-// step kk run. As/Bs are the slab buffers of A (rows along i) and B (rows along j).
-// `between()` runs after the first fragments are requested and before the first MFMA:
-// the caller issues the next slab's DMA there, so the fragment reads are not queued
-// behind it (LDS DMA and ds_read stay in program order).
-template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW, typename Between>
-__device__ __forceinline__ void mfma_slab_lds(T const *As, T const *Bs,
+// step kk run. As/Bs are the slab buffers of A (rows along i) and B (rows along j), An/Bn
+// the buffers the next slab's DMA fills: `dma(An, Bn)` runs after the first fragments are
+// requested and before the first MFMA, so the fragment reads are not queued behind it.
+// All four are restrict parameters of this (inlined) function: the compiler then knows
+// that the fragment reads cannot alias the in-flight DMA and does not wait for it before
+// every LDS read of the slab.
+template <typename T, int BM, int BN, int BK, bool AKG, bool AW, bool BKG, bool BW, typename Dma>
+__device__ __forceinline__ void mfma_slab_lds(T const *__restrict__ As, T const *__restrict__ Bs, T *__restrict__ An,
+                                              T *__restrict__ Bn,
                                               WaveAcc<T, TileCfg<T, BM, BN, BK>::FM, TileCfg<T, BM, BN, BK>::FN> &acc,
-                                              int wm, int wn, int lane, Between const &between) {
+                                              int wm, int wn, int lane, Dma const &dma) {
     using Cfg = TileCfg<T, BM, BN, BK>;
     using LA  = SlabLayout<T, BM, BK, AKG, AW>;
     using LB  = SlabLayout<T, BN, BK, BKG, BW>;
@@ -134,7 +137,7 @@ __device__ __forceinline__ void mfma_slab_lds(T const *As, T const *Bs,
         for (int b = 0; b < Cfg::FN; ++b) bf[slot][b] = Bs[LB::pos(wn * Cfg::WN + b * 16 + r, kk + q)];
     };
     frag(0, 0);
-    between();
+    dma(An, Bn);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
         int const cur = (kk / 4) & 1;

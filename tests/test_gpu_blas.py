@@ -87,6 +87,32 @@ def test_gemm_vs_oracle(oracle, dtype, ta, tb, m, n, k):
         assert rel_fro(got2[:, :m], ref2[:, :m]) < tol
 
 
+# Leading dimensions equal to the extents (no padding), so both operands take the 16-byte
+# DMA staging. Edge tiles (m, n not multiples of 128), k not a multiple of 16, a stream-K
+# tail (576 x 576 x 2304) and odd m (dword staging for A) are covered.
+CONTIG_SHAPES = [(128, 128, 16), (256, 384, 100), (130, 260, 77), (576, 576, 2304), (1000, 1000, 1000),
+                 (2304, 256, 48), (129, 200, 64)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("m,n,k", CONTIG_SHAPES)
+def test_gemm_contiguous_vs_oracle(oracle, dtype, tb, m, n, k):
+    L = _lib()
+    rng = np.random.default_rng(m * 13 + n * 7 + k)
+    ldb = k if tb == "N" else n
+    A = rng.uniform(-1, 1, m * k).astype(dtype)
+    B = rng.uniform(-1, 1, n * k).astype(dtype)
+    C0 = rng.uniform(-1, 1, m * n).astype(dtype)
+    fn = L.lib.ein_gemm_f64 if dtype == np.float64 else L.lib.ein_gemm_f32
+    for alpha, beta in ((1.0, 0.0), (0.25, -1.5)):
+        ref = C0.astype(np.float64).copy()
+        oracle.gemm_colmajor("N", tb, m, n, k, alpha, A.astype(np.float64), m, B.astype(np.float64), ldb, beta, ref, m)
+        dC = dev(C0)
+        L.check(fn(b"N", tb.encode(), m, n, k, alpha, vd(A), m, vd(B), ldb, beta, vp(dC), m, None))
+        assert rel_fro(host(dC), ref) < (1e-13 if dtype == np.float64 else 1e-5)
+
+
 def test_gemm_beta_zero_clears_nan():
     L = _lib()
     rng = np.random.default_rng(5)
