@@ -40,6 +40,8 @@ def parse():
                    help="only run the batched uneven-block benchmarks (one JSON line) and exit")
     p.add_argument("--precision-sweep", action="store_true",
                    help="only run the split-precision vs FP64 GEMM sweep (one JSON line per case) and exit")
+    p.add_argument("--sweep-cases", default="dgemm,wmnij,wabef,wmbej",
+                   help="cases of the precision sweep: any of dgemm, wmnij, wabef, wmbej (comma separated)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES bytes (counter calibration) and exit")
     return p.parse_args()
@@ -128,7 +130,7 @@ def split_comparison(core, plans, W, reps=10):
     return out
 
 
-def precision_sweep(sizes=(1024, 2048, 4096, 8192, 16384), reps=3):
+def precision_sweep(sizes=(1024, 2048, 4096, 8192, 16384), reps=3, cases=("dgemm", "wmnij", "wabef", "wmbej")):
     """Split-precision vs plain FP64 GEMM: DGEMM n x n x n with U(-1,1) and stressed
     U(-1,1)*2^e (e uniform in [-20, 20]) data, and the Wabef GEMM at o=48, v=256
     (65536 x 65536 x 2304, 34 GB output). Errors: relative Frobenius against the FP64
@@ -186,13 +188,64 @@ def precision_sweep(sizes=(1024, 2048, 4096, 8192, 16384), reps=3):
         return x * torch.exp2(e)
 
     out = []
-    for n in sizes:
-        out.append(case(f"dgemm uniform n={n}", n, n, n, uniform))
-        out.append(case(f"dgemm stressed n={n}", n, n, n, stressed))
+    if "dgemm" in cases:
+        for n in sizes:
+            out.append(case(f"dgemm uniform n={n}", n, n, n, uniform))
+            out.append(case(f"dgemm stressed n={n}", n, n, n, stressed))
     # o=48, v=256: Wmnij (o^2 x o^2, k = v^2) and Wabef (v^2 x v^2, k = o^2) as GEMMs
-    out.append(case("Wmnij o=48 v=256", 2304, 2304, 65536, lambda r, c, g: uniform(r, c, g) * 0.1))
-    out.append(case("Wabef o=48 v=256", 65536, 65536, 2304, lambda r, c, g: uniform(r, c, g) * 0.1))
+    if "wmnij" in cases:
+        out.append(case("Wmnij o=48 v=256", 2304, 2304, 65536, lambda r, c, g: uniform(r, c, g) * 0.1))
+    if "wabef" in cases:
+        out.append(case("Wabef o=48 v=256", 65536, 65536, 2304, lambda r, c, g: uniform(r, c, g) * 0.1))
+    if "wmbej" in cases:
+        out.append(wmbej_case(timed))
     return out
+
+
+def wmbej_case(timed, o=48, v=256):
+    """Wmbej = t_jnfb g_mnef at o=48, v=256 (a generic contraction: 12288^3 after packing the
+    operands into GEMM layout) through the einsum plan, plain FP64 vs FP32 operands vs the
+    split-precision modes. Errors: relative Frobenius against the FP64 result, and against a
+    host sum over the links for 64 sampled entries."""
+    import numpy as np
+    import torch
+
+    import einsums_amd.core as core
+
+    g = torch.Generator(device="cuda").manual_seed(4321)
+    t = (torch.rand((o, o, v, v), dtype=torch.float64, device="cuda", generator=g) * 2 - 1) * 0.1
+    gg = (torch.rand((o, o, v, v), dtype=torch.float64, device="cuda", generator=g) * 2 - 1) * 0.1
+    plan = core.compile_plan("mbej", "jnfb", "mnef")
+    W64 = torch.empty((o, v, v, o), dtype=torch.float64, device="cuda")
+    t64 = timed(lambda: plan.execute(0.0, W64, 1.0, t, gg))
+    flop = 2.0 * o ** 3 * v ** 3
+    rng = np.random.default_rng(5)
+    idx = [tuple(int(rng.integers(0, x)) for x in (o, v, v, o)) for _ in range(64)]
+    th, gh = t.cpu().numpy(), gg.cpu().numpy()
+    ref = np.array([np.sum(th[j, :, :, b] * gh[m, :, e, :]) for (m, b, e, j) in idx])
+
+    def sample_err(W):
+        got = np.array([float(W[m, b, e, j]) for (m, b, e, j) in idx])
+        return float(f"{np.linalg.norm(got - ref) / np.linalg.norm(ref):.3e}")
+
+    row = {"case": f"Wmbej o={o} v={v}", "m": o * v, "n": o * v, "k": o * v, "fp64_ms": round(t64, 3),
+           "fp64_tflops": round(flop / t64 / 1e9, 2), "fp64_rel_err_rows_vs_host": sample_err(W64.cpu())}
+    t32, g32 = t.float(), gg.float()
+    W32 = torch.empty((o, v, v, o), dtype=torch.float32, device="cuda")
+    ms32 = timed(lambda: plan.execute(0.0, W32, 1.0, t32, g32))
+    row["fp32"] = {"ms": round(ms32, 3), "speedup": round(t64 / ms32, 3),
+                   "rel_err_vs_fp64": float(f"{float(torch.linalg.norm(W32.double() - W64) / torch.linalg.norm(W64)):.3e}")}
+    del t32, g32, W32
+    Ws = torch.empty_like(W64)
+    for st, ns in SPLIT_MODES:
+        ts = timed(lambda: plan.execute(0.0, Ws, 1.0, t, gg, split=ns, slice_type=st))
+        err = float(torch.linalg.norm(Ws - W64) / torch.linalg.norm(W64))
+        row[f"{st}x{ns}"] = {"ms": round(ts, 3), "speedup": round(t64 / ts, 3), "rel_err_vs_fp64": float(f"{err:.3e}"),
+                             "rel_err_rows_vs_host": sample_err(Ws.cpu())}
+    del t, gg, W64, Ws
+    torch.cuda.empty_cache()
+    print(json.dumps(row), flush=True)
+    return row
 
 
 def grouped_bench(reps=5):
@@ -294,7 +347,7 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.precision_sweep:
-        precision_sweep()
+        precision_sweep(cases=tuple(args.sweep_cases.split(",")))
         return
     if args.grouped:
         print(json.dumps({"grouped": grouped_bench()}))
