@@ -182,34 +182,43 @@ __device__ __forceinline__ void gather_segment(GatherProblem<T> const &P, uint64
             kbB = P.kB[k0];
         }
     };
-    auto offA = [&](int row, int64_t p, bool &valid) -> int64_t {
-        int64_t const i = i0 + row;
+    // with `check` false (interior tile and slab) the table reads need no guard
+    auto offA = [&](int row, int64_t k0, int k, bool check, bool &valid) -> int64_t {
+        int64_t const i = i0 + row, p = k0 + k;
         valid           = i < P.m && p < kend;
-        if (!valid) return 0;
+        if (check && !valid) return 0;
         return P.rowA[i] + (P.k_affine ? kbA + (p - kslab) * P.kA_in : P.kA[p]);
     };
-    auto offB = [&](int row, int64_t p, bool &valid) -> int64_t {
-        int64_t const j = j0 + row;
+    auto offB = [&](int row, int64_t k0, int k, bool check, bool &valid) -> int64_t {
+        int64_t const j = j0 + row, p = k0 + k;
         valid           = j < P.n && p < kend;
-        if (!valid) return 0;
+        if (check && !valid) return 0;
         return P.colB[j] + (P.k_affine ? kbB + (p - kslab) * P.kB_in : P.kB[p]);
     };
-
+    constexpr bool dword_krow = (AKG && !AW) || (BKG && !BW);
+    bool const full_mn        = !dword_krow && i0 + BM <= P.m && j0 + BN <= P.n;
+    auto dma_both      = [&](T *Ad, T *Bd, int64_t k0) {
+        if (full_mn && k0 + BK <= kend) {
+            dma_slab<T, BM, BK, AKG, AW, false>(Ab, Ad, wave, lane, k0, offA);
+            dma_slab<T, BN, BK, BKG, BW, false>(Bb, Bd, wave, lane, k0, offB);
+        } else {
+            dma_slab<T, BM, BK, AKG, AW>(Ab, Ad, wave, lane, k0, offA);
+            dma_slab<T, BN, BK, BKG, BW>(Bb, Bd, wave, lane, k0, offB);
+        }
+    };
     WaveAcc<T, Cfg::FM, Cfg::FN> acc;
     acc.zero();
     int64_t const nslabs = int64_t(s1 - s0);
     __syncthreads(); // LDS of the previous segment no longer read
     set_slab(kbeg);
-    dma_slab<T, BM, BK, AKG, AW>(Ab, As, wave, lane, kbeg, offA);
-    dma_slab<T, BN, BK, BKG, BW>(Bb, Bs, wave, lane, kbeg, offB);
+    dma_both(As, Bs, kbeg);
     for (int64_t sl = 0; sl < nslabs; ++sl) {
         int const buf = int(sl & 1);
         dma_wait_all();
         auto next_dma = [&](T *An, T *Bn) {
             if (sl + 1 < nslabs) {
                 set_slab(kbeg + (sl + 1) * BK);
-                dma_slab<T, BM, BK, AKG, AW>(Ab, An, wave, lane, kbeg + (sl + 1) * BK, offA);
-                dma_slab<T, BN, BK, BKG, BW>(Bb, Bn, wave, lane, kbeg + (sl + 1) * BK, offB);
+                dma_both(An, Bn, kbeg + (sl + 1) * BK);
             }
         };
         mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE, As + (buf ^ 1) * LA::SIZE,

@@ -57,13 +57,25 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
         T const *__restrict__ Ab = P.A + batch * P.a_sb + i0 * P.a_sm;
         T const *__restrict__ Bb = P.B + batch * P.b_sb + j0 * P.b_sn;
         int64_t const mrem = P.m - i0, nrem = P.n - j0;
-        auto offA = [&](int row, int64_t p, bool &valid) -> int64_t {
-            valid = row < mrem && p < kend;
-            return row * P.a_sm + p * P.a_sk;
+        auto offA = [&](int row, int64_t k0, int k, bool, bool &valid) -> int64_t {
+            valid = row < mrem && k0 + k < kend;
+            return row * P.a_sm + k0 * P.a_sk + int64_t(k) * P.a_sk;
         };
-        auto offB = [&](int row, int64_t p, bool &valid) -> int64_t {
-            valid = row < nrem && p < kend;
-            return row * P.b_sn + p * P.b_sk;
+        auto offB = [&](int row, int64_t k0, int k, bool, bool &valid) -> int64_t {
+            valid = row < nrem && k0 + k < kend;
+            return row * P.b_sn + k0 * P.b_sk + int64_t(k) * P.b_sk;
+        };
+        // both operands' slabs at k0; without bounds checks when tile and slab are inside
+        // (not for the dword row-major staging, whose registers a second copy would spill)
+        bool const full_mn = !dword_krow && mrem >= BM && nrem >= BN;
+        auto dma_both      = [&](T *Ad, T *Bd, int64_t k0) {
+            if (full_mn && k0 + BK <= kend) {
+                dma_slab<T, BM, BK, AKG, AW, false>(Ab, Ad, wave, lane, k0, offA);
+                dma_slab<T, BN, BK, BKG, BW, false>(Bb, Bd, wave, lane, k0, offB);
+            } else {
+                dma_slab<T, BM, BK, AKG, AW>(Ab, Ad, wave, lane, k0, offA);
+                dma_slab<T, BN, BK, BKG, BW>(Bb, Bd, wave, lane, k0, offB);
+            }
         };
 
         WaveAcc<T, Cfg::FM, Cfg::FN> acc;
@@ -71,16 +83,12 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void gemm_dense_kernel(GemmProblem
         int64_t const nslabs = int64_t(s1 - s0);
         int64_t const kbeg   = int64_t(s0) * BK;
         __syncthreads(); // LDS of the previous segment no longer read
-        dma_slab<T, BM, BK, AKG, AW>(Ab, As, wave, lane, kbeg, offA);
-        dma_slab<T, BN, BK, BKG, BW>(Bb, Bs, wave, lane, kbeg, offB);
+        dma_both(As, Bs, kbeg);
         for (int64_t sl = 0; sl < nslabs; ++sl) {
             int const buf = int(sl & 1);
             dma_wait_all();
             auto next_dma = [&](T *An, T *Bn) {
-                if (sl + 1 < nslabs) {
-                    dma_slab<T, BM, BK, AKG, AW>(Ab, An, wave, lane, kbeg + (sl + 1) * BK, offA);
-                    dma_slab<T, BN, BK, BKG, BW>(Bb, Bn, wave, lane, kbeg + (sl + 1) * BK, offB);
-                }
+                if (sl + 1 < nslabs) dma_both(An, Bn, kbeg + (sl + 1) * BK);
             };
             mfma_slab_lds<T, BM, BN, BK, AKG, AW, BKG, BW>(As + buf * LA::SIZE, Bs + buf * LB::SIZE,
                                                            As + (buf ^ 1) * LA::SIZE, Bs + (buf ^ 1) * LB::SIZE, acc, wm,
@@ -285,12 +293,11 @@ GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, in
     }
     // 128 x 128 tiles do four times the MFMA work of 64 x 64 per slab barrier. Measured on
     // MI355X (FP64): with fewer than a quarter round of them (576 x 576 x 9216) 128 x 64
-    // tiles balance better and halve the stream-K pieces; with many rounds of short-k tiles
-    // (9216 x 9216 x 576, beta = 1) one workgroup per CU is ~5% faster than two.
-    int64_t const cus = num_cus();
+    // tiles balance better and halve the stream-K pieces.
+    (void)k;
+    (void)elem_bytes;
     int64_t const t128 = ceil_div(m, 128) * ceil_div(n, 128) * batch;
-    if (t128 < cus / 4) cfg.bn = 64;
-    else if (elem_bytes == 8 && t128 >= 8 * cus && k <= 2048) cfg.blocks_per_cu = 1;
+    if (t128 < num_cus() / 4) cfg.bn = 64;
     return cfg;
 }
 

@@ -87,11 +87,15 @@ struct SlabLayout {
     }
 };
 
-// Issues the DMA of the slab starting at contraction step k0. `off(row, p, valid)`
-// returns the element offset of (tile row, contraction step p) from `base` and whether
-// that element is inside the problem. For a 16-byte chunk, (row, p) is its first element
-// and the host guarantees the chunk is entirely inside or entirely outside.
-template <typename T, int ROWS, int BK, bool KROW, bool WIDE, typename OffsetFn>
+// Issues the DMA of the slab starting at contraction step k0. `off(row, k0, k, check, valid)`
+// returns the element offset of (tile row, contraction step k0 + k) from `base` and sets
+// `valid` to whether that element is inside the problem. With CHECK false the caller knows
+// the whole slab is inside (an interior tile and slab), the checks drop out and `check`
+// tells the functor it may skip its own guards. For a 16-byte chunk, (row, k) is its first
+// element and the host guarantees the chunk is entirely inside or entirely outside.
+// k0 enters separately from k so that k0 * stride is formed once per slab (scalar) and
+// k * stride once per kernel.
+template <typename T, int ROWS, int BK, bool KROW, bool WIDE, bool CHECK = true, typename OffsetFn>
 __device__ __forceinline__ void dma_slab(T const *__restrict__ base, T *lds, int wave, int lane, int64_t k0,
                                          OffsetFn const &off) {
     using L = SlabLayout<T, ROWS, BK, KROW, WIDE>;
@@ -104,9 +108,10 @@ __device__ __forceinline__ void dma_slab(T const *__restrict__ base, T *lds, int
         // a KROW lane visits a different row per instruction: keep the row offsets from
         // being hoisted out of the slab loop (they would pin 2*INS_PER_WAVE registers)
         if constexpr (KROW) asm volatile("" : "+v"(row));
-        bool valid;
-        int64_t const e = off(row, k0 + k, valid);
-        void const *src = valid ? static_cast<void const *>(bdw + e * L::EW + h) : static_cast<void const *>(ein_zero_page);
+        bool valid      = true;
+        int64_t const e = off(row, k0, k, CHECK, valid);
+        void const *src = (!CHECK || valid) ? static_cast<void const *>(bdw + e * L::EW + h)
+                                            : static_cast<void const *>(ein_zero_page);
         uint32_t *dst   = reinterpret_cast<uint32_t *>(lds) + L::ins_base_dw(wave, x);
         __builtin_amdgcn_global_load_lds(src, EIN_LDS_PTR(dst), 4 * L::VW, 0, 0);
     }
