@@ -42,6 +42,8 @@ def parse():
                    help="only run the split-precision vs FP64 GEMM sweep (one JSON line per case) and exit")
     p.add_argument("--sweep-cases", default="dgemm,wmnij,wabef,wmbej",
                    help="cases of the precision sweep: any of dgemm, wmnij, wabef, wmbej (comma separated)")
+    p.add_argument("--bandwidth", action="store_true",
+                   help="only run the HBM-bound kernels (HF J/K, dot, permute; cold and warm cache) and exit")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES bytes (counter calibration) and exit")
     return p.parse_args()
@@ -202,6 +204,87 @@ def precision_sweep(sizes=(1024, 2048, 4096, 8192, 16384), reps=3, cases=("dgemm
     return out
 
 
+def bandwidth_bench(reps=20):
+    """HBM-bound kernels against the 8 TB/s peak: the HF J (GEMV) and K (generic) builds at
+    nbf = 48 and 96, the CCD energy dot and the ijab->iajb permute at o=24, v=96 and
+    o=48, v=256, and a plain copy. Each is timed warm (back-to-back calls, operands may sit
+    in the 256 MB infinity cache) and cold (a 512 MB scratch write between calls, timed
+    separately); bytes are the algorithmic ones: every operand element read once, the
+    output written once (read too when beta != 0)."""
+    import numpy as np
+    import torch
+
+    import einsums_amd.core as core
+
+    scratch = torch.empty(64 << 20, dtype=torch.float64, device="cuda")  # 512 MB
+
+    def measure(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        warm = e0.elapsed_time(e1) / reps
+        cold = []
+        for _ in range(reps):
+            scratch.fill_(1.0)
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            cold.append(a.elapsed_time(b))
+        return warm, float(np.median(cold))
+
+    out = []
+
+    def report(name, nbytes, fn, alg=None):
+        warm, cold = measure(fn)
+        row = {"case": name, "bytes": int(nbytes), "cold_us": round(cold * 1e3, 2), "cold_TBs": round(nbytes / cold / 1e9, 3),
+               "cold_frac": round(nbytes / cold / 1e9 / 8.0, 3), "warm_us": round(warm * 1e3, 2),
+               "warm_TBs": round(nbytes / warm / 1e9, 3)}
+        if alg is not None:
+            row["algorithm"] = alg
+        print(json.dumps(row), flush=True)
+        out.append(row)
+
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    rng = np.random.default_rng(3)
+    for nbf in (48, 96):
+        g = dev(rng.standard_normal((nbf,) * 4))
+        D = dev(rng.standard_normal((nbf, nbf)))
+        J = torch.empty((nbf, nbf), dtype=torch.float64, device="cuda")
+        K = torch.empty_like(J)
+        pj = core.compile_plan("pq", "pqrs", "rs")
+        pk = core.compile_plan("pq", "prqs", "rs")
+        nb = 8 * (nbf ** 4 + 2 * nbf ** 2)
+        report(f"HF J nbf={nbf} (pq <- pqrs, rs)", nb, lambda: pj.execute(0.0, J, 1.0, g, D))
+        out[-1]["algorithm"] = core.algorithm_name(pj.last_algorithm)
+        report(f"HF K nbf={nbf} (pq <- prqs, rs)", nb, lambda: pk.execute(0.0, K, 1.0, g, D))
+        out[-1]["algorithm"] = core.algorithm_name(pk.last_algorithm)
+        del g, D, J, K
+    for (o, v) in ((24, 96), (48, 256)):
+        t = torch.rand((o, o, v, v), dtype=torch.float64, device="cuda")
+        gg = torch.rand((o, o, v, v), dtype=torch.float64, device="cuda")
+        E = torch.zeros((), dtype=torch.float64, device="cuda")
+        P = torch.empty((o, v, o, v), dtype=torch.float64, device="cuda")
+        pe = core.compile_plan("", "ijab", "ijab")
+        n = o * o * v * v
+        report(f"E dot o={o} v={v}", 16 * n, lambda: pe.execute(0.0, E, 0.25, t, gg), "DOT")
+        report(f"permute ijab->iajb o={o} v={v}", 16 * n, lambda: core.permute(0.0, "iajb", P, 1.0, "ijab", t), "PERMUTE")
+        if o == 48:
+            report(f"copy {8 * n} B", 16 * n, lambda: core.permute(0.0, "ijab", P.view(o, o, v, v), 1.0, "ijab", t),
+                   "PERMUTE (identity)")
+        del t, gg, E, P
+        torch.cuda.empty_cache()
+    return out
+
+
 def wmbej_case(timed, o=48, v=256):
     """Wmbej = t_jnfb g_mnef at o=48, v=256 (a generic contraction: 12288^3 after packing the
     operands into GEMM layout) through the einsum plan, plain FP64 vs FP32 operands vs the
@@ -348,6 +431,9 @@ def main():
         return
     if args.precision_sweep:
         precision_sweep(cases=tuple(args.sweep_cases.split(",")))
+        return
+    if args.bandwidth:
+        bandwidth_bench()
         return
     if args.grouped:
         print(json.dumps({"grouped": grouped_bench()}))

@@ -10,6 +10,7 @@
 //   blas::gpu::dirprod -> ddirprod_kernel (hipBLASVendor/src/direct_prod.hip:20-271)
 //   blas::gpu::scal / axpby               (hipBLASVendor/src/scal.hip:25-68)
 #include <algorithm>
+#include <type_traits>
 
 #include "blas1.hpp"
 
@@ -57,7 +58,10 @@ __global__ __launch_bounds__(256) void gemv_n_finish(int64_t m, int splits, T al
     y[i * incy] = v;
 }
 
-// y[j] (+)= alpha * sum_i A[i + j*lda] x[i]; one wave per column, lanes along i.
+// y[j] (+)= alpha * sum_i A[i + j*lda] x[i]; one wave per column, lanes along i. With
+// unit incx, lda a multiple of the vector width and 16-byte aligned A and x, 16-byte
+// loads, four of each operand in flight per lane (3.0 -> 3.9 TB/s on a 680 MB matrix).
+// Taking several columns per wave to share the x loads measured slower: too few waves.
 template <typename T>
 __global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, T alpha, T const *__restrict__ A, int64_t lda,
                                                      T const *__restrict__ x, int64_t incx, T beta, T *__restrict__ y,
@@ -66,14 +70,41 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, T alp
     int64_t const j = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     if (j >= n) return;
     T const *col = A + j * lda;
-    T acc0 = T(0), acc1 = T(0);
+    T acc[4]     = {T(0), T(0), T(0), T(0)};
+    constexpr int VW = 16 / int(sizeof(T));
+    bool const vec   = incx == 1 && lda % VW == 0 &&
+                     (reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(x)) % 16 == 0;
     int64_t i = lane;
-    for (; i + 64 < m; i += 128) {
-        acc0 += col[i] * x[i * incx];
-        acc1 += col[i + 64] * x[(i + 64) * incx];
+    if (vec) {
+        using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+        V const *cv = reinterpret_cast<V const *>(col);
+        V const *xv = reinterpret_cast<V const *>(x);
+        int64_t const mv = m / VW;
+        auto dotv = [](V const &a, V const &b) -> T {
+            if constexpr (sizeof(T) == 8) return a.x * b.x + a.y * b.y;
+            else return (a.x * b.x + a.y * b.y) + (a.z * b.z + a.w * b.w);
+        };
+        for (; i + 192 < mv; i += 256) {
+            V a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = cv[i + 64 * u];
+                b[u] = xv[i + 64 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] += dotv(a[u], b[u]);
+        }
+        for (; i < mv; i += 64) acc[0] += dotv(cv[i], xv[i]);
+        int64_t const t = mv * VW + lane;
+        if (t < m) acc[1] += col[t] * x[t];
+    } else {
+        for (; i + 64 < m; i += 128) {
+            acc[0] += col[i] * x[i * incx];
+            acc[1] += col[i + 64] * x[(i + 64) * incx];
+        }
+        if (i < m) acc[0] += col[i] * x[i * incx];
     }
-    if (i < m) acc0 += col[i] * x[i * incx];
-    T s = wave_reduce_sum(acc0 + acc1);
+    T s = wave_reduce_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
     if (lane == 0) {
         T v = alpha * s;
         if (beta != T(0)) v += beta * y[j * incy];
@@ -126,40 +157,79 @@ int gemv_impl(char trans, int64_t m, int64_t n, T alpha, T const *A, int64_t lda
 }
 
 // -------------------------------------------------------------------- dot
+// Unit strides: workgroup b sums the contiguous elements [b*DOT_CHUNK, (b+1)*DOT_CHUNK), so
+// consecutive workgroups stream consecutive memory (6.3 TB/s measured on 2.4 GB against
+// 5.3 TB/s for a grid-stride loop), with 16-byte loads when both operands are aligned.
+// Other strides: a grid-stride loop. Partials are summed in a fixed order (deterministic).
+constexpr int DOT_CHUNK = 256 * 32; // elements per workgroup: 64 KB of each FP64 operand
+
 template <typename T>
 __global__ __launch_bounds__(256) void dot_partial_kernel(int64_t n, T const *__restrict__ x, int64_t incx,
                                                           T const *__restrict__ y, int64_t incy, T *__restrict__ partial) {
     __shared__ T red[4];
-    T acc0 = T(0), acc1 = T(0);
-    int64_t const stride = int64_t(gridDim.x) * blockDim.x;
-    int64_t e            = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    T acc[4] = {T(0), T(0), T(0), T(0)};
     if (incx == 1 && incy == 1) {
-        for (; e + stride < n; e += 2 * stride) {
-            acc0 += x[e] * y[e];
-            acc1 += x[e + stride] * y[e + stride];
+        int64_t const b0 = int64_t(blockIdx.x) * DOT_CHUNK;
+        int64_t const b1 = b0 + DOT_CHUNK < n ? b0 + DOT_CHUNK : n;
+        bool const vec   = (reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) % 16 == 0;
+        if (vec && b1 - b0 == DOT_CHUNK) {
+            constexpr int VW = 16 / int(sizeof(T));
+            using V          = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+            V const *xv = reinterpret_cast<V const *>(x + b0);
+            V const *yv = reinterpret_cast<V const *>(y + b0);
+            constexpr int NV = DOT_CHUNK / VW / 256; // vectors per thread
+#pragma unroll
+            for (int g = 0; g < NV; g += 8) {
+                V a[8], c[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    a[u] = xv[(g + u) * 256 + threadIdx.x];
+                    c[u] = yv[(g + u) * 256 + threadIdx.x];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    T d;
+                    if constexpr (sizeof(T) == 8) d = a[u].x * c[u].x + a[u].y * c[u].y;
+                    else d = (a[u].x * c[u].x + a[u].y * c[u].y) + (a[u].z * c[u].z + a[u].w * c[u].w);
+                    acc[u & 3] += d;
+                }
+            }
+        } else {
+            for (int64_t e = b0 + threadIdx.x; e < b1; e += 256) acc[0] += x[e] * y[e];
         }
-        if (e < n) acc0 += x[e] * y[e];
     } else {
-        for (; e < n; e += stride) acc0 += x[e * incx] * y[e * incy];
+        int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+        for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) acc[0] += x[e * incx] * y[e * incy];
     }
-    T s = wave_reduce_sum(acc0 + acc1);
+    T s = wave_reduce_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// Sums the partials in a fixed order: 1024 threads, eight independent loads in flight per
+// thread (tens of thousands of partials for a 2.4 GB dot).
 template <typename T>
-__global__ __launch_bounds__(256) void reduce_finish_kernel(int parts, T const *__restrict__ partial, T alpha, T beta,
-                                                            T *__restrict__ result) {
-    __shared__ T red[4];
-    T acc = T(0);
-    for (int k = threadIdx.x; k < parts; k += 256) acc += partial[k];
-    T s = wave_reduce_sum(acc);
+__global__ __launch_bounds__(1024) void reduce_finish_kernel(int parts, T const *__restrict__ partial, T alpha, T beta,
+                                                             T *__restrict__ result) {
+    __shared__ T red[16];
+    T acc[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+    int k = threadIdx.x;
+    for (; k + 7 * 1024 < parts; k += 8 * 1024) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = partial[k + u * 1024];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] += v[u];
+    }
+    for (; k < parts; k += 1024) acc[0] += partial[k];
+    T s = wave_reduce_sum(((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7])));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-        T tot = (red[0] + red[1]) + (red[2] + red[3]);
-        T v   = alpha * tot;
+        T tot = T(0);
+        for (int w = 0; w < 16; ++w) tot += red[w];
+        T v = alpha * tot;
         if (beta != T(0)) v += beta * result[0];
         result[0] = v;
     }
@@ -172,13 +242,16 @@ int dot_impl(int64_t n, T alpha, T const *x, int64_t incx, T const *y, int64_t i
     if (result == nullptr) return fail(-8, "dot: null result");
     if (n > 0 && incx < 0) x = x - (n - 1) * incx;
     if (n > 0 && incy < 0) y = y - (n - 1) * incy;
-    int parts = int(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256 * 8), int64_t(num_cus()) * 4)));
+    int64_t const chunks = ceil_div(n, DOT_CHUNK);
+    int parts = int(incx == 1 && incy == 1 ? std::max<int64_t>(1, chunks)
+                                           : std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256 * 8), int64_t(num_cus()) * 4)));
+    if (incx == 1 && incy == 1 && chunks > (int64_t{1} << 30)) return fail(EIN_ERROR_UNSUPPORTED, "dot: vector too long");
     void *ws  = nullptr;
     EIN_TRY(workspace(stream, size_t(parts) * sizeof(T), &ws));
     T *partial = static_cast<T *>(ws);
     dot_partial_kernel<T><<<parts, 256, 0, stream>>>(n, x, incx, y, incy, partial);
     EIN_LAUNCH_CHECK();
-    reduce_finish_kernel<T><<<1, 256, 0, stream>>>(parts, partial, alpha, beta, result);
+    reduce_finish_kernel<T><<<1, 1024, 0, stream>>>(parts, partial, alpha, beta, result);
     EIN_LAUNCH_CHECK();
     return EIN_SUCCESS;
 }

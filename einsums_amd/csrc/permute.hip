@@ -242,7 +242,10 @@ int permute_impl(int32_t rank, int64_t const *dims_A, int32_t const *perm, T alp
                           return true;
                       }();
     int64_t work   = vec2 ? total / 2 : total;
-    uint32_t grid  = uint32_t(std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, 256), int64_t(nblk_cap) * 4)));
+    // one element (vector) per thread: consecutive workgroups stream consecutive memory,
+    // which measured 6.2 TB/s on a 2.4 GB copy against 4.5-4.8 TB/s for a grid-stride loop
+    // over 2k-16k workgroups (the loop remains for grids beyond 2^31 - 1 workgroups)
+    uint32_t grid  = uint32_t(std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, 256), int64_t(0x7fffffff))));
     if (vec2)
         permute_elem_kernel<T, 2, false><<<grid, 256, 0, stream>>>(P, alpha, A, beta, C, copy);
     else if (wide)
