@@ -277,6 +277,12 @@ def bandwidth_bench(reps=20):
         n = o * o * v * v
         report(f"E dot o={o} v={v}", 16 * n, lambda: pe.execute(0.0, E, 0.25, t, gg), "DOT")
         report(f"permute ijab->iajb o={o} v={v}", 16 * n, lambda: core.permute(0.0, "iajb", P, 1.0, "ijab", t), "PERMUTE")
+        # unit-stride axes differ (b in A, f in C): the LDS-tiled transpose
+        P2 = P.view(o, v, o, v)
+        report(f"permute jnfb->jbnf o={o} v={v} (tiled transpose)", 16 * n,
+               lambda: core.permute(0.0, "jbnf", P2, 1.0, "jnfb", t), "PERMUTE")
+        report(f"permute jnfb->jbnf o={o} v={v}, beta=1", 24 * n,
+               lambda: core.permute(1.0, "jbnf", P2, 1.0, "jnfb", t), "PERMUTE")
         if o == 48:
             report(f"copy {8 * n} B", 16 * n, lambda: core.permute(0.0, "ijab", P.view(o, o, v, v), 1.0, "ijab", t),
                    "PERMUTE (identity)")

@@ -9,6 +9,7 @@
 //     A read runs along A's unit-stride axis and the C write along C's;
 // alpha == 1 and beta == 0 is a pure copy, so the result is bit-exact.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "blas1.hpp"
@@ -187,7 +188,6 @@ int permute_impl(int32_t rank, int64_t const *dims_A, int32_t const *perm, T alp
         if (std::llabs(m[i].sc) < std::llabs(m[ic].sc)) ic = i;
         if (std::llabs(m[i].sa) < std::llabs(m[ia].sa)) ia = i;
     }
-    int const nblk_cap = num_cus() * 8;
 
     if (ic != ia && m[ic].sc == 1 && m[ia].sa == 1 && m[ic].e >= 16 && m[ia].e >= 16) {
         PermAxes P{};
@@ -205,14 +205,15 @@ int permute_impl(int32_t rank, int64_t const *dims_A, int32_t const *perm, T alp
         int64_t ex = m[ic].e, axs = m[ic].sa, ey = m[ia].e, cy = m[ia].sc;
         int64_t tiles_x = ceil_div(ex, 64), tiles = tiles_x * ceil_div(ey, 64);
         int64_t gy      = std::min<int64_t>(P.total, 65535);
-        int64_t gx      = std::min<int64_t>(tiles, std::max<int64_t>(1, nblk_cap / gy + 1));
+        // one tile per workgroup (tiles of one batch entry used to share a workgroup and ran
+        // one after another)
+        int64_t gx = std::min<int64_t>(tiles, 65535);
         bool const wide = P.total >= (int64_t{1} << 31);
+        dim3 const grid{uint32_t(gx), uint32_t(gy)};
         if (wide)
-            permute_tile_kernel<T, true><<<dim3(uint32_t(gx), uint32_t(gy)), 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x,
-                                                                                               alpha, A, beta, C, copy);
+            permute_tile_kernel<T, true><<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy);
         else
-            permute_tile_kernel<T, false><<<dim3(uint32_t(gx), uint32_t(gy)), 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x,
-                                                                                                alpha, A, beta, C, copy);
+            permute_tile_kernel<T, false><<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy);
         EIN_LAUNCH_CHECK();
         return EIN_SUCCESS;
     }

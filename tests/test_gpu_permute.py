@@ -25,6 +25,7 @@ def labels(n):
 CASES = [
     ((24, 24, 96, 96), "ikjl"),   # CCD ijab -> iajb: innermost axis kept (row copy, 2-wide vectors)
     ((24, 24, 96, 96), "klij"),   # ijab -> abij: innermost axis changes (LDS tile transpose)
+    ((3, 130, 66), "ikj"),        # tile transpose with partial edge tiles on both axes
     ((5, 7, 9), "kji"),
     ((64, 65), "ji"),
     ((1, 130, 1, 67), "lkji"),
