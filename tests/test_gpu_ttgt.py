@@ -80,3 +80,23 @@ def test_auto_route_at_ccd_size_matches_gather(core):
         m, b, e, j = (int(rng.integers(0, x)) for x in (o, v, v, o))
         want = W0[m, b, e, j] - 0.5 * np.sum(t[j, :, :, b] * g[m, :, e, :])
         assert abs(out["auto"][m, b, e, j] - want) <= 1e-13 * max(1.0, abs(want))
+
+
+@pytest.mark.parametrize("slice_type,ns,tol", [("int8", 6, 1e-11), ("bf16", 5, 1e-11)])
+def test_ttgt_split_precision_vs_oracle(core, oracle, slice_type, ns, tol):
+    """In split mode the packed route runs its GEMM on the split-precision kernel."""
+    Ci, Ai, Bi = "mbej", "jnfb", "mnef"
+    ext = dict(m=20, b=40, e=40, j=20, n=20, f=40)
+    rng = seeded(Ci, Ai, Bi, "split")
+    A = rng.uniform(-1, 1, [ext[c] for c in Ai])
+    B = rng.uniform(-1, 1, [ext[c] for c in Bi])
+    C0 = rng.uniform(-1, 1, [ext[c] for c in Ci])
+    ref = C0.copy()
+    oracle.einsum_generic(0.5, Ci, ref, -0.5, Ai, A, Bi, B)
+    dC = dev(C0)
+    with core.contraction_route("ttgt"):
+        core.compile_plan(Ci, Ai, Bi).execute(0.5, dC, -0.5, dev(A), dev(B), split=ns, slice_type=slice_type)
+    err = rel_fro(host(dC), ref)
+    assert err < tol
+    # the split kernel ran: slicing leaves ~1e-13, a plain FP64 GEMM would match to ~1e-16
+    assert err > 1e-15
