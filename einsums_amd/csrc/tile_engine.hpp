@@ -151,6 +151,9 @@ __device__ __forceinline__ void mfma_slab_lds(T const *__restrict__ As, T const 
         for (int a = 0; a < Cfg::FM; ++a)
 #pragma unroll
             for (int b = 0; b < Cfg::FN; ++b) acc.c[a][b] = Mfma<T>::mma(bf[cur][b], af[cur][a], acc.c[a][b]);
+        // LLVM's MFMA/LDS interleaving strategy for this region (measured +1% on FP64
+        // 4096^3..16384^2; the wave-priority alternative measured -2%)
+        __builtin_amdgcn_iglp_opt(0);
     }
 }
 
