@@ -160,7 +160,7 @@ def test_gemv(trans, m, n, incx, incy):
     assert rel_fro(host(dy), ref) < 1e-13
 
 
-@pytest.mark.parametrize("n", [1, 100, 65536, 5308416])
+@pytest.mark.parametrize("n", [1, 100, 65536, 5308416, 16778453])
 @pytest.mark.parametrize("inc", [1, 3])
 def test_dot(n, inc):
     L = _lib()
