@@ -335,6 +335,13 @@ int gemm_strided_impl(GemmProblem<T> P, GemmConfig const *force, hipStream_t str
         return EIN_SUCCESS;
     }
     GemmConfig cfg = force ? *force : choose_gemm_config(P.m, P.n, P.k, P.batch, int(sizeof(T)));
+    if (!force && cfg.bm == 128 && cfg.bn == 64) {
+        // few tiles with both operands staged row-major (contiguous along k): 64 x 64
+        // measured 4% faster than 128 x 64 on 576 x 576 x 9216
+        bool const a_krow = choose_staging<T>(128, P.a_sm, P.a_sk, P.a_sb, P.batch, P.m, P.k, P.A).krow;
+        bool const b_krow = choose_staging<T>(64, P.b_sn, P.b_sk, P.b_sb, P.batch, P.n, P.k, P.B).krow;
+        if (a_krow && b_krow) cfg.bm = 64;
+    }
     P.force_dp     = cfg.data_parallel;
     constexpr int BK = 16;
     if (cfg.bm == 128 && cfg.bn == 128) return launch_dense<T, 128, 128, BK>(P, cfg.blocks_per_cu, stream, ws_bytes);
