@@ -445,7 +445,8 @@ def main():
         print(json.dumps({"grouped": grouped_bench()}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of this build",
+        print(json.dumps({"error": "this benchmark runs on one GPU only: the contraction path has no multi-GPU "
+                                   "sharding (the output-tile sharding it would use is described, not implemented)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
