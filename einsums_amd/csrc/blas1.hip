@@ -58,58 +58,81 @@ __global__ __launch_bounds__(256) void gemv_n_finish(int64_t m, int splits, T al
     y[i * incy] = v;
 }
 
-// y[j] (+)= alpha * sum_i A[i + j*lda] x[i]; one wave per column, lanes along i. With
-// unit incx, lda a multiple of the vector width and 16-byte aligned A and x, 16-byte
-// loads, four of each operand in flight per lane (3.0 -> 3.9 TB/s on a 680 MB matrix).
-// Taking several columns per wave to share the x loads measured slower: too few waves.
+// y[j] (+)= alpha * sum_i A[i + j*lda] x[i]. A block is four waves. Column j is cut into
+// splits*wpc contiguous segments, one per wave: the wpc waves of a block that share a column
+// take consecutive segments and add their sums in LDS in fixed order, and with splits > 1
+// each block row writes a partial [split][j] that gemv_n_finish adds in fixed order. Cutting
+// columns keeps many short waves in flight, so the grid has no long low-occupancy tail (one
+// wave per column on a 9216 x 9216 matrix is 1.1 rounds of resident waves). With unit incx,
+// lda a multiple of the vector width and 16-byte aligned A and x: 16-byte loads, four of
+// each operand in flight per lane.
 template <typename T>
 __global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, T alpha, T const *__restrict__ A, int64_t lda,
                                                      T const *__restrict__ x, int64_t incx, T beta, T *__restrict__ y,
-                                                     int64_t incy) {
-    int const lane = threadIdx.x & 63;
-    int64_t const j = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    if (j >= n) return;
-    T const *col = A + j * lda;
-    T acc[4]     = {T(0), T(0), T(0), T(0)};
+                                                     int64_t incy, int wpc, int splits, T *__restrict__ partial) {
+    __shared__ T red[4];
+    int const lane   = threadIdx.x & 63;
+    int const wave   = threadIdx.x >> 6;
+    int64_t const j  = int64_t(blockIdx.x) * (4 / wpc) + wave / wpc;
+    int const seg    = int(blockIdx.y) * wpc + wave % wpc;
+    int const nseg   = splits * wpc;
+    T acc[4]         = {T(0), T(0), T(0), T(0)};
     constexpr int VW = 16 / int(sizeof(T));
     bool const vec   = incx == 1 && lda % VW == 0 &&
                      (reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(x)) % 16 == 0;
-    int64_t i = lane;
-    if (vec) {
-        using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
-        V const *cv = reinterpret_cast<V const *>(col);
-        V const *xv = reinterpret_cast<V const *>(x);
-        int64_t const mv = m / VW;
-        auto dotv = [](V const &a, V const &b) -> T {
-            if constexpr (sizeof(T) == 8) return a.x * b.x + a.y * b.y;
-            else return (a.x * b.x + a.y * b.y) + (a.z * b.z + a.w * b.w);
-        };
-        for (; i + 192 < mv; i += 256) {
-            V a[4], b[4];
+    if (j < n) {
+        T const *col = A + j * lda;
+        if (vec) {
+            using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+            V const *cv = reinterpret_cast<V const *>(col);
+            V const *xv = reinterpret_cast<V const *>(x);
+            int64_t const mv    = m / VW;
+            int64_t const chunk = (mv + nseg - 1) / nseg;
+            int64_t const v1    = std::min<int64_t>(mv, (seg + 1) * chunk);
+            auto dotv           = [](V const &a, V const &b) -> T {
+                if constexpr (sizeof(T) == 8) return a.x * b.x + a.y * b.y;
+                else return (a.x * b.x + a.y * b.y) + (a.z * b.z + a.w * b.w);
+            };
+            int64_t i = seg * chunk + lane;
+            for (; i + 192 < v1; i += 256) {
+                V a[4], b[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                a[u] = cv[i + 64 * u];
-                b[u] = xv[i + 64 * u];
+                for (int u = 0; u < 4; ++u) {
+                    a[u] = cv[i + 64 * u];
+                    b[u] = xv[i + 64 * u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc[u] += dotv(a[u], b[u]);
             }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc[u] += dotv(a[u], b[u]);
+            for (; i < v1; i += 64) acc[0] += dotv(cv[i], xv[i]);
+            int64_t const t = mv * VW + lane;
+            if (seg == nseg - 1 && t < m) acc[1] += col[t] * x[t];
+        } else {
+            int64_t const chunk = (m + nseg - 1) / nseg;
+            int64_t const e1    = std::min<int64_t>(m, (seg + 1) * chunk);
+            int64_t i           = seg * chunk + lane;
+            for (; i + 64 < e1; i += 128) {
+                acc[0] += col[i] * x[i * incx];
+                acc[1] += col[i + 64] * x[(i + 64) * incx];
+            }
+            if (i < e1) acc[0] += col[i] * x[i * incx];
         }
-        for (; i < mv; i += 64) acc[0] += dotv(cv[i], xv[i]);
-        int64_t const t = mv * VW + lane;
-        if (t < m) acc[1] += col[t] * x[t];
-    } else {
-        for (; i + 64 < m; i += 128) {
-            acc[0] += col[i] * x[i * incx];
-            acc[1] += col[i + 64] * x[(i + 64) * incx];
-        }
-        if (i < m) acc[0] += col[i] * x[i * incx];
     }
     T s = wave_reduce_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
-    if (lane == 0) {
-        T v = alpha * s;
-        if (beta != T(0)) v += beta * y[j * incy];
-        y[j * incy] = v;
+    if (wpc > 1) {
+        if (lane == 0) red[wave] = s;
+        __syncthreads();
+        if (wave % wpc != 0) return;
+        for (int u = 1; u < wpc; ++u) s += red[wave + u];
     }
+    if (j >= n || lane != 0) return;
+    if (splits > 1) {
+        partial[int64_t(blockIdx.y) * n + j] = s;
+        return;
+    }
+    T v = alpha * s;
+    if (beta != T(0)) v += beta * y[j * incy];
+    y[j * incy] = v;
 }
 
 template <typename T>
@@ -150,8 +173,27 @@ int gemv_impl(char trans, int64_t m, int64_t n, T alpha, T const *A, int64_t lda
             EIN_LAUNCH_CHECK();
         }
     } else {
-        gemv_t_kernel<T><<<uint32_t(ceil_div(n, 4)), 256, 0, stream>>>(m, n, alpha, A, lda, x, incx, beta, y, incy);
+        // waves per column (1, 2, 4) until about four rounds of resident waves, keeping
+        // >= 256 elements per wave; then split across block rows when even that is short
+        int64_t const resident = int64_t(num_cus()) * 32;
+        int wpc                = 1;
+        while (wpc < 4 && n * wpc < 4 * resident && m / (2 * wpc) >= 256) wpc *= 2;
+        int splits = 1;
+        if (wpc == 4 && n * 4 < resident)
+            splits = int(std::clamp<int64_t>(std::min<int64_t>(ceil_div(4 * resident, n * 4), m / 4096), 1, 1024));
+        T *partial = nullptr;
+        if (splits > 1) {
+            void *ws = nullptr;
+            EIN_TRY(workspace(stream, size_t(splits) * size_t(n) * sizeof(T), &ws));
+            partial = static_cast<T *>(ws);
+        }
+        dim3 grid{uint32_t(ceil_div(n, 4 / wpc)), uint32_t(splits)};
+        gemv_t_kernel<T><<<grid, 256, 0, stream>>>(m, n, alpha, A, lda, x, incx, beta, y, incy, wpc, splits, partial);
         EIN_LAUNCH_CHECK();
+        if (splits > 1) {
+            gemv_n_finish<T><<<uint32_t(ceil_div(n, 256)), 256, 0, stream>>>(n, splits, alpha, partial, beta, y, incy);
+            EIN_LAUNCH_CHECK();
+        }
     }
     return EIN_SUCCESS;
 }

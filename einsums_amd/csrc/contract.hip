@@ -340,7 +340,8 @@ struct ReduceProblem {
     T const *B;
     T *C;
     T alpha, beta;
-    int32_t splits;
+    int32_t splits; // K pieces per target: 1, or 4 * parts
+    int32_t parts;  // partials per target written for the finish pass
     int64_t k_chunk;
     T *partial;
 };
@@ -379,14 +380,20 @@ __global__ __launch_bounds__(256) void reduce_thread_kernel(ReduceProblem<T> P) 
     }
 }
 
-// One wave per (target, K chunk); lanes stride the chunk, wave-level tree reduction.
-// With splits == 1 the wave writes C, otherwise a partial that the finish pass sums in
-// fixed order.
+// Waves over (target, K piece). splits == 1: one wave per target, lanes stride its K range.
+// splits = 4*parts: the four waves of a block take consecutive contiguous pieces of one
+// target's K range and add their sums in LDS in fixed order; with parts > 1 each block
+// writes a partial that the finish pass sums in fixed order. Cutting K this way keeps many
+// short waves in flight, so the grid has no long low-occupancy tail.
 template <typename T>
 __global__ __launch_bounds__(256) void reduce_wave_kernel(ReduceProblem<T> P) {
-    int const lane        = threadIdx.x & 63;
-    int64_t const w       = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    int64_t const nwork   = P.nt * P.splits;
+    __shared__ T red[4];
+    int const lane      = threadIdx.x & 63;
+    int const wave      = threadIdx.x >> 6;
+    int64_t const w     = int64_t(blockIdx.x) * 4 + wave;
+    int64_t const nwork = P.nt * P.splits;
+    // with splits a multiple of 4 the grid holds exactly nwork waves, so only splits == 1
+    // (which uses no barrier) returns here
     if (w >= nwork) return;
     int64_t const t  = w / P.splits;
     int64_t const sp = w - t * P.splits;
@@ -394,31 +401,43 @@ __global__ __launch_bounds__(256) void reduce_wave_kernel(ReduceProblem<T> P) {
     group_decode<3>(uint32_t(t), P.gt, o);
     int64_t const kb = sp * P.k_chunk;
     int64_t const ke = kb + P.k_chunk < P.nk ? kb + P.k_chunk : P.nk;
-    T acc0 = T(0), acc1 = T(0);
-    int64_t p = kb + lane;
-    for (; p + 64 < ke; p += 128) {
-        int64_t k0[2], k1[2];
-        group_decode<2>(uint32_t(p), P.gk, k0);
-        group_decode<2>(uint32_t(p + 64), P.gk, k1);
-        acc0 += P.A[o[0] + k0[0]] * P.B[o[1] + k0[1]];
-        acc1 += P.A[o[0] + k1[0]] * P.B[o[1] + k1[1]];
+    T acc[4]         = {T(0), T(0), T(0), T(0)};
+    int64_t p        = kb + lane;
+    for (; p + 192 < ke; p += 256) {
+        int64_t k[4][2];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) group_decode<2>(uint32_t(p + 64 * u), P.gk, k[u]);
+        T a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = P.A[o[0] + k[u][0]];
+            b[u] = P.B[o[1] + k[u][1]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] += a[u] * b[u];
     }
-    if (p < ke) {
+    for (; p < ke; p += 64) {
         int64_t k0[2];
         group_decode<2>(uint32_t(p), P.gk, k0);
-        acc0 += P.A[o[0] + k0[0]] * P.B[o[1] + k0[1]];
+        acc[0] += P.A[o[0] + k0[0]] * P.B[o[1] + k0[1]];
     }
-    T s = wave_reduce_sum(acc0 + acc1);
-    if (lane == 0) {
-        if (P.splits > 1) {
-            P.partial[t * P.splits + sp] = s;
-        } else {
-            T *cp = P.C + o[2];
-            T val = P.alpha * s;
-            if (P.beta != T(0)) val += P.beta * *cp;
-            *cp = val;
+    T s = wave_reduce_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+    if (P.splits > 1) {
+        if (lane == 0) red[wave] = s;
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        s = (red[0] + red[1]) + (red[2] + red[3]);
+        if (P.parts > 1) {
+            P.partial[t * P.parts + sp / 4] = s;
+            return;
         }
+    } else if (lane != 0) {
+        return;
     }
+    T *cp = P.C + o[2];
+    T val = P.alpha * s;
+    if (P.beta != T(0)) val += P.beta * *cp;
+    *cp = val;
 }
 
 template <typename T>
@@ -426,7 +445,7 @@ __global__ __launch_bounds__(256) void reduce_finish_targets_kernel(ReduceProble
     int64_t const stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < P.nt; t += stride) {
         T s = T(0);
-        for (int sp = 0; sp < P.splits; ++sp) s += P.partial[t * P.splits + sp];
+        for (int sp = 0; sp < P.parts; ++sp) s += P.partial[t * P.parts + sp];
         int64_t o[3];
         group_decode<3>(uint32_t(t), P.gt, o);
         T *cp = P.C + o[2];
@@ -954,22 +973,26 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         EIN_LAUNCH_CHECK();
         return EIN_SUCCESS;
     }
-    int64_t want_waves = int64_t(cus) * 16;
-    if (NT < want_waves && K >= 2048) {
-        int64_t s = std::min<int64_t>(ceil_div(want_waves, NT), ceil_div(K, 1024));
-        s         = std::max<int64_t>(1, std::min<int64_t>(s, 4096));
-        R.k_chunk = ceil_div(K, s);
-        R.splits  = int(ceil_div(K, R.k_chunk));
+    R.parts = 1;
+    // too few targets for about four rounds of resident waves: cut each target's K range
+    // into 4 * parts pieces of >= 256 links
+    int64_t const resident = int64_t(cus) * 32;
+    if (NT < 4 * resident && K >= 1024) {
+        int64_t s = std::min<int64_t>(ceil_div(4 * resident, NT), K / 256) / 4;
+        s         = std::max<int64_t>(1, std::min<int64_t>(s, 1024));
+        R.splits  = int(4 * s);
+        R.parts   = int(s);
+        R.k_chunk = ceil_div(K, int64_t(R.splits));
     }
-    if (R.splits > 1) {
+    if (R.parts > 1) {
         void *ws = nullptr;
-        EIN_TRY(workspace(stream, size_t(NT) * size_t(R.splits) * sizeof(T), &ws));
+        EIN_TRY(workspace(stream, size_t(NT) * size_t(R.parts) * sizeof(T), &ws));
         R.partial = static_cast<T *>(ws);
     }
     int64_t nwaves = NT * R.splits;
     reduce_wave_kernel<T><<<uint32_t(ceil_div(nwaves, 4)), 256, 0, stream>>>(R);
     EIN_LAUNCH_CHECK();
-    if (R.splits > 1) {
+    if (R.parts > 1) {
         uint32_t grid = uint32_t(std::max<int64_t>(1, std::min<int64_t>(ceil_div(NT, 256), int64_t(cus) * 16)));
         reduce_finish_targets_kernel<T><<<grid, 256, 0, stream>>>(R);
         EIN_LAUNCH_CHECK();
