@@ -208,15 +208,16 @@ def bandwidth_bench(reps=20):
     """HBM-bound kernels against the 8 TB/s peak: the HF J (GEMV) and K (generic) builds at
     nbf = 48 and 96, the CCD energy dot and the ijab->iajb permute at o=24, v=96 and
     o=48, v=256, and a plain copy. Each is timed warm (back-to-back calls, operands may sit
-    in the 256 MB infinity cache) and cold (a 512 MB scratch write between calls, timed
-    separately); bytes are the algorithmic ones: every operand element read once, the
+    in the 256 MB infinity cache) and cold (a 512 MB scratch read between calls, timed
+    separately: a scratch write would leave dirty lines whose write-back then competes with
+    the timed call); bytes are the algorithmic ones: every operand element read once, the
     output written once (read too when beta != 0)."""
     import numpy as np
     import torch
 
     import einsums_amd.core as core
 
-    scratch = torch.empty(64 << 20, dtype=torch.float64, device="cuda")  # 512 MB
+    scratch = torch.zeros(64 << 20, dtype=torch.float64, device="cuda")  # 512 MB
 
     def measure(fn):
         fn()
@@ -231,7 +232,7 @@ def bandwidth_bench(reps=20):
         warm = e0.elapsed_time(e1) / reps
         cold = []
         for _ in range(reps):
-            scratch.fill_(1.0)
+            scratch.sum()
             a = torch.cuda.Event(enable_timing=True)
             b = torch.cuda.Event(enable_timing=True)
             a.record()

@@ -6,7 +6,8 @@
 //   * if C's unit-stride axis is also A's unit-stride axis, an element-wise pass in C
 //     order moves contiguous runs (both sides coalesced, 16-byte vectors when aligned);
 //   * otherwise a 64x64 tile is staged through LDS (one padded row per tile row) so the
-//     A read runs along A's unit-stride axis and the C write along C's;
+//     A read runs along A's unit-stride axis and the C write along C's, in 16-byte
+//     vectors on full tiles of aligned operands;
 // alpha == 1 and beta == 0 is a pure copy, so the result is bit-exact.
 #include <algorithm>
 #include <type_traits>
@@ -92,11 +93,21 @@ __global__ __launch_bounds__(256) void permute_elem_kernel(PermAxes P, T alpha, 
 }
 
 // 64x64 LDS tile transpose. X = C's unit-stride axis (A stride ax), Y = A's unit-stride
-// axis (C stride cy). Remaining axes (P, n entries) are decoded from the batch index.
+// axis (C stride cy). Remaining axes (P, n entries) are decoded from the batch index. Every
+// load of a tile is issued before its LDS writes (a load and LDS write under one bounds test
+// per element serialised the loads: 4.3 against 5.2 TB/s on a 2.4 GB transpose). Full
+// tiles of operands whose strides and base addresses allow it (vec) move 16-byte vectors.
 template <typename T, bool WIDE>
 __global__ __launch_bounds__(256) void permute_tile_kernel(PermAxes P, int64_t ex, int64_t ax, int64_t ey, int64_t cy,
                                                            int64_t tiles_x, T alpha, T const *__restrict__ A, T beta,
-                                                           T *__restrict__ C, int copy) {
+                                                           T *__restrict__ C, int copy, int vec) {
+    constexpr int VW  = 16 / int(sizeof(T)); // elements per 16-byte vector
+    constexpr int LPR = 64 / VW;             // lanes per tile row
+    constexpr int RPP = 256 / LPR;           // tile rows per pass of the workgroup
+    constexpr int NP  = 64 / RPP;            // passes per tile
+    struct alignas(16) Vec {
+        T v[VW];
+    };
     __shared__ T tile[64][65];
     int64_t const ntiles = tiles_x * ((ey + 63) / 64);
     int const t          = threadIdx.x;
@@ -106,17 +117,49 @@ __global__ __launch_bounds__(256) void permute_tile_kernel(PermAxes P, int64_t e
         for (int64_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
             int64_t const x0 = (tt % tiles_x) * 64;
             int64_t const y0 = (tt / tiles_x) * 64;
-            {
+            bool const full  = vec != 0 && x0 + 64 <= ex && y0 + 64 <= ey;
+            if (full) {
+                int const c  = t % LPR;
+                int const r0 = t / LPR;
+                T const *src = A + oa + (x0 + r0) * ax + y0 + VW * c;
+                Vec v[NP];
+#pragma unroll
+                for (int p = 0; p < NP; ++p) v[p] = *reinterpret_cast<Vec const *>(src + int64_t(p * RPP) * ax);
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+#pragma unroll
+                    for (int i = 0; i < VW; ++i) tile[r0 + p * RPP][VW * c + i] = v[p].v[i];
+            } else {
                 int const ly = t & 63;
                 int const lx = t >> 6;
+                T v[16];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    int x = lx + 4 * r;
-                    if (x0 + x < ex && y0 + ly < ey) tile[x][ly] = A[oa + (x0 + x) * ax + (y0 + ly)];
+                    int const x = lx + 4 * r;
+                    v[r]        = (x0 + x < ex && y0 + ly < ey) ? A[oa + (x0 + x) * ax + (y0 + ly)] : T(0);
                 }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) tile[lx + 4 * r][ly] = v[r];
             }
             __syncthreads();
-            {
+            if (full) {
+                int const c  = t % LPR;
+                int const r0 = t / LPR;
+                T *dst       = C + oc + (y0 + r0) * cy + x0 + VW * c;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    Vec w;
+#pragma unroll
+                    for (int i = 0; i < VW; ++i) w.v[i] = tile[VW * c + i][r0 + p * RPP];
+                    Vec *cp = reinterpret_cast<Vec *>(dst + int64_t(p * RPP) * cy);
+                    if (!copy) {
+                        Vec const old = beta != T(0) ? *cp : Vec{};
+#pragma unroll
+                        for (int i = 0; i < VW; ++i) w.v[i] = alpha * w.v[i] + (beta != T(0) ? beta * old.v[i] : T(0));
+                    }
+                    *cp = w;
+                }
+            } else {
                 int const lx = t & 63;
                 int const ly = t >> 6;
 #pragma unroll
@@ -210,10 +253,17 @@ int permute_impl(int32_t rank, int64_t const *dims_A, int32_t const *perm, T alp
         int64_t gx = std::min<int64_t>(tiles, 65535);
         bool const wide = P.total >= (int64_t{1} << 31);
         dim3 const grid{uint32_t(gx), uint32_t(gy)};
+        // 16-byte vectors along both unit-stride axes: every batch offset, the row strides
+        // and both base addresses must keep them aligned
+        constexpr int VW = 16 / int(sizeof(T));
+        bool vec = axs % VW == 0 && cy % VW == 0 && (reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(C)) % 16 == 0;
+        for (int d = 0; d < P.n; ++d) vec = vec && P.sa[d] % VW == 0 && P.sc[d] % VW == 0;
         if (wide)
-            permute_tile_kernel<T, true><<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy);
+            permute_tile_kernel<T, true>
+                <<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy, vec ? 1 : 0);
         else
-            permute_tile_kernel<T, false><<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy);
+            permute_tile_kernel<T, false>
+                <<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy, vec ? 1 : 0);
         EIN_LAUNCH_CHECK();
         return EIN_SUCCESS;
     }

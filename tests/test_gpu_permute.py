@@ -26,6 +26,8 @@ CASES = [
     ((24, 24, 96, 96), "ikjl"),   # CCD ijab -> iajb: innermost axis kept (row copy, 2-wide vectors)
     ((24, 24, 96, 96), "klij"),   # ijab -> abij: innermost axis changes (LDS tile transpose)
     ((3, 130, 66), "ikj"),        # tile transpose with partial edge tiles on both axes
+    ((2, 128, 192), "ikj"),       # full tiles only, 16-byte vectors for both widths
+    ((2, 136, 200), "ikj"),       # full (vector) and edge (element) tiles in one launch
     ((5, 7, 9), "kji"),
     ((64, 65), "ji"),
     ((1, 130, 1, 67), "lkji"),
@@ -66,6 +68,19 @@ def test_permute_all_rank4_permutations_roundtrip(core):
         core.permute(0.0, "ijkl", back, 1.0, out, dC)
         assert torch.equal(back, dA)
         assert host(dC).tobytes() == np.ascontiguousarray(A.transpose(p)).tobytes()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-15), (np.float32, 1e-6)])
+def test_permute_tile_alpha_beta(core, oracle, dtype, tol):
+    """C = beta*C + alpha*perm(A) through the tile transpose, vector and edge tiles."""
+    rng = seeded("tile-ab", dtype.__name__)
+    A = rng.standard_normal((2, 136, 200)).astype(dtype)
+    C0 = rng.standard_normal((2, 200, 136)).astype(dtype)
+    dC = dev(C0)
+    core.permute(0.5, "ikj", dC, -2.0, "ijk", dev(A))
+    ref = C0.astype(np.float64)
+    oracle.permute(0.5, "ikj", ref, -2.0, "ijk", A.astype(np.float64))
+    assert rel_fro(host(dC), ref) < tol
 
 
 def test_permute_alpha_beta_and_strided_views(core, oracle):
