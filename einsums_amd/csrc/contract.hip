@@ -384,8 +384,10 @@ __global__ __launch_bounds__(256) void reduce_thread_kernel(ReduceProblem<T> P) 
 // splits = 4*parts: the four waves of a block take consecutive contiguous pieces of one
 // target's K range and add their sums in LDS in fixed order; with parts > 1 each block
 // writes a partial that the finish pass sums in fixed order. Cutting K this way keeps many
-// short waves in flight, so the grid has no long low-occupancy tail.
-template <typename T>
+// short waves in flight, so the grid has no long low-occupancy tail. VEC: the K index
+// counts 16-byte vectors of the innermost link digit, unit-stride in both operands, and
+// every offset is a multiple of the vector width.
+template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void reduce_wave_kernel(ReduceProblem<T> P) {
     __shared__ T red[4];
     int const lane      = threadIdx.x & 63;
@@ -402,24 +404,35 @@ __global__ __launch_bounds__(256) void reduce_wave_kernel(ReduceProblem<T> P) {
     int64_t const kb = sp * P.k_chunk;
     int64_t const ke = kb + P.k_chunk < P.nk ? kb + P.k_chunk : P.nk;
     T acc[4]         = {T(0), T(0), T(0), T(0)};
-    int64_t p        = kb + lane;
+    constexpr int VW = VEC ? 16 / int(sizeof(T)) : 1;
+    struct alignas(VEC ? 16 : sizeof(T)) Vec {
+        T v[VW];
+    };
+    auto ld   = [](T const *ptr) { return *reinterpret_cast<Vec const *>(ptr); };
+    auto dotv = [](Vec const &a, Vec const &b) {
+        T r = a.v[0] * b.v[0];
+#pragma unroll
+        for (int i = 1; i < VW; ++i) r += a.v[i] * b.v[i];
+        return r;
+    };
+    int64_t p = kb + lane;
     for (; p + 192 < ke; p += 256) {
         int64_t k[4][2];
 #pragma unroll
         for (int u = 0; u < 4; ++u) group_decode<2>(uint32_t(p + 64 * u), P.gk, k[u]);
-        T a[4], b[4];
+        Vec a[4], b[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            a[u] = P.A[o[0] + k[u][0]];
-            b[u] = P.B[o[1] + k[u][1]];
+            a[u] = ld(P.A + o[0] + k[u][0]);
+            b[u] = ld(P.B + o[1] + k[u][1]);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] += a[u] * b[u];
+        for (int u = 0; u < 4; ++u) acc[u] += dotv(a[u], b[u]);
     }
     for (; p < ke; p += 64) {
         int64_t k0[2];
         group_decode<2>(uint32_t(p), P.gk, k0);
-        acc[0] += P.A[o[0] + k0[0]] * P.B[o[1] + k0[1]];
+        acc[0] += dotv(ld(P.A + o[0] + k0[0]), ld(P.B + o[1] + k0[1]));
     }
     T s = wave_reduce_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
     if (P.splits > 1) {
@@ -974,6 +987,22 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         return EIN_SUCCESS;
     }
     R.parts = 1;
+    // 16-byte vectors along the innermost link digit when it is unit-stride in both operands
+    // and every operand offset keeps them aligned; the K index then counts vectors
+    constexpr int VW = 16 / int(sizeof(T));
+    bool vec = !gk.empty() && gk.back().s[0] == 1 && gk.back().s[1] == 1 && gk.back().e % VW == 0 &&
+               (reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) % 16 == 0;
+    for (size_t d = 0; vec && d + 1 < gk.size(); ++d) vec = gk[d].s[0] % VW == 0 && gk[d].s[1] % VW == 0;
+    for (auto const &d : gt) vec = vec && d.s[0] % VW == 0 && d.s[1] % VW == 0;
+    if (vec) {
+        auto gkv = gk;
+        gkv.back().e /= VW;
+        gkv.back().s[0] = VW;
+        gkv.back().s[1] = VW;
+        EIN_TRY(fill_group(R.gk, gkv, 2));
+        R.nk      = K / VW;
+        R.k_chunk = R.nk;
+    }
     // too few targets for about four rounds of resident waves: cut each target's K range
     // into 4 * parts pieces of >= 256 links
     int64_t const resident = int64_t(cus) * 32;
@@ -982,7 +1011,7 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         s         = std::max<int64_t>(1, std::min<int64_t>(s, 1024));
         R.splits  = int(4 * s);
         R.parts   = int(s);
-        R.k_chunk = ceil_div(K, int64_t(R.splits));
+        R.k_chunk = ceil_div(R.nk, int64_t(R.splits));
     }
     if (R.parts > 1) {
         void *ws = nullptr;
@@ -990,7 +1019,10 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         R.partial = static_cast<T *>(ws);
     }
     int64_t nwaves = NT * R.splits;
-    reduce_wave_kernel<T><<<uint32_t(ceil_div(nwaves, 4)), 256, 0, stream>>>(R);
+    if (vec)
+        reduce_wave_kernel<T, true><<<uint32_t(ceil_div(nwaves, 4)), 256, 0, stream>>>(R);
+    else
+        reduce_wave_kernel<T, false><<<uint32_t(ceil_div(nwaves, 4)), 256, 0, stream>>>(R);
     EIN_LAUNCH_CHECK();
     if (R.parts > 1) {
         uint32_t grid = uint32_t(std::max<int64_t>(1, std::min<int64_t>(ceil_div(NT, 256), int64_t(cus) * 16)));

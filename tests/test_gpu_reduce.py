@@ -1,7 +1,7 @@
 """GPU parity of the reduction regime of the generic contraction (no MFMA: one operand
 carries only link indices) and of the transposed GEMV at shapes that take the cut-K paths:
 four waves per target adding in LDS, and in addition partials summed by the finish pass.
-FP64 tolerance: relative Frobenius 1e-13."""
+Tolerances (relative Frobenius): FP64 1e-13, FP32 1e-5."""
 import ctypes
 
 import numpy as np
@@ -25,15 +25,18 @@ REDUCE = [
     ("pq", "pqrs", "rs", dict(p=24, q=40, r=45, s=47)),
     ("q", "rqs", "sr", dict(q=7, r=300, s=90)),
     ("pq", "rspq", "sr", dict(p=12, q=9, r=40, s=33)),
+    ("pq", "prqs", "rs", dict(p=40, q=36, r=6, s=20)),   # one wave per target (K < 1024)
 ]
 
 
+# Even (FP64) or multiple-of-4 (FP32) innermost link extents take 16-byte vector loads.
 @pytest.mark.parametrize("pat", REDUCE, ids=lambda p: f"{p[0]}<-{p[1]},{p[2]}-" + "x".join(map(str, p[3].values())))
 @pytest.mark.parametrize("cp", [0.0, 0.5])
-def test_reduction_cut_k_vs_oracle(core, oracle, pat, cp):
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-5)])
+def test_reduction_cut_k_vs_oracle(core, oracle, pat, cp, dtype, tol):
     Ci, Ai, Bi, ext = pat
-    got, ref, _ = run_both(core, oracle, cp, Ci, Ai, Bi, ext, generic_only=True)
-    assert rel_fro(got, ref) < 1e-13
+    got, ref, _ = run_both(core, oracle, cp, Ci, Ai, Bi, ext, dtype=dtype, generic_only=True)
+    assert rel_fro(got, ref) < tol
 
 
 def _lib():
