@@ -6,16 +6,16 @@ include/einsums_amd.h). This package is the Python surface, shaped like EinsumsP
 torch.autograd layer in ``einsums_amd.autograd`` and a planner for contractions over more
 than two operands in ``einsums_amd.chain``.
 """
-from . import _lib
-from ._lib import version
-
 __all__ = ["core", "version"]
 
 
 def __getattr__(name):
-    # core imports torch; keep `import einsums_amd` light for symbol checks
-    if name in ("core", "autograd", "blocks", "chain"):
-        import importlib
+    # submodules load on first use: core imports torch, and _lib binds the built library,
+    # which `einsums_amd.build` must be able to replace without loading the old one
+    import importlib
 
+    if name in ("core", "autograd", "blocks", "chain", "_lib"):
         return importlib.import_module(f".{name}", __name__)
+    if name == "version":
+        return importlib.import_module("._lib", __name__).version
     raise AttributeError(name)

@@ -135,6 +135,8 @@ def _load():
                                                      _desc_p, _dp, ctypes.POINTER(ctypes.c_int32), _vp])
         sig[f"ein_einsum_execute_{sfx}"] = (_c_int, [ctypes.c_int32, T, _desc_p, _dp, T, _desc_p, _dp, _desc_p, _dp,
                                                      ctypes.POINTER(ctypes.c_int32), _vp])
+        sig[f"ein_syev_{sfx}"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _dp, _i64, _dp, _vp, _vp])
+        sig[f"ein_gesv_{sfx}"] = (_c_int, [_i64, _i64, _dp, _i64, _vp, _dp, _i64, _vp, _vp])
     sig["ein_gemm_split_f64"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64, _dp,
                                           _i64, ctypes.c_double, _dp, _i64, ctypes.c_int32, ctypes.c_int32, _vp])
     sig["ein_einsum_split_f64"] = (_c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _desc_p, _dp, ctypes.c_double,

@@ -26,7 +26,7 @@ from ._lib import (DOT, DIRECT, GEMM, GEMV, GENERIC, GER, INDETERMINATE, ALGORIT
 __all__ = [
     "compile_plan", "EinsumGenericPlan", "EinsumDotPlan", "EinsumDirectProductPlan", "EinsumGerPlan",
     "EinsumGemvPlan", "EinsumGemmPlan", "GPUView", "COPY", "MAP", "DEVICE_TENSOR", "gpu_enabled", "einsum",
-    "permute", "stream_handle", "DimensionError", "RankError", "EinsumsError", "contraction_route",
+    "permute", "stream_handle", "DimensionError", "RankError", "EinsumsError", "contraction_route", "syev", "gesv",
 ]
 
 # GPUView modes (libs/EinsumsPy/GPU/include/EinsumsPy/GPU/PyGPUView.hpp:27-46)
@@ -245,6 +245,76 @@ def permute(C_prefactor: float, C_indices: str, C: torch.Tensor, A_prefactor: fl
     if finC is not None:
         torch.cuda.current_stream().synchronize()
         finC()
+
+
+def _square_check(A: torch.Tensor, name: str) -> int:
+    if A.dim() != 2:
+        raise RankError(f"{name}: the matrix must be rank 2")
+    if A.shape[0] != A.shape[1]:
+        raise DimensionError(f"{name}: the matrix must be square")
+    if A.dtype not in _SUPPORTED or A.device.type != "cuda":
+        raise TypeError(f"{name}: float64 or float32 CUDA tensors only")
+    return int(A.shape[0])
+
+
+def syev(A: torch.Tensor, W: torch.Tensor, compute_vectors: bool = True, stream=None, check_info: bool = True):
+    """Symmetric eigendecomposition on the GPU (reference ``linear_algebra::syev(&A, &W)``,
+    LinearAlgebra.hpp:362): the upper triangle of A (n x n) is read; A is overwritten with
+    orthonormal eigenvectors as columns (``A[:, j]`` belongs to ``W[j]``) and W with the
+    eigenvalues in ascending order. Returns the device ``info`` tensor; with ``check_info``
+    it is read (a synchronisation) and a failure to converge raises, as in the reference."""
+    n = _square_check(A, "syev")
+    if W.dim() != 1:
+        raise RankError("syev: W must be rank 1")
+    if W.shape[0] != n:
+        raise DimensionError("syev: W must have n entries")
+    if W.dtype != A.dtype or not W.is_contiguous():
+        raise TypeError("syev: W must be a contiguous vector of A's dtype")
+    info = torch.zeros((), dtype=torch.int32, device=A.device)
+    job = b"V" if compute_vectors else b"N"
+    fn = _fn("ein_syev", A.dtype)
+    if A.stride(0) == 1 and A.stride(1) >= max(1, n):
+        # column-major storage: the upper triangle as the solver sees it
+        check(fn(job, b"U", n, _ptr(A), A.stride(1), _ptr(W), _ptr(info), stream_handle(stream)), "syev")
+    elif A.stride(1) == 1 and A.stride(0) >= max(1, n):
+        # row-major storage is the column-major transpose: its upper triangle is the solver's
+        # lower one, and the eigenvector columns come out as rows
+        check(fn(job, b"L", n, _ptr(A), A.stride(0), _ptr(W), _ptr(info), stream_handle(stream)), "syev")
+        if compute_vectors:
+            A.copy_(A.t().contiguous())
+    else:
+        tmp = A.contiguous()
+        syev(tmp, W, compute_vectors, stream, check_info=False)
+        A.copy_(tmp)
+    if check_info and int(info.item()) != 0:
+        raise EinsumsError("syev: the Jacobi iteration did not converge")
+    return info
+
+
+def gesv(A: torch.Tensor, B: torch.Tensor, stream=None, check_info: bool = True):
+    """Solve A X = B on the GPU (reference ``linear_algebra::gesv(&A, &B)``,
+    LinearAlgebra.hpp:495): A (n x n) is overwritten with its LU factors (unit L below the
+    diagonal, U on and above), B (n or n x nrhs) with X. Returns ``(info, ipiv)``: ipiv is
+    1-based on the device; info is an int read from the device when ``check_info``
+    (0, or i > 0 when U(i,i) is exactly zero, as from LAPACK), else the device tensor."""
+    n = _square_check(A, "gesv")
+    if B.dim() not in (1, 2) or B.shape[0] != n:
+        raise DimensionError("gesv: B must have n rows")
+    if B.dtype != A.dtype:
+        raise TypeError("gesv: A and B must share a dtype")
+    info = torch.zeros((), dtype=torch.int32, device=A.device)
+    ipiv = torch.zeros(max(n, 1), dtype=torch.int64, device=A.device)
+    Ac = A.t().contiguous().t()  # column-major copy
+    B2 = B.reshape(n, 1) if B.dim() == 1 else B
+    Bc = B2.t().contiguous().t()
+    nrhs = int(B2.shape[1])
+    fn = _fn("ein_gesv", A.dtype)
+    check(fn(n, nrhs, _ptr(Ac), max(1, n), _ptr(ipiv), _ptr(Bc), max(1, n), _ptr(info), stream_handle(stream)), "gesv")
+    A.copy_(Ac)
+    B2.copy_(Bc)
+    if check_info:
+        return int(info.item()), ipiv[:n]
+    return info, ipiv[:n]
 
 
 def gemm(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch.Tensor, beta: float, C: torch.Tensor,

@@ -225,6 +225,25 @@ int ein_einsum_execute_f32(int32_t algorithm, float c_prefactor, const ein_tenso
                            const ein_tensor_desc *A, const float *a, const ein_tensor_desc *B, const float *b,
                            int32_t *algorithm_used, ein_stream_t stream);
 
+/* Dense solvers for the small matrices of the SCF loop around the path, on one workgroup,
+ * asynchronous: LAPACK's `info` goes to the device int *info (0 = success) instead of the
+ * return value, which reports argument errors only.
+ * syev replaces blas::syev(job, uplo, n, a, lda, w, work, lwork)
+ * (libs/Einsums/BLAS/include/Einsums/BLAS.hpp:306; linear_algebra::syev, LinearAlgebra.hpp:362).
+ * Column-major; uplo names the triangle read; job 'V' overwrites A with orthonormal
+ * eigenvectors (columns), 'N' leaves A; W gets the eigenvalues ascending. Cyclic Jacobi,
+ * n <= 1024; *info = 1 if 60 sweeps do not converge. Workspace 2*n*n elements from the pool. */
+int ein_syev_f64(char job, char uplo, int64_t n, double *A, int64_t lda, double *W, int32_t *info, ein_stream_t stream);
+int ein_syev_f32(char job, char uplo, int64_t n, float *A, int64_t lda, float *W, int32_t *info, ein_stream_t stream);
+/* gesv replaces blas::gesv(n, nrhs, a, lda, ipiv, b, ldb) (BLAS.hpp:529; linear_algebra::gesv,
+ * LinearAlgebra.hpp:495): A = P L U with partial pivoting (ipiv 1-based, device int64),
+ * B overwritten with the solution; *info = i > 0 if U(i,i) is exactly zero (the factorization
+ * is completed and B is left as it was). Column-major, n <= 4096. */
+int ein_gesv_f64(int64_t n, int64_t nrhs, double *A, int64_t lda, int64_t *ipiv, double *B, int64_t ldb, int32_t *info,
+                 ein_stream_t stream);
+int ein_gesv_f32(int64_t n, int64_t nrhs, float *A, int64_t lda, int64_t *ipiv, float *B, int64_t ldb, int32_t *info,
+                 ein_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

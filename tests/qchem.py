@@ -1,8 +1,10 @@
 """RHF + MP2 on water/STO-3G, written once over an einsum backend.
 
 Follows the reference test QChemTests.cpp:185-415 step by step (same einsum calls,
-same index strings, same convergence rule); the symmetric eigensolver and S^-1/2 use
-numpy on the host, as the reference uses host LAPACK (syev) there.
+same index strings, same convergence rule). S^-1/2 uses numpy on the host; the Fock
+diagonalisation uses numpy too unless the backend brings its own ``syev`` (the GPU
+backend's ``device_syev=True`` keeps it on the device, as the reference's syev keeps it in
+LAPACK).
 """
 from __future__ import annotations
 
@@ -34,10 +36,19 @@ class NumpyOracleBackend:
 class TorchGpuBackend:
     """GPU backend: torch tensors on the device, einsums_amd compile_plan/execute."""
 
-    def __init__(self, core, torch):
+    def __init__(self, core, torch, device_syev=False):
         self.core = core
         self.torch = torch
         self.algorithms = []
+        if device_syev:
+            self.syev = self._syev
+
+    def _syev(self, F):
+        """Eigenvalues (host, for the MP2 denominators) and eigenvectors (device) of F."""
+        vecs = F.clone()
+        vals = self.torch.empty(F.shape[0], dtype=F.dtype, device=F.device)
+        self.core.syev(vecs, vals)
+        return vals.cpu().numpy(), vecs
 
     def zeros(self, shape):
         return self.torch.zeros(shape, dtype=self.torch.float64, device="cuda")
@@ -70,6 +81,8 @@ def rhf_mp2(be, S, T, V, TEI, nocc=5, enuc=8.002367061810450):
     def diagonalize(F):
         be.einsum(0.0, "ij", temp1, 1.0, "ik", F, "kj", X)
         be.einsum(0.0, "ij", Ft, 1.0, "kj", temp1, "ki", X)
+        if hasattr(be, "syev"):
+            return be.syev(Ft)
         evals, evecs = np.linalg.eigh(be.to_np(Ft))
         return evals, be.from_np(evecs)
 

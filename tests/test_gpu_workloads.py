@@ -75,13 +75,14 @@ def test_ccd_energy_and_permute(core, oracle, ccd_small):
     assert host(P).tobytes() == np.ascontiguousarray(I.t_oovv.transpose(0, 2, 1, 3)).tobytes()
 
 
-def test_water_rhf_mp2_on_gpu(core):
+@pytest.mark.parametrize("device_syev", [False, True], ids=["host-eigh", "device-syev"])
+def test_water_rhf_mp2_on_gpu(core, device_syev):
     from tests.helpers import read_water
     from tests.qchem import TorchGpuBackend, rhf_mp2
 
     w = json.loads((GOLD / "reference_known_answers.json").read_text())["water_sto3g"]
     S, T, V, G = read_water(GOLD / "water_sto3g")
-    be = TorchGpuBackend(core, torch)
+    be = TorchGpuBackend(core, torch, device_syev=device_syev)
     res = rhf_mp2(be, S, T, V, G, nocc=w["nocc"], enuc=w["E_nuc"])
     assert abs(res["E_HF"] - w["E_HF"]) < w["tolerance_abs"]
     assert abs(res["E_MP2"] - w["E_MP2"]) < w["tolerance_abs"]
