@@ -139,6 +139,11 @@ def _load():
         sig[f"ein_gesv_{sfx}"] = (_c_int, [_i64, _i64, _dp, _i64, _vp, _dp, _i64, _vp, _vp])
     sig["ein_gemm_split_f64"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64, _dp,
                                           _i64, ctypes.c_double, _dp, _i64, ctypes.c_int32, ctypes.c_int32, _vp])
+    sig["ein_gemm_host_c_f64"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64,
+                                           _dp, _i64, ctypes.c_double, _vp, _i64, ctypes.c_size_t, ctypes.c_int32,
+                                           ctypes.c_int32, _vp])
+    sig["ein_gemm_host_c_f32"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_float, _dp, _i64,
+                                           _dp, _i64, ctypes.c_float, _vp, _i64, ctypes.c_size_t, _vp])
     sig["ein_einsum_split_f64"] = (_c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _desc_p, _dp, ctypes.c_double,
                                             _desc_p, _dp, _desc_p, _dp, ctypes.POINTER(ctypes.c_int32), _vp])
     sig["ein_grouped_plan_create"] = (_c_int, [ctypes.c_int32, ctypes.POINTER(ContractionItem), ctypes.c_int32,

@@ -27,6 +27,7 @@ __all__ = [
     "compile_plan", "EinsumGenericPlan", "EinsumDotPlan", "EinsumDirectProductPlan", "EinsumGerPlan",
     "EinsumGemvPlan", "EinsumGemmPlan", "GPUView", "COPY", "MAP", "DEVICE_TENSOR", "gpu_enabled", "einsum",
     "permute", "stream_handle", "DimensionError", "RankError", "EinsumsError", "contraction_route", "syev", "gesv",
+    "gemm_host_c",
 ]
 
 # GPUView modes (libs/EinsumsPy/GPU/include/EinsumsPy/GPU/PyGPUView.hpp:27-46)
@@ -347,6 +348,41 @@ def gemm_split(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch
     status = lib.ein_gemm_split_f64(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0), _ptr(A),
                                     A.stride(0), float(beta), _ptr(C), C.stride(0), int(nslices), st, stream_handle(stream))
     check(status, "gemm_split")
+
+
+def gemm_host_c(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch.Tensor, beta: float, C: torch.Tensor,
+                panel_bytes: int = 0, split: int = 0, slice_type: str = "int8", stream=None) -> None:
+    """Row-major C = alpha*op(A) op(B) + beta*C with A, B on the GPU and C a host tensor
+    (pin it: ``torch.empty(..., pin_memory=True)``), streamed through the GPU in panels of
+    about ``panel_bytes``; ``split=s`` runs each panel in split precision (FP64 only).
+    Returns when C holds the result."""
+    for t in (A, B):
+        if t.dim() != 2 or t.stride(1) != 1 or t.device.type != "cuda":
+            raise ValueError("gemm_host_c: A and B must be 2-D row-major CUDA tensors")
+    if C.dim() != 2 or C.stride(1) != 1 or C.device.type != "cpu":
+        raise ValueError("gemm_host_c: C must be a 2-D row-major host tensor")
+    if not (A.dtype == B.dtype == C.dtype) or A.dtype not in _SUPPORTED:
+        raise TypeError("gemm_host_c: operands must share float64 or float32")
+    ta, tb = transA.upper() != "N", transB.upper() != "N"
+    m = A.shape[1] if ta else A.shape[0]
+    k = A.shape[0] if ta else A.shape[1]
+    n = B.shape[0] if tb else B.shape[1]
+    if tuple(C.shape) != (m, n) or (B.shape[1] if tb else B.shape[0]) != k:
+        raise DimensionError("gemm_host_c: shapes do not match")
+    cptr = ctypes.c_void_p(C.data_ptr())
+    # row-major C = op(A) op(B) is column-major C^T = op(B)^T op(A)^T: swap the operands
+    if A.dtype == torch.float64:
+        st = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16, "int8": _lib.SLICE_INT8}[slice_type]
+        status = lib.ein_gemm_host_c_f64(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0),
+                                         _ptr(A), A.stride(0), float(beta), cptr, C.stride(0), int(panel_bytes),
+                                         int(split), st, stream_handle(stream))
+    else:
+        if split:
+            raise TypeError("gemm_host_c: split precision applies to float64")
+        status = lib.ein_gemm_host_c_f32(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0),
+                                         _ptr(A), A.stride(0), float(beta), cptr, C.stride(0), int(panel_bytes),
+                                         stream_handle(stream))
+    check(status, "gemm_host_c")
 
 
 _ROUTES = {"auto": _lib.ROUTE_AUTO, "gather": _lib.ROUTE_GATHER, "ttgt": _lib.ROUTE_TTGT}

@@ -8,6 +8,7 @@
 // both land here. Reference leaf replaced: blas::gpu::gemm -> hipblasDgemm_64
 // (libs/Einsums/hipBLAS/include/Einsums/hipBLAS.hpp:64, hipBLASVendor/src/gemm.hip:82-155).
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "gemm.hpp"
@@ -426,10 +427,111 @@ int gemm_colmajor(char ta, char tb, int64_t m, int64_t n, int64_t k, T alpha, T 
     return gemm_strided<T>(P, nullptr, stream);
 }
 
+// C = alpha*op(A)*op(B) + beta*C with C in host memory (column-major, ldc) and A, B on the
+// device: the output for a C that does not fit in HBM next to its operands (the v^4
+// intermediates of a large CCD). Column panels of C cycle through two device buffers. Panel
+// i's upload (stream `up`), its GEMM (the caller's stream) and its download (stream `down`)
+// are chained by events, so the upload of panel i+1 and the download of panel i-1 overlap the
+// GEMM of panel i, and both PCIe directions run at once. A buffer is reused only after its
+// previous download. Returns when C is back in host memory. Host C should be pinned
+// (hipHostMalloc); pageable memory works but copies through a staging buffer.
+template <typename T>
+int gemm_host_c(char ta, char tb, int64_t m, int64_t n, int64_t k, T alpha, T const *A, int64_t lda, T const *B,
+                int64_t ldb, T beta, T *C, int64_t ldc, size_t panel_bytes, int nslices, int slice_type,
+                hipStream_t stream) {
+    EIN_TRY(check_gemm_args(ta, tb, m, n, k, lda, ldb, ldc));
+    if (C == nullptr && m > 0 && n > 0) return fail(-12, "gemm_host_c: C is null");
+    if (m == 0 || n == 0) return EIN_SUCCESS;
+    if (panel_bytes == 0) panel_bytes = size_t(256) << 20;
+    int64_t const cols = std::max<int64_t>(1, std::min<int64_t>(n, int64_t(panel_bytes / (size_t(m) * sizeof(T)))));
+    int64_t const npan = ceil_div(n, cols);
+    static std::mutex mu; // the side streams and events below are shared by all calls
+    std::lock_guard<std::mutex> lock(mu);
+    static hipStream_t up = nullptr, down = nullptr;
+    static hipEvent_t ev_start, ev_up[2], ev_comp[2], ev_down[2];
+    if (up == nullptr) {
+        EIN_HIP_CHECK(hipStreamCreateWithFlags(&up, hipStreamNonBlocking));
+        EIN_HIP_CHECK(hipStreamCreateWithFlags(&down, hipStreamNonBlocking));
+        EIN_HIP_CHECK(hipEventCreateWithFlags(&ev_start, hipEventDisableTiming));
+        for (int b = 0; b < 2; ++b) {
+            EIN_HIP_CHECK(hipEventCreateWithFlags(&ev_up[b], hipEventDisableTiming));
+            EIN_HIP_CHECK(hipEventCreateWithFlags(&ev_comp[b], hipEventDisableTiming));
+            EIN_HIP_CHECK(hipEventCreateWithFlags(&ev_down[b], hipEventDisableTiming));
+        }
+    }
+    void *buf[2] = {nullptr, nullptr};
+    size_t const pbytes = size_t(m) * size_t(cols) * sizeof(T);
+    EIN_TRY(ein_malloc(&buf[0], pbytes));
+    if (int const rc = ein_malloc(&buf[1], pbytes); rc != EIN_SUCCESS) {
+        ein_free(buf[0]);
+        return rc;
+    }
+    int status = EIN_SUCCESS;
+    auto run = [&]() -> int {
+        // the side streams start after the work already queued on the caller's stream
+        EIN_HIP_CHECK(hipEventRecord(ev_start, stream));
+        EIN_HIP_CHECK(hipStreamWaitEvent(up, ev_start, 0));
+        EIN_HIP_CHECK(hipStreamWaitEvent(down, ev_start, 0));
+        for (int64_t p = 0; p < npan; ++p) {
+            int const b     = int(p & 1);
+            int64_t const j0 = p * cols;
+            int64_t const nc = std::min<int64_t>(cols, n - j0);
+            T *Cd            = static_cast<T *>(buf[b]);
+            T *Ch            = C + j0 * ldc;
+            if (p >= 2) EIN_HIP_CHECK(hipStreamWaitEvent(up, ev_down[b], 0)); // buffer free again
+            if (beta != T(0))
+                EIN_HIP_CHECK(hipMemcpy2DAsync(Cd, size_t(m) * sizeof(T), Ch, size_t(ldc) * sizeof(T),
+                                               size_t(m) * sizeof(T), size_t(nc), hipMemcpyHostToDevice, up));
+            EIN_HIP_CHECK(hipEventRecord(ev_up[b], up));
+            EIN_HIP_CHECK(hipStreamWaitEvent(stream, ev_up[b], 0));
+            T const *Bp = is_notrans(tb) ? B + j0 * ldb : B + j0;
+            if (nslices > 0) {
+                if constexpr (sizeof(T) == 8)
+                    EIN_TRY(ein_gemm_split_f64(ta, tb, m, nc, k, alpha, A, lda, Bp, ldb, beta, Cd, m, nslices, slice_type,
+                                               stream));
+                else
+                    return fail(EIN_ERROR_UNSUPPORTED, "gemm_host_c: split precision is FP64 only");
+            } else {
+                EIN_TRY(gemm_colmajor<T>(ta, tb, m, nc, k, alpha, A, lda, 0, Bp, ldb, 0, beta, Cd, m, 0, 1, stream));
+            }
+            EIN_HIP_CHECK(hipEventRecord(ev_comp[b], stream));
+            EIN_HIP_CHECK(hipStreamWaitEvent(down, ev_comp[b], 0));
+            EIN_HIP_CHECK(hipMemcpy2DAsync(Ch, size_t(ldc) * sizeof(T), Cd, size_t(m) * sizeof(T), size_t(m) * sizeof(T),
+                                           size_t(nc), hipMemcpyDeviceToHost, down));
+            EIN_HIP_CHECK(hipEventRecord(ev_down[b], down));
+        }
+        return EIN_SUCCESS;
+    };
+    status = run();
+    // the buffers go back to the pool only after every queued copy and GEMM has finished
+    hipError_t const e1 = hipStreamSynchronize(down);
+    hipError_t const e2 = hipStreamSynchronize(up);
+    hipError_t const e3 = hipStreamSynchronize(stream);
+    ein_free(buf[0]);
+    ein_free(buf[1]);
+    if (status != EIN_SUCCESS) return status;
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+        return fail(EIN_ERROR_HIP, std::string("gemm_host_c: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3));
+    return EIN_SUCCESS;
+}
+
 } // namespace
 } // namespace ein
 
 extern "C" {
+
+int ein_gemm_host_c_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+                        int64_t lda, const double *B, int64_t ldb, double beta, double *C, int64_t ldc,
+                        size_t panel_bytes, int32_t nslices, int32_t slice_type, ein_stream_t stream) {
+    return ein::gemm_host_c<double>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, panel_bytes, nslices,
+                                    slice_type, ein::as_stream(stream));
+}
+int ein_gemm_host_c_f32(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                        int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc, size_t panel_bytes,
+                        ein_stream_t stream) {
+    return ein::gemm_host_c<float>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, panel_bytes, 0, 0,
+                                   ein::as_stream(stream));
+}
 
 int ein_gemm_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
                  const double *B, int64_t ldb, double beta, double *C, int64_t ldc, ein_stream_t stream) {

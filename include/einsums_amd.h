@@ -134,6 +134,19 @@ int ein_axpby_f32(int64_t n, float alpha, const float *x, int64_t incx, float be
 int ein_gemm_split_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
                        const double *B, int64_t ldb, double beta, double *C, int64_t ldc, int32_t nslices,
                        int32_t slice_type, ein_stream_t stream);
+/* C = alpha*op(A)*op(B) + beta*C with A, B on the device and C (column-major, ldc) in host
+ * memory, for outputs larger than HBM (the out-of-core case of the v^4 CCD intermediates,
+ * which the reference keeps as a DiskTensor, libs/Einsums/Tensor/include/Einsums/
+ * DiskTensor.hpp:200-306). Column panels of about panel_bytes (0 = 256 MiB) stream through
+ * two device buffers; uploads, GEMMs and downloads of consecutive panels overlap. nslices > 0
+ * runs each panel's GEMM in split precision (slice_type EIN_SLICE_*). Returns when C is back
+ * in host memory; host C should be pinned (hipHostMalloc). Calls are serialised internally. */
+int ein_gemm_host_c_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+                        int64_t lda, const double *B, int64_t ldb, double beta, double *C, int64_t ldc,
+                        size_t panel_bytes, int32_t nslices, int32_t slice_type, ein_stream_t stream);
+int ein_gemm_host_c_f32(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                        int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc, size_t panel_bytes,
+                        ein_stream_t stream);
 /* ein_einsum_f64 with the GEMM-mapped case run by the split-precision GEMM; other
  * patterns run exactly as in ein_einsum_f64. */
 int ein_einsum_split_f64(int32_t nslices, int32_t slice_type, double c_prefactor, const ein_tensor_desc *C, double *c,
