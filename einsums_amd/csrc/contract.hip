@@ -1004,10 +1004,12 @@ int contract_generic(T c_prefactor, ein_tensor_desc const &C, T *c, T ab_prefact
         R.k_chunk = R.nk;
     }
     // too few targets for about four rounds of resident waves: cut each target's K range
-    // into 4 * parts pieces of >= 256 links
+    // into 4 * parts pieces of >= 256 links; partials (parts > 1, one more launch) only when
+    // four waves per target do not even fill the chip once
     int64_t const resident = int64_t(cus) * 32;
     if (NT < 4 * resident && K >= 1024) {
         int64_t s = std::min<int64_t>(ceil_div(4 * resident, NT), K / 256) / 4;
+        if (4 * NT >= resident) s = 1;
         s         = std::max<int64_t>(1, std::min<int64_t>(s, 1024));
         R.splits  = int(4 * s);
         R.parts   = int(s);
