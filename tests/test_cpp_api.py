@@ -45,6 +45,14 @@ def test_device_blocks_program():
     assert "PASS test_device_blocks" in res.stdout
 
 
+@pytest.mark.gpu
+def test_device_linalg_program():
+    """linear_algebra::syev / gesv on row-major DeviceTensors."""
+    res = _run("test_device_linalg", 120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "PASS test_device_linalg" in res.stdout
+
+
 def test_lds_layout_program():
     """Host check of the direct-to-LDS slab layouts: exact coverage, LDS positions the MFMA
     fragment reads expect, and conflict-free fragment reads."""
