@@ -299,6 +299,11 @@ GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, in
     (void)elem_bytes;
     int64_t const t128 = ceil_div(m, 128) * ceil_div(n, 128) * batch;
     if (t128 < num_cus() / 4) cfg.bn = 64;
+    // One to two rounds of 128 x 128 tiles per CU (2304^3: 324 tiles on 256 CUs): planning the
+    // schedule for one workgroup per CU runs one full data-parallel round and spreads only the
+    // rest by stream-K, where planning for two splits every tile into pieces (2304^3 FP64:
+    // 0.407 -> 0.395 ms; the packed Wmbej at o=24, v=96: 0.484 -> 0.466 ms).
+    if (cfg.bn == 128 && t128 >= num_cus() && t128 < 2 * int64_t(num_cus())) cfg.blocks_per_cu = 1;
     return cfg;
 }
 
