@@ -299,11 +299,13 @@ GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, in
     (void)elem_bytes;
     int64_t const t128 = ceil_div(m, 128) * ceil_div(n, 128) * batch;
     if (t128 < num_cus() / 4) cfg.bn = 64;
-    // One to two rounds of 128 x 128 tiles per CU (2304^3: 324 tiles on 256 CUs): planning the
-    // schedule for one workgroup per CU runs one full data-parallel round and spreads only the
-    // rest by stream-K, where planning for two splits every tile into pieces (2304^3 FP64:
-    // 0.407 -> 0.395 ms; the packed Wmbej at o=24, v=96: 0.484 -> 0.466 ms).
-    if (cfg.bn == 128 && t128 >= num_cus() && t128 < 2 * int64_t(num_cus())) cfg.blocks_per_cu = 1;
+    // At least one round of 128 x 128 tiles: plan the schedule for one workgroup per CU. A
+    // second resident workgroup per CU adds no MFMA throughput here (8192^3 and 16384^2 x
+    // 4096 FP64 measure the same either way), and planning for one keeps whole tiles in the
+    // data-parallel rounds where planning for two splits the last round, or with one to two
+    // rounds every tile, into stream-K pieces. Measured FP64: 2304^3 56.2 -> 58.8 TFLOP/s,
+    // 3072^3 65.8 -> 66.4, the Wabef update (9216 x 9216 x 576) 1.565 -> 1.555 ms.
+    if (cfg.bn == 128 && t128 >= num_cus()) cfg.blocks_per_cu = 1;
     return cfg;
 }
 
