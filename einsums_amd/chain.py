@@ -200,6 +200,9 @@ class ContractionChain:
         return f"ContractionChain({','.join(self.inputs)}->{self.output}, {len(self.steps)} steps, {self.flops:.3g} flop)"
 
     def execute(self, C_prefactor, C, AB_prefactor, *operands, stream=None) -> None:
+        """Run the chain on ``stream`` (None = torch's current stream). Intermediates are torch
+        allocations, so with another stream run the call under ``torch.cuda.stream(stream)``
+        to keep the caching allocator's reuse ordered with the kernels."""
         import torch
 
         if len(operands) != len(self.inputs):
