@@ -6,8 +6,11 @@ Wmbej += -0.5 t_jnfb g_mnef (generic, non-GEMM layout), E = 0.25 t.g (DOT) and t
 permute ijab -> iajb. `value` is the step's algorithmic FP64 FLOP / step time (GFLOP/s).
 
 Prints one JSON line with the roofline of the dominant kernel (Wabef, FP64 MFMA bound,
-timed with HIP events on the stream the kernels run on) and the CPU baseline (the
-oracle's restatement of the reference CPU einsum timed on this host).
+timed with HIP events on the stream the kernels run on), the split-precision block (the
+same W intermediates at o=48, v=256 as plain FP64, int8 / BF16 / FP16 slices and FP32,
+timed on the GPU; their errors against the oracle on sampled rows come from the CPU leg)
+and the CPU baseline (the oracle's restatement of the reference CPU einsum timed on this
+host).
 """
 from __future__ import annotations
 
@@ -35,7 +38,11 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--traffic", type=str, default=str(ROOT / "profiles" / "gemm_dense_wabef_o24_v96_traffic.txt"),
                    help="rocprof summary holding the measured HBM bytes per Wabef launch (o=24, v=96 only)")
-    p.add_argument("--split", type=int, default=1, help="1: add the split-precision comparison, 0: skip it")
+    p.add_argument("--split", type=int, default=1,
+                   help="1: add the split-precision vs FP64 block at o=48, v=256 (checked against the oracle in "
+                        "the CPU leg), 0: skip it")
+    p.add_argument("--kernel-trace", type=str, default=str(ROOT / "profiles" / "ccd_step_o24_v96_kernel_trace.txt"),
+                   help="rocprof kernel-trace summary of this bench holding the Wabef kernel's average duration")
     p.add_argument("--grouped", action="store_true",
                    help="only run the batched uneven-block benchmarks (one JSON line) and exit")
     p.add_argument("--precision-sweep", action="store_true",
@@ -66,45 +73,128 @@ def calibrate_copy(nbytes: int) -> None:
     print(json.dumps({"calibrate_copy_bytes_read": n * 8, "calibrate_copy_bytes_written": n * 8, "launches": 3}))
 
 
-def cpu_baseline(I, o, v, flops):
-    """Times the oracle (reference CPU einsum restated: dispatch + host BLAS / generic loop
-    nest) on one full step of the same workload."""
+def rocprof_figure(path, flop):
+    """The Wabef kernel's average duration in the committed rocprofv3 kernel trace of this
+    bench (line `Wabef_kernel_avg_us: <us>`), turned into the same roofline fraction as the
+    live HIP-event figure. The trace comes from a separate run; the file names it."""
+    p = Path(path)
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("Wabef_kernel_avg_us:"):
+            us = float(line.split(":")[1])
+            tf = flop / (us * 1e-6) / 1e12
+            return {"file": str(p.relative_to(ROOT)) if p.is_relative_to(ROOT) else str(p), "avg_us": us,
+                    "achieved": round(tf, 3), "frac": round(tf / FP64_MFMA_PEAK_TFLOPS, 4)}
+    return None
+
+
+def host_cpu_info():
+    """(threads to use, CPU model). The GPU box allots 16 host CPUs to its one GPU whatever
+    `nproc` reports there, so the oracle uses at most 16 threads (this container: 8)."""
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_leg(I, o, v, flops, large, budget_s=20.0, max_reps=20, warmups=3):
+    """The bench's CPU leg: everything here runs the oracle (the restated reference CPU
+    einsum, oracle/), which the GPU legs never touch.
+
+    1. CPU baseline: one full step (Wmnij, Wabef, Wmbej, E, permute) at (o, v) on the
+       host's cores. The host BLAS is resolved before timing; W is restored from a pristine
+       copy before every repetition, untimed; `warmups` untimed steps, then repetitions
+       until `max_reps` or `budget_s` seconds of timed work, whichever comes first.
+       Reports GFLOP/s of the median step, with the minimum alongside.
+    2. Checker of the split-precision block: the oracle's contraction on the sampled rows of
+       the o=48, v=256 W intermediates, against which every GPU mode's rows are compared."""
+    cores, model = host_cpu_info()
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import numpy as np
 
-    from oracle import pyoracle
+    from oracle import pyoracle, sampled
 
-    cores = min(len(os.sched_getaffinity(0)), 16)
-    os.environ.setdefault("OPENBLAS_NUM_THREADS", str(cores))
     pyoracle.set_threads(cores)
-    W = {"Wmnij": I.g_oooo.copy(), "Wabef": I.g_vvvv.copy(), "Wmbej": I.g_ovvo.copy()}
+    blas = os.path.basename(pyoracle.blas_name())  # dlopen of the host BLAS, outside the timing
+    W0 = {"Wmnij": I.g_oooo, "Wabef": I.g_vvvv, "Wmbej": I.g_ovvo}
+    W = {k: a.copy() for k, a in W0.items()}
     E = np.zeros(())
     P = np.empty((o, v, o, v))
-    t0 = time.perf_counter()
-    pyoracle.einsum(1.0, "mnij", W["Wmnij"], 0.25, "ijef", I.t_oovv, "mnef", I.g_oovv)
-    pyoracle.einsum(1.0, "abef", W["Wabef"], 0.25, "mnef", I.g_oovv, "mnab", I.t_oovv)
-    pyoracle.einsum(1.0, "mbej", W["Wmbej"], -0.5, "jnfb", I.t_oovv, "mnef", I.g_oovv)
-    pyoracle.einsum(0.0, "", E, 0.25, "ijab", I.t_oovv, "ijab", I.g_oovv)
-    pyoracle.permute(0.0, "iajb", P, 1.0, "ijab", I.t_oovv)
-    dt = time.perf_counter() - t0
+
+    def step():
+        pyoracle.einsum(1.0, "mnij", W["Wmnij"], 0.25, "ijef", I.t_oovv, "mnef", I.g_oovv)
+        pyoracle.einsum(1.0, "abef", W["Wabef"], 0.25, "mnef", I.g_oovv, "mnab", I.t_oovv)
+        pyoracle.einsum(1.0, "mbej", W["Wmbej"], -0.5, "jnfb", I.t_oovv, "mnef", I.g_oovv)
+        pyoracle.einsum(0.0, "", E, 0.25, "ijab", I.t_oovv, "ijab", I.g_oovv)
+        pyoracle.permute(0.0, "iajb", P, 1.0, "ijab", I.t_oovv)
+
+    def restore():
+        for k in W:
+            np.copyto(W[k], W0[k])
+
+    for _ in range(warmups):
+        restore()
+        step()
+    times = []
+    while len(times) < max_reps and sum(times) < budget_s:
+        restore()
+        t0 = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - t0)
     total = sum(flops.values())
-    return {"value": round(total / dt / 1e9, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"one full step (Wmnij, Wabef, Wmbej, E, permute) at o={o} v={v}, {dt:.2f} s; "
-                      f"host BLAS: {os.path.basename(pyoracle.blas_name())}"}
+    med, mn = float(np.median(times)), float(min(times))
+    base = {"value": round(total / med / 1e9, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+            "sample": f"full steps (Wmnij, Wabef, Wmbej, E, permute) at o={o} v={v}: {warmups} warm-ups, "
+                      f"{len(times)} timed reps, median {med:.3f} s (min {mn:.3f} s, "
+                      f"{total / mn / 1e9:.1f} GFLOP/s); CPU {model}; host BLAS {blas}; "
+                      f"OMP_PROC_BIND={os.environ['OMP_PROC_BIND']}"}
+    if large is not None:
+        t_h, g_h = large.pop("t_host"), large.pop("g_host")
+        for name, row in large["contractions"].items():
+            rows = row.pop("_rows")
+            ref = sampled.ROWS_OF[name](t_h, g_h, rows).reshape(len(rows), -1)
+            for key, got in row.pop("_got").items():
+                row[key]["rel_err_vs_oracle"] = float(f"{sampled.rel_err(got, ref):.3e}")
+        for key in large["total"]:
+            large["total"][key]["max_rel_err_vs_oracle"] = max(
+                row[key]["rel_err_vs_oracle"] for row in large["contractions"].values())
+    return base
 
 
-SPLIT_MODES = (("bf16", 5), ("int8", 6))
+SPLIT_MODES = (("int8", 6), ("bf16", 5), ("fp16", 5))
 
 
-def split_comparison(core, plans, W, reps=10):
-    """Wabef and Wmnij through the split-precision GEMM (BF16 x5 and int8 x6 slices) vs plain
-    FP64 on the same inputs: kernel-time speedup and relative Frobenius error of the
-    contraction against the plain FP64 result (itself within 1e-13 of the CPU oracle in the
-    parity tests)."""
+def split_large(core, o=48, v=256, reps=3):
+    """Split precision vs plain FP64 on the CCD W intermediates at o=48, v=256: Wmnij (GEMM
+    2304 x 2304 x 65536), Wabef (GEMM 65536 x 65536 x 2304, 34 GB output) and Wmbej
+    (generic, 12288^3 after packing), each `W += prefactor * contraction` with inputs
+    resident in HBM. Every mode is timed with HIP events over `reps` calls after one warm-up
+    (W accumulates; the timing does not depend on W's values). A separate call on a fresh
+    copy of W gives the rows that the CPU leg compares with the oracle: 64 sampled rows of
+    each intermediate (workloads.sample_rows, seed 97)."""
     import torch
 
+    from einsums_amd import workloads
+
+    I = workloads.ccd_inputs_device(o, v, seed=11)
+    specs = {c[0]: c for c in workloads.CCD_CONTRACTIONS}
+    arrs = {"t_oovv": I.t_oovv, "g_oovv": I.g_oovv, "g_oooo": I.g_oooo, "g_vvvv": I.g_vvvv, "g_ovvo": I.g_ovvo}
+    nrows = {"Wmnij": o * o, "Wabef": v * v, "Wmbej": o * v}
+    modes = [("fp64", 0)] + list(SPLIT_MODES) + [("fp32", 0)]
+
+    def key(st, ns):
+        return st if not ns else f"{st}x{ns}"
+
     def timed(fn):
-        for _ in range(2):
-            fn()
+        fn()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -114,21 +204,41 @@ def split_comparison(core, plans, W, reps=10):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    out = {}
-    for name in ("Wabef", "Wmnij"):
-        plan, A, B, abp, _ = plans[name]
-        ref = torch.zeros_like(W[name])
-        plan.execute(0.0, ref, abp, A, B)
-        got = torch.zeros_like(ref)
-        fp64_ms = timed(lambda: plan.execute(0.0, got, abp, A, B))
-        row = {"fp64_ms": round(fp64_ms, 4)}
-        for st, ns in SPLIT_MODES:
-            plan.execute(0.0, got, abp, A, B, split=ns, slice_type=st)
-            torch.cuda.synchronize()
-            err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
-            ms = timed(lambda: plan.execute(0.0, got, abp, A, B, split=ns, slice_type=st))
-            row[f"{st}x{ns}"] = {"ms": round(ms, 4), "speedup": round(fp64_ms / ms, 3), "rel_err_vs_fp64": float(f"{err:.3e}")}
-        out[name] = row
+    out = {"workload": f"CCD W intermediates o={o} v={v}, W += prefactor*contraction (beta=1), FP64 in HBM",
+           "rows_checked": "64 sampled rows of each intermediate against the oracle (CPU leg)",
+           "contractions": {}, "total": {}}
+    for name in ("Wmnij", "Wabef", "Wmbej"):
+        _, Ci, An, Ai, Bn, Bi, cp, abp, init = specs[name]
+        A, B, W0 = arrs[An], arrs[Bn], arrs[init]
+        flop = {"Wmnij": 2.0 * o ** 4 * v ** 2, "Wabef": 2.0 * v ** 4 * o ** 2, "Wmbej": 2.0 * o ** 3 * v ** 3}[name]
+        rows = workloads.sample_rows(nrows[name], 64)
+        r = torch.from_numpy(rows).cuda()
+        row = {"gflop": round(flop / 1e9, 1), "_rows": rows, "_got": {}}
+        plan = core.compile_plan(Ci, Ai, Bi)
+        for st, ns in modes:
+            if st == "fp32":
+                A_, B_, W = A.float(), B.float(), W0.float()
+            else:
+                A_, B_, W = A, B, W0.clone()
+            base = W0.reshape(nrows[name], -1)[r].to(W.dtype).double()
+            kw = {"split": ns, "slice_type": st} if ns else {}
+            plan.execute(cp, W, abp, A_, B_, **kw)
+            row["_got"][key(st, ns)] = (W.reshape(nrows[name], -1)[r].double() - base).cpu().numpy()
+            ms = timed(lambda: plan.execute(cp, W, abp, A_, B_, **kw))
+            row[key(st, ns)] = {"ms": round(ms, 3), "tflops": round(flop / ms / 1e9, 2)}
+            del W, A_, B_
+            torch.cuda.empty_cache()
+        for st, ns in modes[1:]:
+            row[key(st, ns)]["speedup"] = round(row["fp64"]["ms"] / row[key(st, ns)]["ms"], 3)
+        out["contractions"][name] = row
+    fp64_total = sum(out["contractions"][n]["fp64"]["ms"] for n in out["contractions"])
+    for st, ns in modes:
+        tot = sum(out["contractions"][n][key(st, ns)]["ms"] for n in out["contractions"])
+        out["total"][key(st, ns)] = {"ms": round(tot, 3), "speedup": round(fp64_total / tot, 3)}
+    out["t_host"] = I.t_oovv.cpu().numpy()
+    out["g_host"] = I.g_oovv.cpu().numpy()
+    del I, arrs
+    torch.cuda.empty_cache()
     return out
 
 
@@ -540,10 +650,21 @@ def main():
                      "traffic": traffic},
         "kernels": kernels,
     }
+    prof = rocprof_figure(args.kernel_trace, flops[dom]) if (o, v) == (24, 96) else None
+    if prof is not None:
+        out["roofline"]["rocprof"] = prof
+    large = None
     if args.split:
-        out["split"] = split_comparison(core, plans, W)
+        large = split_large(core)
+        out["split"] = large
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(I, o, v, flops)
+        out["cpu_baseline"] = cpu_leg(I, o, v, flops, large)
+    elif large is not None:
+        for k in ("t_host", "g_host"):
+            large.pop(k)
+        for row in large["contractions"].values():
+            row.pop("_rows")
+            row.pop("_got")
     print(json.dumps(out))
 
 

@@ -90,3 +90,37 @@ def ccd_bytes(o: int, v: int, elem: int = 8) -> dict:
         "E": elem * (2 * t),
         "permute": elem * (2 * t),
     }
+
+
+def sample_rows(nrows: int, count: int = 64, seed: int = 97) -> np.ndarray:
+    """`count` distinct row indices out of `nrows`, sorted, from a seeded generator, with
+    the first and last row always included: the rows on which a large result is compared
+    with a host computation."""
+    rng = np.random.default_rng(seed)
+    count = min(count, nrows)
+    inner = rng.choice(np.arange(1, nrows - 1), size=max(0, count - 2), replace=False) if nrows > 2 else []
+    return np.unique(np.concatenate([[0, nrows - 1], np.asarray(inner, dtype=np.int64)]))[:count].astype(np.int64)
+
+
+def ccd_inputs_device(o: int, v: int, seed: int = 11, with_vvvv: bool = True, device: str = "cuda") -> CCDInputs:
+    """The CCD inputs of `ccd_inputs` (same distributions, antisymmetrised the same way),
+    drawn on the device with torch's counter-based generator so that o=48, v=256 (g_vvvv
+    of 34.4 GB) is made in milliseconds. Fields are contiguous float64 device tensors."""
+    import torch
+
+    gen = torch.Generator(device=device).manual_seed(seed)
+
+    def normal(shape, scale):
+        return torch.randn(shape, dtype=torch.float64, device=device, generator=gen).mul_(scale)
+
+    def anti(x):
+        x = (x - x.transpose(0, 1)).mul_(0.5)
+        return (x - x.transpose(2, 3)).mul_(0.5)
+
+    s = 1.0 / float(np.sqrt(v))
+    t = anti(normal((o, o, v, v), 0.01))
+    goooo = anti(normal((o, o, o, o), s))
+    goovv = anti(normal((o, o, v, v), s))
+    gvvvv = anti(normal((v, v, v, v), s)) if with_vvvv else None
+    govvo = normal((o, v, v, o), s)
+    return CCDInputs(o, v, t, goooo, goovv, gvvvv, govvo)

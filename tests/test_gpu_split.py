@@ -50,8 +50,11 @@ SHAPES = [(1, 1, 1), (5, 7, 3), (64, 64, 32), (100, 130, 257), (257, 129, 1000),
 
 @pytest.mark.parametrize("m,n,k", SHAPES)
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
-@pytest.mark.parametrize("slice_type,nslices,tol", [(0, 5, 1e-11), (0, 3, 1e-6), (2, 6, 1e-11), (2, 3, 1e-5)],
-                         ids=["bf16x5", "bf16x3", "int8x6", "int8x3"])
+@pytest.mark.parametrize("slice_type,nslices,tol", [(0, 5, 1e-11), (0, 3, 1e-6), (0, 4, 2e-9), (0, 6, 1e-13),
+                                                     (1, 4, 2e-9), (1, 6, 1e-13), (2, 6, 1e-11), (2, 3, 1e-5),
+                                                     (2, 4, 2e-7), (2, 5, 2e-10)],
+                         ids=["bf16x5", "bf16x3", "bf16x4", "bf16x6", "fp16x4", "fp16x6", "int8x6", "int8x3", "int8x4",
+                              "int8x5"])
 def test_split_gemm_vs_oracle(oracle, m, n, k, ta, tb, slice_type, nslices, tol):
     from einsums_amd import _lib
 
