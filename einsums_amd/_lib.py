@@ -106,6 +106,7 @@ def _load():
     sig["ein_init"] = (_c_int, [])
     sig["ein_malloc"] = (_c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t])
     sig["ein_free"] = (_c_int, [_vp])
+    sig["ein_free_async"] = (_c_int, [_vp, _vp])
     sig["ein_workspace_reserve"] = (_c_int, [_vp, ctypes.c_size_t])
     sig["ein_workspace_size"] = (ctypes.c_size_t, [_vp])
     sig["ein_stream_synchronize"] = (_c_int, [_vp])

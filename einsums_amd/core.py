@@ -142,14 +142,23 @@ class EinsumGenericPlan:
 
     def execute(self, C_prefactor, C, AB_prefactor, A, B, stream: torch.cuda.Stream | None = None,
                 split: int = 0, slice_type: str = "bf16") -> None:
-        """Run the plan. ``split=s`` (3..6) runs an FP64 GEMM-mapped contraction with the
-        split-precision (Ozaki) GEMM using s BF16 or FP16 slices per operand."""
+        """Run the plan on ``stream`` (torch's current stream when None). ``split=s`` (3..6)
+        runs an FP64 GEMM-mapped contraction with the split-precision (Ozaki) GEMM, cutting
+        each operand into s slices of ``slice_type`` "bf16", "fp16" or "int8"."""
         if isinstance(C, (list, tuple)):
             if not (isinstance(A, (list, tuple)) and isinstance(B, (list, tuple))) or not (len(C) == len(A) == len(B)):
                 raise ValueError("The lengths of the collections need to be the same!")
             for c, a, b in zip(C, A, B):
                 self.execute(C_prefactor, c, AB_prefactor, a, b, stream, split, slice_type)
             return
+        # every torch op of the call (staging copies, temporaries, the copy back) runs on the
+        # kernel's stream, so they are ordered with it and the caching allocator reuses the
+        # temporaries only after the kernel
+        with torch.cuda.stream(stream):
+            self._execute_one(C_prefactor, C, AB_prefactor, A, B, split, slice_type)
+
+    def _execute_one(self, C_prefactor, C, AB_prefactor, A, B, split, slice_type) -> None:
+        stream = None  # the current stream, set by execute()
         dC, finC = _as_device(C, write=True)
         dA, _ = _as_device(A, write=False)
         dB, _ = _as_device(B, write=False)
@@ -227,6 +236,12 @@ def permute(C_prefactor: float, C_indices: str, C: torch.Tensor, A_prefactor: fl
     ``permute(beta, Indices{...}, &C, alpha, Indices{...}, A)``, Permute.hpp:330-407)."""
     if sorted(C_indices) != sorted(A_indices) or len(set(A_indices)) != len(A_indices):
         raise ValueError(f"permute: {C_indices!r} is not a permutation of {A_indices!r}")
+    with torch.cuda.stream(stream):
+        _permute_one(C_prefactor, C_indices, C, A_prefactor, A_indices, A)
+
+
+def _permute_one(C_prefactor, C_indices, C, A_prefactor, A_indices, A) -> None:
+    stream = None  # the current stream, set by permute()
     dC, finC = _as_device(C, write=True)
     dA, _ = _as_device(A, write=False)
     if dC.dtype != dA.dtype or dC.dtype not in _SUPPORTED:
@@ -264,6 +279,12 @@ def syev(A: torch.Tensor, W: torch.Tensor, compute_vectors: bool = True, stream=
     orthonormal eigenvectors as columns (``A[:, j]`` belongs to ``W[j]``) and W with the
     eigenvalues in ascending order. Returns the device ``info`` tensor; with ``check_info``
     it is read (a synchronisation) and a failure to converge raises, as in the reference."""
+    with torch.cuda.stream(stream):
+        return _syev_body(A, W, compute_vectors, check_info)
+
+
+def _syev_body(A, W, compute_vectors, check_info):
+    stream = None  # the current stream, set by the caller
     n = _square_check(A, "syev")
     if W.dim() != 1:
         raise RankError("syev: W must be rank 1")
@@ -298,6 +319,12 @@ def gesv(A: torch.Tensor, B: torch.Tensor, stream=None, check_info: bool = True)
     diagonal, U on and above), B (n or n x nrhs) with X. Returns ``(info, ipiv)``: ipiv is
     1-based on the device; info is an int read from the device when ``check_info``
     (0, or i > 0 when U(i,i) is exactly zero, as from LAPACK), else the device tensor."""
+    with torch.cuda.stream(stream):
+        return _gesv_body(A, B, check_info)
+
+
+def _gesv_body(A, B, check_info):
+    stream = None  # the current stream, set by the caller
     n = _square_check(A, "gesv")
     if B.dim() not in (1, 2) or B.shape[0] != n:
         raise DimensionError("gesv: B must have n rows")
@@ -321,6 +348,12 @@ def gesv(A: torch.Tensor, B: torch.Tensor, stream=None, check_info: bool = True)
 def gemm(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch.Tensor, beta: float, C: torch.Tensor,
          stream=None) -> None:
     """Row-major C = alpha*op(A) op(B) + beta*C on 2-D tensors (linear_algebra::gemm)."""
+    with torch.cuda.stream(stream):
+        return _gemm_body(transA, transB, alpha, A, B, beta, C)
+
+
+def _gemm_body(transA, transB, alpha, A, B, beta, C):
+    stream = None  # the current stream, set by the caller
     for t in (A, B, C):
         if t.dim() != 2 or t.stride(1) != 1:
             raise ValueError("gemm: 2-D row-major tensors with unit column stride required")
@@ -337,6 +370,12 @@ def gemm(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch.Tenso
 def gemm_split(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch.Tensor, beta: float, C: torch.Tensor,
                nslices: int = 5, slice_type: str = "bf16", stream=None) -> None:
     """Row-major C = alpha*op(A) op(B) + beta*C in FP64 through the split-precision GEMM."""
+    with torch.cuda.stream(stream):
+        return _gemm_split_body(transA, transB, alpha, A, B, beta, C, nslices, slice_type)
+
+
+def _gemm_split_body(transA, transB, alpha, A, B, beta, C, nslices, slice_type):
+    stream = None  # the current stream, set by the caller
     for t in (A, B, C):
         if t.dim() != 2 or t.stride(1) != 1 or t.dtype != torch.float64:
             raise ValueError("gemm_split: 2-D row-major float64 tensors with unit column stride required")
@@ -356,6 +395,12 @@ def gemm_host_c(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torc
     (pin it: ``torch.empty(..., pin_memory=True)``), streamed through the GPU in panels of
     about ``panel_bytes``; ``split=s`` runs each panel in split precision (FP64 only).
     Returns when C holds the result."""
+    with torch.cuda.stream(stream):
+        return _gemm_host_c_body(transA, transB, alpha, A, B, beta, C, panel_bytes, split, slice_type)
+
+
+def _gemm_host_c_body(transA, transB, alpha, A, B, beta, C, panel_bytes, split, slice_type):
+    stream = None  # the current stream, set by the caller
     for t in (A, B):
         if t.dim() != 2 or t.stride(1) != 1 or t.device.type != "cuda":
             raise ValueError("gemm_host_c: A and B must be 2-D row-major CUDA tensors")

@@ -1135,13 +1135,20 @@ struct GroupedWave {
     uint32_t ntiles[2] = {0, 0};
 };
 
-void free_wave(GroupedWave &w) {
+// Returns a wave's device buffers to the pool. With `stream` known (the only stream the
+// plan ran on) the blocks are reused once that stream's queued work is done; otherwise only
+// after a device synchronisation.
+void free_wave(GroupedWave &w, hipStream_t stream = nullptr, bool known = false) {
+    auto release = [&](void *p) {
+        if (known) (void)ein_free_async(p, stream);
+        else (void)ein_free(p);
+    };
     for (int v = 0; v < 2; ++v) {
-        (void)ein_free(w.d_items[v]);
-        (void)ein_free(w.d_tiles[v]);
+        release(w.d_items[v]);
+        release(w.d_tiles[v]);
         w.d_items[v] = w.d_tiles[v] = nullptr;
     }
-    (void)ein_free(w.d_tables);
+    release(w.d_tables);
     w.d_tables = nullptr;
 }
 
@@ -1153,13 +1160,21 @@ struct GroupedPlan {
     std::vector<ein_contraction_item> items; // host copy (for the autotuner)
     std::vector<int> wave_of;                // wave of each kept item
     int64_t tiles_total[2] = {0, 0};
+    std::vector<hipStream_t> streams;        // streams the plan was executed on
 };
 
-int64_t span_of(ein_tensor_desc const &d) {
-    int64_t hi = 0;
-    for (int i = 0; i < d.rank; ++i)
-        if (d.dims[i] > 0) hi += (d.dims[i] - 1) * std::llabs(d.strides[i]);
-    return hi + 1;
+// Elements a descriptor addresses, and how far below its base pointer the negative strides
+// reach (both in elements): a scratch buffer of `span` elements serves it at base + below.
+int64_t span_of(ein_tensor_desc const &d, int64_t *below = nullptr) {
+    int64_t hi = 0, lo = 0;
+    for (int i = 0; i < d.rank; ++i) {
+        if (d.dims[i] <= 0) continue;
+        int64_t const reach = (d.dims[i] - 1) * d.strides[i];
+        if (reach < 0) lo += reach;
+        else hi += reach;
+    }
+    if (below != nullptr) *below = -lo;
+    return hi - lo + 1;
 }
 
 // Byte range [lo, hi) a descriptor addresses from base p (strides of either sign).
@@ -1272,7 +1287,7 @@ int plan_create(ein_contraction_item const *items, int count, GroupedPlan &plan,
     std::vector<Placed> placed;
     size_t const eb = sizeof(T);
     for (int i = 0; i < count; ++i) {
-        GroupedItem<T> gi[2];
+        GroupedItem<T> gi[2]{};
         int kind = 0;
         bool rep = false;
         EIN_TRY(build_item<T>(items[i], gi, kind, rep));
@@ -1302,7 +1317,7 @@ int plan_create(ein_contraction_item const *items, int count, GroupedPlan &plan,
         // (k >= 2048), whose few tiles would each run k/16 slabs on one CU here while
         // stream-K splits them over the GPU. The persistent kernel takes the many small,
         // short items, for which one launch each would cost more than the work.
-        bool const big_tiles = gi[0].P.m >= 128 && gi[0].P.n >= 128 &&
+        bool const big_tiles = kind == 0 && gi[0].P.m >= 128 && gi[0].P.n >= 128 &&
                                ceil_div(gi[0].P.m, 128) * ceil_div(gi[0].P.n, 128) * gi[0].P.batch >= 16;
         if (kind == 0 && (big_tiles || gi[0].P.k >= 2048)) {
             plan.waves[w].large.push_back(items[i]);
@@ -1400,6 +1415,8 @@ int plan_launch(GroupedPlan const &plan, int bm_index, hipStream_t stream) {
 template <typename T>
 int plan_autotune(GroupedPlan &plan, hipStream_t stream) {
     if (plan.items.empty()) return EIN_SUCCESS;
+    // it allocates, clears and times scratch operands: not something a graph can hold
+    if (stream_capturing(stream)) return fail(EIN_ERROR_VALUE, "grouped plan autotune: not allowed during graph capture");
     int64_t sa = 1, sb = 1, sc = 1;
     for (auto const &it : plan.items) {
         sa = std::max(sa, span_of(it.A));
@@ -1415,9 +1432,13 @@ int plan_autotune(GroupedPlan &plan, hipStream_t stream) {
     EIN_HIP_CHECK(hipMemsetAsync(pc, 0, size_t(sc) * sizeof(T), stream));
     std::vector<ein_contraction_item> scratch = plan.items;
     for (auto &it : scratch) {
-        it.a = pa;
-        it.b = pb;
-        it.c = pc;
+        int64_t ba = 0, bb = 0, bc = 0;
+        (void)span_of(it.A, &ba);
+        (void)span_of(it.B, &bb);
+        (void)span_of(it.C, &bc);
+        it.a = static_cast<T *>(pa) + ba;
+        it.b = static_cast<T *>(pb) + bb;
+        it.c = static_cast<T *>(pc) + bc;
     }
     // the scratch items all write one buffer; keep the plan's own wave structure (the
     // aliasing only affects the values, which are discarded)
@@ -1449,10 +1470,10 @@ int plan_autotune(GroupedPlan &plan, hipStream_t stream) {
         plan.tuned    = true;
     }
     (void)hipStreamSynchronize(stream);
-    for (auto &w : tp.waves) free_wave(w);
-    (void)ein_free(pa);
-    (void)ein_free(pb);
-    (void)ein_free(pc);
+    for (auto &w : tp.waves) free_wave(w, stream, true);
+    (void)ein_free_async(pa, stream);
+    (void)ein_free_async(pb, stream);
+    (void)ein_free_async(pc, stream);
     return st;
 }
 
@@ -1479,8 +1500,9 @@ int ein_grouped_plan_create(int32_t elem_bytes, const ein_contraction_item *item
 int ein_grouped_plan_execute(void *plan, ein_stream_t stream) {
     if (plan == nullptr) return -1;
     auto *p = static_cast<ein::GroupedPlan *>(plan);
-    return p->elem == 8 ? ein::plan_launch<double>(*p, p->bm_index, ein::as_stream(stream))
-                        : ein::plan_launch<float>(*p, p->bm_index, ein::as_stream(stream));
+    hipStream_t const s = ein::as_stream(stream);
+    if (std::find(p->streams.begin(), p->streams.end(), s) == p->streams.end()) p->streams.push_back(s);
+    return p->elem == 8 ? ein::plan_launch<double>(*p, p->bm_index, s) : ein::plan_launch<float>(*p, p->bm_index, s);
 }
 
 int ein_grouped_plan_autotune(void *plan, ein_stream_t stream) {
@@ -1502,7 +1524,9 @@ int ein_grouped_plan_info(void *plan, int32_t *tile_rows, int32_t *waves, int64_
 int ein_grouped_plan_destroy(void *plan) {
     if (plan == nullptr) return EIN_SUCCESS;
     auto *p = static_cast<ein::GroupedPlan *>(plan);
-    for (auto &w : p->waves) ein::free_wave(w);
+    // kernels of the plan may still be queued: the buffers are reused only after them
+    bool const one = p->streams.size() == 1;
+    for (auto &w : p->waves) ein::free_wave(w, one ? p->streams[0] : nullptr, one);
     delete p;
     return EIN_SUCCESS;
 }

@@ -4,7 +4,8 @@
 // Reference counterparts: GPUAllocator / BlockManager (libs/Einsums/GPUMemory,
 // libs/Einsums/BlockManager/src/BlockManager.cpp:18-90) and the per-thread streams of
 // libs/Einsums/GPUStreams/src/GPUStreams.hip:55-200. Here leaves never synchronise;
-// scratch is per stream so concurrent streams never share it.
+// scratch is per stream so concurrent streams never share it. Freed pool blocks are reused
+// only once the work that may read them is known to be finished (pool_book.hpp).
 #include <map>
 #include <mutex>
 #include <string>
@@ -12,41 +13,40 @@
 #include <vector>
 
 #include "ein_common.hpp"
+#include "pool_book.hpp"
 
 namespace ein {
 
 namespace {
 thread_local std::string g_last_error;
 std::mutex g_ws_mutex;
-struct Workspace {
-    void *ptr    = nullptr;
-    size_t bytes = 0;
-};
 // slot 0 serves the kernels of one call; slot 1 holds a call's packed operands while a
 // nested call uses slot 0
-std::map<std::pair<hipStream_t, int>, Workspace> g_ws;
+std::map<std::pair<hipStream_t, int>, WorkspaceSlot> g_ws;
+// buffers superseded while a captured graph may still point at them; kept to process end
+std::vector<void *> g_ws_retired;
+// per-stream arrival counters of single-launch reductions (zeroed at creation; every
+// kernel that counts leaves them at zero)
+std::map<hipStream_t, uint32_t *> g_counters;
+constexpr size_t COUNTER_BYTES = 4096;
 
-// Size-class pool: freed blocks are kept for reuse (a stream-ordered arena would need
-// per-stream bookkeeping; callers of ein_malloc own their tensors for long periods, so
-// a simple pool keyed on rounded size is enough and keeps hipMalloc off hot paths).
 std::mutex g_pool_mutex;
-std::multimap<size_t, void *> g_pool_free;
-std::unordered_map<void *, size_t> g_pool_live;
-
-size_t round_size(size_t bytes) {
-    if (bytes < 256) return 256;
-    // round to 1/8 of the next power of two to bound internal fragmentation
-    size_t p = 1;
-    while (p < bytes) p <<= 1;
-    size_t step = p >= 8192 ? p / 8 : 256;
-    return (bytes + step - 1) / step * step;
-}
+PoolBook<hipEvent_t> g_pool;
 
 bool stream_is_capturing(hipStream_t s) {
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
     return st == hipStreamCaptureStatusActive;
 }
+
+bool event_done(hipEvent_t e) {
+    hipError_t const r = hipEventQuery(e);
+    if (r == hipSuccess) return true;
+    (void)hipGetLastError(); // hipErrorNotReady is not an error of the caller's next call
+    return false;
+}
+
+void event_destroy(hipEvent_t e) { (void)hipEventDestroy(e); }
 } // namespace
 
 void set_error(std::string const &msg) { g_last_error = msg; }
@@ -68,32 +68,55 @@ bool workspace_fits(hipStream_t stream, int slot, size_t bytes) {
 
 int workspace_slot(hipStream_t stream, int slot, size_t bytes, void **out) {
     std::lock_guard<std::mutex> lock(g_ws_mutex);
-    Workspace &w = g_ws[{stream, slot}];
-    if (w.bytes >= bytes) {
+    WorkspaceSlot &w          = g_ws[{stream, slot}];
+    bool const capturing      = stream_is_capturing(stream);
+    WorkspaceDecision const d = workspace_decide(w, bytes);
+    if (!d.grow) {
+        if (capturing) w.captured = true;
         *out = w.ptr;
         return EIN_SUCCESS;
     }
-    if (stream_is_capturing(stream)) {
+    if (capturing) {
         return fail(EIN_ERROR_WORKSPACE, "workspace of " + std::to_string(bytes) +
                                              " bytes needed during graph capture; call ein_workspace_reserve first");
     }
     if (w.ptr != nullptr) {
-        // the old buffer may still be in use by queued kernels on this stream
+        // queued kernels on this stream may still use the old buffer
         EIN_HIP_CHECK(hipStreamSynchronize(stream));
-        EIN_HIP_CHECK(hipFree(w.ptr));
-        w.ptr   = nullptr;
-        w.bytes = 0;
+        if (d.retire_old) g_ws_retired.push_back(w.ptr);
+        else EIN_HIP_CHECK(hipFree(w.ptr));
+        w.ptr      = nullptr;
+        w.bytes    = 0;
+        w.captured = false;
     }
-    size_t want = bytes < (size_t{1} << 20) ? (size_t{1} << 20) : bytes;
-    want        = (want + 4095) / 4096 * 4096;
-    void *p     = nullptr;
-    if (hipMalloc(&p, want) != hipSuccess) {
+    void *p = nullptr;
+    if (hipMalloc(&p, d.new_bytes) != hipSuccess) {
         (void)hipGetLastError();
-        return fail(EIN_ERROR_ALLOC, "workspace hipMalloc of " + std::to_string(want) + " bytes failed");
+        return fail(EIN_ERROR_ALLOC, "workspace hipMalloc of " + std::to_string(d.new_bytes) + " bytes failed");
     }
     w.ptr   = p;
-    w.bytes = want;
+    w.bytes = d.new_bytes;
     *out    = p;
+    return EIN_SUCCESS;
+}
+
+int workspace_counter(hipStream_t stream, uint32_t **out) {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    auto it = g_counters.find(stream);
+    if (it != g_counters.end()) {
+        *out = it->second;
+        return EIN_SUCCESS;
+    }
+    if (stream_is_capturing(stream))
+        return fail(EIN_ERROR_WORKSPACE, "reduction counters needed during graph capture; call ein_workspace_reserve first");
+    void *p = nullptr;
+    if (hipMalloc(&p, COUNTER_BYTES) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(EIN_ERROR_ALLOC, "counter allocation failed");
+    }
+    EIN_HIP_CHECK(hipMemset(p, 0, COUNTER_BYTES));
+    g_counters[stream] = static_cast<uint32_t *>(p);
+    *out               = static_cast<uint32_t *>(p);
     return EIN_SUCCESS;
 }
 
@@ -142,49 +165,79 @@ int ein_init(void) {
 
 int ein_malloc(void **ptr, size_t bytes) {
     if (ptr == nullptr) return -1;
-    size_t sz = ein::round_size(bytes);
-    {
-        std::lock_guard<std::mutex> lock(ein::g_pool_mutex);
-        auto it = ein::g_pool_free.lower_bound(sz);
-        if (it != ein::g_pool_free.end() && it->first <= sz + sz / 4) {
-            *ptr = it->second;
-            ein::g_pool_live[it->second] = it->first;
-            ein::g_pool_free.erase(it);
-            return EIN_SUCCESS;
+    size_t const sz = ein::pool_round_size(bytes);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        bool needs_sync = false;
+        {
+            std::lock_guard<std::mutex> lock(ein::g_pool_mutex);
+            void *p = ein::g_pool.take(sz, ein::event_done, ein::event_destroy, &needs_sync);
+            if (p != nullptr) {
+                *ptr = p;
+                return EIN_SUCCESS;
+            }
         }
+        // a block freed without a stream fits: make it safe by waiting for the device once
+        if (!needs_sync || attempt == 1 || hipDeviceSynchronize() != hipSuccess) break;
+        std::lock_guard<std::mutex> lock(ein::g_pool_mutex);
+        ein::g_pool.synchronized();
     }
     void *p = nullptr;
     if (hipMalloc(&p, sz) != hipSuccess) {
         (void)hipGetLastError();
-        // release cached blocks and retry once
-        {
+        // return every cached block to the device (after the device is idle) and retry once
+        if (hipDeviceSynchronize() == hipSuccess) {
             std::lock_guard<std::mutex> lock(ein::g_pool_mutex);
-            for (auto &kv : ein::g_pool_free) (void)hipFree(kv.second);
-            ein::g_pool_free.clear();
+            ein::g_pool.synchronized();
+            ein::g_pool.drain(ein::event_done, [](void *q, auto const &b) {
+                if (b.has_event) ein::event_destroy(b.event);
+                (void)hipFree(q);
+            });
         }
+        (void)hipGetLastError();
         if (hipMalloc(&p, sz) != hipSuccess) {
             (void)hipGetLastError();
             return ein::fail(EIN_ERROR_ALLOC, "ein_malloc of " + std::to_string(bytes) + " bytes failed");
         }
     }
     std::lock_guard<std::mutex> lock(ein::g_pool_mutex);
-    ein::g_pool_live[p] = sz;
-    *ptr                = p;
+    ein::g_pool.add_live(p, sz);
+    *ptr = p;
     return EIN_SUCCESS;
 }
 
 int ein_free(void *ptr) {
     if (ptr == nullptr) return EIN_SUCCESS;
     std::lock_guard<std::mutex> lock(ein::g_pool_mutex);
-    auto it = ein::g_pool_live.find(ptr);
-    if (it == ein::g_pool_live.end()) return ein::fail(EIN_ERROR_VALUE, "ein_free: pointer not from ein_malloc");
-    ein::g_pool_free.emplace(it->second, ptr);
-    ein::g_pool_live.erase(it);
+    if (!ein::g_pool.release(ptr, nullptr, false))
+        return ein::fail(EIN_ERROR_VALUE, "ein_free: pointer not from ein_malloc");
+    return EIN_SUCCESS;
+}
+
+int ein_free_async(void *ptr, ein_stream_t stream) {
+    if (ptr == nullptr) return EIN_SUCCESS;
+    hipStream_t const s = ein::as_stream(stream);
+    {
+        std::lock_guard<std::mutex> lock(ein::g_pool_mutex);
+        if (!ein::g_pool.is_live(ptr)) return ein::fail(EIN_ERROR_VALUE, "ein_free_async: pointer not from ein_malloc");
+    }
+    hipEvent_t ev = nullptr;
+    bool const use_event =
+        !ein::stream_is_capturing(s) && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
+        hipEventRecord(ev, s) == hipSuccess;
+    if (!use_event) {
+        (void)hipGetLastError();
+        if (ev != nullptr) (void)hipEventDestroy(ev);
+        ev = nullptr;
+    }
+    std::lock_guard<std::mutex> lock(ein::g_pool_mutex);
+    (void)ein::g_pool.release(ptr, ev, use_event);
     return EIN_SUCCESS;
 }
 
 int ein_workspace_reserve(ein_stream_t stream, size_t bytes) {
-    void *p = nullptr;
+    void *p           = nullptr;
+    uint32_t *counter = nullptr;
+    EIN_TRY(ein::workspace_counter(ein::as_stream(stream), &counter));
     return ein::workspace(ein::as_stream(stream), bytes, &p);
 }
 

@@ -65,11 +65,21 @@ typedef struct ein_tensor_desc {
 const char *ein_version(void);
 const char *ein_last_error_string(void);
 int ein_init(void);
-/* Stream-ordered pooled device memory (256-byte aligned). */
+/* Pooled device memory (256-byte aligned), replacing the per-tensor hipMalloc/hipFree of
+ * GPUAllocator / BlockManager (libs/Einsums/GPUMemory/include/Einsums/GPUMemory/
+ * GPUAllocator.hpp:53-180, libs/Einsums/BlockManager/src/BlockManager.cpp:66).
+ * ein_free returns a block to the pool without waiting; like hipFree, it is safe with work
+ * still queued: the block is handed out again only after the device has been synchronised
+ * once after the free (ein_malloc does that when it needs the block).
+ * ein_free_async is the stream-ordered form: the block is reused once the work queued on
+ * `stream` at the time of the call has completed. */
 int ein_malloc(void **ptr, size_t bytes);
 int ein_free(void *ptr);
+int ein_free_async(void *ptr, ein_stream_t stream);
 /* Pre-size the per-stream scratch used by split-K partials, reductions and
- * split-precision slices; required before hipGraph capture of large calls. */
+ * split-precision slices; required before hipGraph capture of large calls (and before
+ * capturing the first call on a stream). A scratch buffer used inside a capture is never
+ * freed when a later, larger call grows it: the captured graph keeps pointing at it. */
 int ein_workspace_reserve(ein_stream_t stream, size_t bytes);
 size_t ein_workspace_size(ein_stream_t stream);
 int ein_stream_synchronize(ein_stream_t stream);
@@ -98,7 +108,8 @@ int ein_gemv_f64(char trans, int64_t m, int64_t n, double alpha, const double *A
                  double beta, double *y, int64_t incy, ein_stream_t stream);
 int ein_gemv_f32(char trans, int64_t m, int64_t n, float alpha, const float *A, int64_t lda, const float *x, int64_t incx,
                  float beta, float *y, int64_t incy, ein_stream_t stream);
-/* result (device pointer) = beta*result + alpha*dot(x,y); deterministic two-pass tree. */
+/* result (device pointer) = beta*result + alpha*dot(x,y). One launch; the partial sums are
+ * added in a fixed order, so repeated calls give identical bits. */
 int ein_dot_f64(int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy, double beta,
                 double *result, ein_stream_t stream);
 int ein_dot_f32(int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float beta, float *result,

@@ -58,3 +58,10 @@ def test_lds_layout_program():
     fragment reads expect, and conflict-free fragment reads."""
     res = _run("test_lds_layout", 60)
     assert res.returncode == 0 and "0 failures" in res.stdout
+
+
+def test_pool_book_program():
+    """Host check of the pool's reuse conditions (stream events, device synchronisation) and
+    of the workspace growth rule (buffers used inside a graph capture are never freed)."""
+    res = _run("test_pool_book", 60)
+    assert res.returncode == 0 and "0 failures" in res.stdout
