@@ -30,8 +30,9 @@ __all__ = [
     "gemm_host_c",
 ]
 
-# GPUView modes (libs/EinsumsPy/GPU/include/EinsumsPy/GPU/PyGPUView.hpp:27-46)
-COPY, MAP, DEVICE_TENSOR = 0, 1, 2
+# GPUView modes, numbered as the reference's PyViewMode
+# (libs/EinsumsPy/GPU/include/EinsumsPy/GPU/PyGPUView.hpp:27-46)
+MAP, COPY, DEVICE_TENSOR = 0, 1, 2
 
 _SUPPORTED = (torch.float64, torch.float32)
 
@@ -51,15 +52,27 @@ def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
 
 
 class GPUView:
-    """Device mirror of a host array (reference PyGPUView, COPY and MAP modes).
+    """Device view of a buffer (reference PyGPUView, PyGPUView.hpp:27-120).
 
-    COPY: the device copy is refreshed only by ``update_H2D``/``update_D2H``.
-    MAP: host memory is kept in step automatically after every execute that writes it.
-    (The reference maps the host buffer with hipHostRegister; this backend copies,
-    because registering existing host memory is not available on every system.)
+    COPY: a device copy of a host array, refreshed only by ``update_H2D``/``update_D2H``.
+    MAP: a host array kept in step automatically after every execute that writes it.
+    DEVICE_TENSOR: a tensor already in device memory (a CUDA torch tensor), used in place;
+    the update calls do nothing. Passing a CUDA tensor selects this mode.
+    The reference maps a MAP buffer into the device address space by registering the host
+    memory (hipHostRegister); registering existing host memory is refused on the systems
+    this backend targets, so MAP keeps a device copy and writes it back after each execute.
     """
 
     def __init__(self, array, mode: int = COPY):
+        if isinstance(array, torch.Tensor) and array.is_cuda:
+            if array.dtype not in _SUPPORTED:
+                raise TypeError(f"GPUView: unsupported dtype {array.dtype}")
+            self._host, self._array, self.mode, self.tensor = None, array, DEVICE_TENSOR, array
+            return
+        if mode == DEVICE_TENSOR:
+            raise ValueError("GPUView: DEVICE_TENSOR mode views a device tensor; host buffers take COPY or MAP")
+        if mode not in (COPY, MAP):
+            raise ValueError(f"GPUView: unknown mode {mode}")
         if isinstance(array, torch.Tensor):
             host = array
         else:
@@ -72,10 +85,12 @@ class GPUView:
         self.tensor = host.to("cuda")
 
     def update_H2D(self) -> None:
-        self.tensor.copy_(self._host)
+        if self.mode != DEVICE_TENSOR:
+            self.tensor.copy_(self._host)
 
     def update_D2H(self) -> None:
-        self._host.copy_(self.tensor.cpu())
+        if self.mode != DEVICE_TENSOR:
+            self._host.copy_(self.tensor.cpu())
 
     @property
     def shape(self):

@@ -34,6 +34,11 @@ struct tensor_compat_error : std::logic_error {
 struct gpu_error : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+// A call whose arguments cannot work together (reference: einsums::bad_logic, thrown by the
+// batched einsum for lists of different lengths, Dispatch.hpp:1270).
+struct bad_logic : std::logic_error {
+    using std::logic_error::logic_error;
+};
 
 inline void check(int status, char const *what) {
     if (status == EIN_SUCCESS) return;
@@ -69,8 +74,9 @@ class DeviceTensorView;
 template <typename T, size_t R>
 class DeviceTensorBase {
   public:
-    using ValueType              = T;
-    static constexpr size_t Rank = R;
+    using ValueType                 = T;
+    static constexpr size_t Rank    = R;
+    static constexpr bool is_device = true;
 
     int64_t dim(size_t i) const { return dims_[i]; }
     int64_t stride(size_t i) const { return strides_[i]; }

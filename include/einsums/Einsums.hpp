@@ -3,4 +3,5 @@
 
 #include "DeviceTensor.hpp"
 #include "Index.hpp"
+#include "Tensor.hpp"
 #include "TensorAlgebra.hpp"

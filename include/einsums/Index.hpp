@@ -36,6 +36,11 @@ constexpr int32_t label_id(char const *s) {
     inline constexpr struct x x;                                                                                       \
     }
 
+// The reference's spelling of the macro (Detail/Index.hpp:32).
+#ifndef MAKE_INDEX
+#define MAKE_INDEX(x) EINSUMS_MAKE_INDEX(x)
+#endif
+
 EINSUMS_MAKE_INDEX(A)
 EINSUMS_MAKE_INDEX(a)
 EINSUMS_MAKE_INDEX(B)

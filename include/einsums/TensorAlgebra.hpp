@@ -1,8 +1,10 @@
 // einsum / permute on device tensors with compile-time pattern analysis.
 //
 // Reference API kept: einsum(C_prefactor, Indices{...}, &C, AB_prefactor, Indices{...}, A,
-// Indices{...}, B, &algorithm_choice) and the default-prefactor form
-// (libs/Einsums/TensorAlgebra/include/Einsums/TensorAlgebra/TensorAlgebra.hpp:48-146),
+// Indices{...}, B, &algorithm_choice), the default-prefactor form, the smart-pointer form and
+// the batched form over collections
+// (libs/Einsums/TensorAlgebra/include/Einsums/TensorAlgebra/TensorAlgebra.hpp:48-146), with
+// device tensors, host tensors (staged through device memory) or a scalar C,
 // permute(C_prefactor, Indices{...}, &C, A_prefactor, Indices{...}, A) (Permute.hpp:330-407),
 // and detail::AlgorithmChoice (Detail/Utilities.hpp:26-34).
 //
@@ -19,6 +21,7 @@
 
 #include "DeviceTensor.hpp"
 #include "Index.hpp"
+#include "Tensor.hpp"
 
 namespace einsums::tensor_algebra {
 
@@ -150,13 +153,64 @@ constexpr std::array<int32_t, sizeof...(Ls) == 0 ? 1 : sizeof...(Ls)> label_ids(
 }
 
 template <typename T>
-concept DeviceTensorConcept = requires(T const &t) {
+concept StridedTensor = requires(T const &t) {
     typename T::ValueType;
     { T::Rank } -> std::convertible_to<size_t>;
+    { T::is_device } -> std::convertible_to<bool>;
     t.data();
     t.dims();
     t.strides();
 };
+// Tensors whose memory is on the GPU (DeviceTensor, DeviceTensorView, block views).
+template <typename T>
+concept DeviceTensorConcept = StridedTensor<T> && T::is_device;
+// Host tensors (Tensor, TensorView): their einsums stage through device memory.
+template <typename T>
+concept HostTensorConcept = StridedTensor<T> && !T::is_device;
+
+// Smart pointers to tensors (reference concept SmartPointer, Concepts/SmartPointer.hpp).
+template <typename T>
+concept SmartPointer = requires(T const &p) {
+    typename T::element_type;
+    { p.get() } -> std::convertible_to<typename T::element_type *>;
+};
+// Collections of tensors for the batched einsum (reference concept Container).
+template <typename T>
+concept Container = !StridedTensor<T> && !SmartPointer<T> && requires(T const &c) {
+    typename T::value_type;
+    { c.size() } -> std::convertible_to<size_t>;
+    c.at(size_t{0});
+};
+
+template <typename T>
+struct value_of {
+    using type = T; // scalars
+};
+template <StridedTensor T>
+struct value_of<T> {
+    using type = typename T::ValueType;
+};
+template <SmartPointer T>
+struct value_of<T> {
+    using type = typename value_of<typename T::element_type>::type;
+};
+template <Container T>
+struct value_of<T> {
+    using type = typename value_of<typename T::value_type>::type;
+};
+template <typename T>
+using value_of_t = typename value_of<std::remove_cvref_t<T>>::type;
+
+template <typename T>
+decltype(auto) deref(T const &t) {
+    if constexpr (SmartPointer<T>) return *t;
+    else return (t);
+}
+template <typename T>
+auto deref_c(T *p) {
+    if constexpr (SmartPointer<T>) return p->get();
+    else return p;
+}
 
 } // namespace detail
 
@@ -195,14 +249,87 @@ void einsum(U const C_prefactor, std::tuple<CIndices...> const &C_indices, CType
     if (algorithm_choice != nullptr) *algorithm_choice = detail::AlgorithmChoice(used);
 }
 
-// Default prefactors: C = A*B (C_prefactor 0, AB_prefactor 1).
-template <bool ConjA = false, bool ConjB = false, detail::DeviceTensorConcept CType, detail::DeviceTensorConcept AType,
-          detail::DeviceTensorConcept BType, typename... CIndices, typename... AIndices, typename... BIndices>
+// Host tensors (Tensor / TensorView): the operands' memory spans go to device memory, the
+// device path runs, and C comes back; the call returns with C updated on the host.
+template <bool ConjA = false, bool ConjB = false, detail::HostTensorConcept CType, detail::HostTensorConcept AType,
+          detail::HostTensorConcept BType, typename U, typename... CIndices, typename... AIndices, typename... BIndices>
+void einsum(U const C_prefactor, std::tuple<CIndices...> const &C_indices, CType *C, U const AB_prefactor,
+            std::tuple<AIndices...> const &A_indices, AType const &A, std::tuple<BIndices...> const &B_indices,
+            BType const &B, detail::AlgorithmChoice *algorithm_choice = nullptr) {
+    using T = typename CType::ValueType;
+    ::einsums::detail::StagedView<T, CType::Rank> c(*C, C_prefactor != U(0));
+    ::einsums::detail::StagedView<typename AType::ValueType, AType::Rank> a(A, true);
+    ::einsums::detail::StagedView<typename BType::ValueType, BType::Rank> b(B, true);
+    einsum<ConjA, ConjB>(C_prefactor, C_indices, &c.view, AB_prefactor, A_indices, a.view, B_indices, b.view,
+                         algorithm_choice);
+    c.download();
+}
+
+// Scalar C (a plain T*, as the reference accepts for a rank-0 target, Dispatch.hpp:900-920):
+// C = C_prefactor * C + AB_prefactor * (full contraction of A and B). Reads the result back
+// (a synchronisation of the calling thread's stream).
+template <bool ConjA = false, bool ConjB = false, typename CType, typename AType, typename BType, typename U,
+          typename... AIndices, typename... BIndices>
+    requires(std::is_arithmetic_v<CType> &&
+             ((detail::DeviceTensorConcept<AType> && detail::DeviceTensorConcept<BType>) ||
+              (detail::HostTensorConcept<AType> && detail::HostTensorConcept<BType>)))
+void einsum(U const C_prefactor, std::tuple<> const &C_indices, CType *C, U const AB_prefactor,
+            std::tuple<AIndices...> const &A_indices, AType const &A, std::tuple<BIndices...> const &B_indices,
+            BType const &B, detail::AlgorithmChoice *algorithm_choice = nullptr) {
+    if constexpr (detail::DeviceTensorConcept<AType>) {
+        DeviceTensor<CType, 0> c("scalar C");
+        if (C_prefactor != U(0)) c = *C;
+        einsum<ConjA, ConjB>(C_prefactor, C_indices, &c, AB_prefactor, A_indices, A, B_indices, B, algorithm_choice);
+        *C = CType(c);
+    } else {
+        Tensor<CType, 0> c("scalar C");
+        c = *C;
+        einsum<ConjA, ConjB>(C_prefactor, C_indices, &c, AB_prefactor, A_indices, A, B_indices, B, algorithm_choice);
+        *C = CType(c);
+    }
+}
+
+// Smart pointers to any of the operands (TensorAlgebra.hpp:116-132): dereferenced, then the
+// matching overload above runs. C is passed as a pointer to its smart pointer.
+template <bool ConjA = false, bool ConjB = false, typename AType, typename BType, typename CType, typename U,
+          typename... CIndices, typename... AIndices, typename... BIndices>
+    requires(detail::SmartPointer<AType> || detail::SmartPointer<BType> || detail::SmartPointer<CType>)
+void einsum(U const C_prefactor, std::tuple<CIndices...> const &C_indices, CType *C, U const AB_prefactor,
+            std::tuple<AIndices...> const &A_indices, AType const &A, std::tuple<BIndices...> const &B_indices,
+            BType const &B, detail::AlgorithmChoice *algorithm_choice = nullptr) {
+    einsum<ConjA, ConjB>(C_prefactor, C_indices, detail::deref_c(C), AB_prefactor, A_indices, detail::deref(A), B_indices,
+                         detail::deref(B), algorithm_choice);
+}
+
+// Batched einsum over equally long collections (TensorAlgebra.hpp:57, Dispatch.hpp:1264-1286):
+// entry i computes C[i] from A[i] and B[i]; the calls are queued back to back on the
+// stream. *algorithm_choice (when given) is the first entry's.
+template <bool ConjA = false, bool ConjB = false, detail::Container CType, detail::Container AType,
+          detail::Container BType, typename CPrefactorType, typename ABPrefactorType, typename... CIndices,
+          typename... AIndices, typename... BIndices>
+void einsum(CPrefactorType const C_prefactor, std::tuple<CIndices...> const &C_indices, CType *C_list,
+            ABPrefactorType const AB_prefactor, std::tuple<AIndices...> const &A_indices, AType const &A_list,
+            std::tuple<BIndices...> const &B_indices, BType const &B_list,
+            detail::AlgorithmChoice *algorithm_choice = nullptr) {
+    if (C_list->size() != A_list.size() || C_list->size() != B_list.size())
+        throw bad_logic("Lists passed to batched einsum call do not have the same size!");
+    using T = detail::value_of_t<CType>;
+    for (size_t i = 0; i < C_list->size(); ++i) {
+        detail::AlgorithmChoice used = detail::INDETERMINATE;
+        einsum<ConjA, ConjB>(T(C_prefactor), C_indices, &(C_list->at(i)), T(AB_prefactor), A_indices, A_list.at(i),
+                             B_indices, B_list.at(i), &used);
+        if (i == 0 && algorithm_choice != nullptr) *algorithm_choice = used;
+    }
+}
+
+// Default prefactors: C = A*B (C_prefactor 0, AB_prefactor 1), for every operand kind above.
+template <bool ConjA = false, bool ConjB = false, typename CType, typename AType, typename BType, typename... CIndices,
+          typename... AIndices, typename... BIndices>
 void einsum(std::tuple<CIndices...> const &C_indices, CType *C, std::tuple<AIndices...> const &A_indices, AType const &A,
             std::tuple<BIndices...> const &B_indices, BType const &B,
             detail::AlgorithmChoice *algorithm_choice = nullptr) {
-    einsum<ConjA, ConjB>(typename CType::ValueType(0), C_indices, C, typename CType::ValueType(1), A_indices, A, B_indices, B,
-                         algorithm_choice);
+    using T = detail::value_of_t<CType>;
+    einsum<ConjA, ConjB>(T(0), C_indices, C, T(1), A_indices, A, B_indices, B, algorithm_choice);
 }
 
 // C = C_prefactor*C + A_prefactor*perm(A); the permutation is resolved at compile time.
@@ -244,7 +371,18 @@ void permute(U const C_prefactor, std::tuple<CIndices...> const &, CType *C, U c
     check(st, "permute");
 }
 
-template <detail::DeviceTensorConcept CType, detail::DeviceTensorConcept AType, typename... CIndices, typename... AIndices>
+// Host tensors: staged through device memory like the host einsum.
+template <detail::HostTensorConcept CType, detail::HostTensorConcept AType, typename U, typename... CIndices,
+          typename... AIndices>
+void permute(U const C_prefactor, std::tuple<CIndices...> const &Ci, CType *C, U const A_prefactor,
+             std::tuple<AIndices...> const &Ai, AType const &A) {
+    ::einsums::detail::StagedView<typename CType::ValueType, CType::Rank> c(*C, C_prefactor != U(0));
+    ::einsums::detail::StagedView<typename AType::ValueType, AType::Rank> a(A, true);
+    permute(C_prefactor, Ci, &c.view, A_prefactor, Ai, a.view);
+    c.download();
+}
+
+template <detail::StridedTensor CType, detail::StridedTensor AType, typename... CIndices, typename... AIndices>
 void permute(std::tuple<CIndices...> const &Ci, CType *C, std::tuple<AIndices...> const &Ai, AType const &A) {
     permute(typename CType::ValueType(0), Ci, C, typename CType::ValueType(1), Ai, A);
 }

@@ -2,6 +2,8 @@
 // reference tests assert (libs/Einsums/TensorAlgebra/tests/unit/Einsum*.cpp, Dot.cpp).
 // Compiles only if every static_assert holds; runs without a GPU.
 #include <cstdio>
+#include <memory>
+#include <vector>
 
 #include "einsums/Einsums.hpp"
 
@@ -38,6 +40,17 @@ CHECK((m, b, e, j), (j, n, f, b), (m, n, e, f), GENERIC); // CCD Wmbej
 CHECK((p, q), (p, q, r, s), (r, s), GEMV);            // HF J
 CHECK((p, q), (p, r, q, s), (r, s), GENERIC);         // HF K
 CHECK((i), (i, i), (i), GENERIC);                     // Hadamard
+
+// Operand kinds of the overload set (TensorAlgebra.hpp:48-146 of the reference): device and
+// host tensors are told apart, smart pointers and collections have their own overloads.
+static_assert(ta::DeviceTensorConcept<DeviceTensor<double, 2>> && ta::DeviceTensorConcept<DeviceTensorView<float, 3>>);
+static_assert(ta::HostTensorConcept<Tensor<double, 2>> && ta::HostTensorConcept<TensorView<double, 4>>);
+static_assert(!ta::DeviceTensorConcept<Tensor<double, 2>> && !ta::HostTensorConcept<DeviceTensor<double, 2>>);
+static_assert(ta::SmartPointer<std::shared_ptr<Tensor<double, 2>>> && !ta::SmartPointer<Tensor<double, 2>>);
+static_assert(ta::Container<std::vector<DeviceTensor<double, 2>>> && !ta::Container<DeviceTensor<double, 2>>);
+static_assert(std::is_same_v<ta::value_of_t<std::vector<Tensor<float, 2>>>, float>);
+static_assert(std::is_same_v<ta::value_of_t<std::shared_ptr<DeviceTensor<double, 1>>>, double>);
+static_assert(std::is_same_v<ta::value_of_t<double>, double>);
 
 int main() {
     std::printf("compile-time dispatch checks passed\n");

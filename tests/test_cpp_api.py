@@ -65,3 +65,12 @@ def test_pool_book_program():
     of the workspace growth rule (buffers used inside a graph capture are never freed)."""
     res = _run("test_pool_book", 60)
     assert res.returncode == 0 and "0 failures" in res.stdout
+
+
+@pytest.mark.gpu
+def test_api_overloads_program():
+    """The reference's einsum spellings (MAKE_INDEX, host Tensor/TensorView, scalar C,
+    shared_ptr operands, batched collections, host permute) against the CPU oracle."""
+    res = _run("test_api_overloads", 120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "PASS test_api_overloads" in res.stdout
