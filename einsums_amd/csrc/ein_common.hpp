@@ -46,9 +46,6 @@ int workspace(hipStream_t stream, size_t bytes, void **out);
 // Same for a numbered buffer of the stream (slot 0 is the one above); a call that keeps
 // data in slot 1 can run nested calls that use slot 0.
 int workspace_slot(hipStream_t stream, int slot, size_t bytes, void **out);
-// Per-stream arrival counters (1024 x uint32, zero between calls) for single-launch
-// reductions whose last workgroup finishes the sum; created outside graph capture.
-int workspace_counter(hipStream_t stream, uint32_t **out);
 // Whether `stream` is being captured into a graph.
 bool stream_capturing(hipStream_t stream);
 // Whether slot `slot` of `stream` already holds `bytes` (no allocation needed).

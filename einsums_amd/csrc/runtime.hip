@@ -25,10 +25,6 @@ std::mutex g_ws_mutex;
 std::map<std::pair<hipStream_t, int>, WorkspaceSlot> g_ws;
 // buffers superseded while a captured graph may still point at them; kept to process end
 std::vector<void *> g_ws_retired;
-// per-stream arrival counters of single-launch reductions (zeroed at creation; every
-// kernel that counts leaves them at zero)
-std::map<hipStream_t, uint32_t *> g_counters;
-constexpr size_t COUNTER_BYTES = 4096;
 
 std::mutex g_pool_mutex;
 PoolBook<hipEvent_t> g_pool;
@@ -97,26 +93,6 @@ int workspace_slot(hipStream_t stream, int slot, size_t bytes, void **out) {
     w.ptr   = p;
     w.bytes = d.new_bytes;
     *out    = p;
-    return EIN_SUCCESS;
-}
-
-int workspace_counter(hipStream_t stream, uint32_t **out) {
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    auto it = g_counters.find(stream);
-    if (it != g_counters.end()) {
-        *out = it->second;
-        return EIN_SUCCESS;
-    }
-    if (stream_is_capturing(stream))
-        return fail(EIN_ERROR_WORKSPACE, "reduction counters needed during graph capture; call ein_workspace_reserve first");
-    void *p = nullptr;
-    if (hipMalloc(&p, COUNTER_BYTES) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(EIN_ERROR_ALLOC, "counter allocation failed");
-    }
-    EIN_HIP_CHECK(hipMemset(p, 0, COUNTER_BYTES));
-    g_counters[stream] = static_cast<uint32_t *>(p);
-    *out               = static_cast<uint32_t *>(p);
     return EIN_SUCCESS;
 }
 
@@ -235,9 +211,7 @@ int ein_free_async(void *ptr, ein_stream_t stream) {
 }
 
 int ein_workspace_reserve(ein_stream_t stream, size_t bytes) {
-    void *p           = nullptr;
-    uint32_t *counter = nullptr;
-    EIN_TRY(ein::workspace_counter(ein::as_stream(stream), &counter));
+    void *p = nullptr;
     return ein::workspace(ein::as_stream(stream), bytes, &p);
 }
 

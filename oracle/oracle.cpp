@@ -19,8 +19,9 @@
 //                              generic loop otherwise
 //   oracle_permute_*         - permute (Permute.hpp:330-407): C = beta*C + alpha*perm(A)
 //   oracle_gemm/gemv/dot/ger - column-major BLAS semantics of blas::gemm etc. (BLAS.hpp:156)
-// Host BLAS for GEMM is resolved at run time (MKL, then numpy's OpenBLAS ILP64, then a
-// blocked loop), as the reference links whichever vendor BLAS it finds.
+// Host BLAS for GEMM is resolved at run time (a system OpenBLAS/BLAS, numpy's OpenBLAS ILP64,
+// MKL, then a blocked loop), as the reference links whichever vendor BLAS it finds; a library
+// is used only after it reproduces the blocked loop on a small case (blas_agrees).
 #include <dlfcn.h>
 #include <omp.h>
 
@@ -137,43 +138,95 @@ using dgemm_ilp64_t = void (*)(char const *, char const *, int64_t const *, int6
 dgemm_lp64_t g_dgemm_lp64   = nullptr;
 dgemm_ilp64_t g_dgemm_ilp64 = nullptr;
 std::string g_blas_name     = "builtin blocked loop";
+std::string g_rejected; // libraries that loaded but failed the self-check
 bool g_blas_probed          = false;
+
+// The blocked loop: per (i,j) a sequential sum over p, over arbitrary 2-D strides.
+template <typename T>
+void gemm_loop(int64_t m, int64_t n, int64_t k, T alpha, T const *A, int64_t a_sm, int64_t a_sk, T const *B,
+               int64_t b_sk, int64_t b_sn, T beta, T *C, int64_t c_sm, int64_t c_sn) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t j0 = 0; j0 < n; j0 += 64) {
+        for (int64_t i0 = 0; i0 < m; i0 += 64) {
+            int64_t i1 = std::min(m, i0 + 64), j1 = std::min(n, j0 + 64);
+            for (int64_t j = j0; j < j1; ++j)
+                for (int64_t i = i0; i < i1; ++i) {
+                    T s = T(0);
+                    for (int64_t p = 0; p < k; ++p) s += A[i * a_sm + p * a_sk] * B[p * b_sk + j * b_sn];
+                    T *cp = C + i * c_sm + j * c_sn;
+                    *cp = beta == T(0) ? alpha * s : alpha * s + beta * *cp;
+                }
+        }
+    }
+}
+
+// A dlopen'ed dgemm is trusted only after it reproduces the blocked loop on a small case in
+// all four transpose combinations (padded leading dimensions, alpha and beta not 1). A
+// library that loads but computes something else in this process (seen with MKL's runtime
+// dispatch in a process that also holds the HIP runtime) is rejected and the next one tried.
+bool blas_agrees(dgemm_lp64_t f32i, dgemm_ilp64_t f64i) {
+    int64_t const m = 37, n = 29, k = 41, ld = 53;
+    std::vector<double> A(size_t(ld) * 64), B(size_t(ld) * 64), C0(size_t(ld) * n), C(C0.size()), R(C0.size());
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    auto next = [&] {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        return double(int64_t(x % 2001) - 1000) / 1000.0;
+    };
+    for (auto &v : A) v = next();
+    for (auto &v : B) v = next();
+    for (auto &v : C0) v = next();
+    for (int t = 0; t < 4; ++t) {
+        char ta = (t & 1) ? 'T' : 'N', tb = (t & 2) ? 'T' : 'N';
+        double const alpha = 0.75, beta = -1.25;
+        C = C0;
+        R = C0;
+        if (f32i) {
+            int im = int(m), in = int(n), ik = int(k), il = int(ld);
+            f32i(&ta, &tb, &im, &in, &ik, &alpha, A.data(), &il, B.data(), &il, &beta, C.data(), &il);
+        } else {
+            int64_t im = m, in = n, ik = k, il = ld;
+            f64i(&ta, &tb, &im, &in, &ik, &alpha, A.data(), &il, B.data(), &il, &beta, C.data(), &il);
+        }
+        // column-major: op(A)(i,p), op(B)(p,j), C(i,j) = C[i + j*ld]
+        int64_t a_sm = ta == 'N' ? 1 : ld, a_sk = ta == 'N' ? ld : 1;
+        int64_t b_sk = tb == 'N' ? 1 : ld, b_sn = tb == 'N' ? ld : 1;
+        gemm_loop<double>(m, n, k, alpha, A.data(), a_sm, a_sk, B.data(), b_sk, b_sn, beta, R.data(), 1, ld);
+        double num = 0, den = 0;
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < m; ++i) {
+                double d = C[size_t(i + j * ld)] - R[size_t(i + j * ld)];
+                num += d * d;
+                den += R[size_t(i + j * ld)] * R[size_t(i + j * ld)];
+            }
+        if (!(num <= 1e-26 * den)) return false;
+    }
+    return true;
+}
 
 void probe_blas() {
     if (g_blas_probed) return;
     g_blas_probed = true;
     if (std::getenv("ORACLE_NO_BLAS")) return;
-    char const *mkl_paths[] = {"libopenblas.so.0", "libopenblas.so", "libblas.so.3"};
-    for (char const *p : mkl_paths) {
-        void *h = dlopen(p, RTLD_NOW | RTLD_LOCAL);
-        if (h) {
-            g_dgemm_lp64 = reinterpret_cast<dgemm_lp64_t>(dlsym(h, "dgemm_"));
-            if (g_dgemm_lp64) {
-                g_blas_name = p;
-                return;
-            }
-        }
-    }
-    if (char const *np = std::getenv("ORACLE_OPENBLAS64")) {
-        void *h = dlopen(np, RTLD_NOW | RTLD_LOCAL);
-        if (h) {
-            g_dgemm_ilp64 = reinterpret_cast<dgemm_ilp64_t>(dlsym(h, "scipy_dgemm_64_"));
-            if (g_dgemm_ilp64) {
-                g_blas_name = np;
-                return;
-            }
-        }
-    }
+    std::vector<std::pair<std::string, bool>> cands; // (path, ILP64 scipy-prefixed symbols)
+    for (char const *p : {"libopenblas.so.0", "libopenblas.so", "libblas.so.3"}) cands.push_back({p, false});
+    if (char const *np = std::getenv("ORACLE_OPENBLAS64")) cands.push_back({np, true});
     std::string conda = std::getenv("CONDA_PREFIX") ? std::getenv("CONDA_PREFIX") : "/opt/conda";
-    for (std::string p : {conda + "/lib/libmkl_rt.so", std::string("/opt/conda/lib/libmkl_rt.so")}) {
-        void *h = dlopen(p.c_str(), RTLD_NOW | RTLD_LOCAL);
-        if (h) {
-            g_dgemm_lp64 = reinterpret_cast<dgemm_lp64_t>(dlsym(h, "dgemm_"));
-            if (g_dgemm_lp64) {
-                g_blas_name = p;
-                return;
-            }
+    cands.push_back({conda + "/lib/libmkl_rt.so", false});
+    cands.push_back({"/opt/conda/lib/libmkl_rt.so", false});
+    for (auto const &c : cands) {
+        void *h = dlopen(c.first.c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!h) continue;
+        dgemm_lp64_t f32i  = c.second ? nullptr : reinterpret_cast<dgemm_lp64_t>(dlsym(h, "dgemm_"));
+        dgemm_ilp64_t f64i = c.second ? reinterpret_cast<dgemm_ilp64_t>(dlsym(h, "scipy_dgemm_64_")) : nullptr;
+        if ((f32i || f64i) && blas_agrees(f32i, f64i)) {
+            g_dgemm_lp64  = f32i;
+            g_dgemm_ilp64 = f64i;
+            g_blas_name   = c.first;
+            return;
         }
+        g_rejected += (g_rejected.empty() ? "" : ", ") + c.first;
     }
 }
 
@@ -222,20 +275,7 @@ void gemm_strided(int64_t m, int64_t n, int64_t k, T alpha, T const *A, int64_t 
             }
         }
     }
-    // blocked loop: per (i,j) a sequential sum over p
-#pragma omp parallel for collapse(2) schedule(static)
-    for (int64_t j0 = 0; j0 < n; j0 += 64) {
-        for (int64_t i0 = 0; i0 < m; i0 += 64) {
-            int64_t i1 = std::min(m, i0 + 64), j1 = std::min(n, j0 + 64);
-            for (int64_t j = j0; j < j1; ++j)
-                for (int64_t i = i0; i < i1; ++i) {
-                    T s = T(0);
-                    for (int64_t p = 0; p < k; ++p) s += A[i * a_sm + p * a_sk] * B[p * b_sk + j * b_sn];
-                    T *cp = C + i * c_sm + j * c_sn;
-                    *cp = beta == T(0) ? alpha * s : alpha * s + beta * *cp;
-                }
-        }
-    }
+    gemm_loop<T>(m, n, k, alpha, A, a_sm, a_sk, B, b_sk, b_sn, beta, C, c_sm, c_sn);
 }
 
 // ------------------------------------------------------------ generic loop
@@ -508,7 +548,9 @@ extern "C" {
 
 const char *oracle_blas_name() {
     probe_blas();
-    return g_blas_name.c_str();
+    static std::string full;
+    full = g_blas_name + (g_rejected.empty() ? "" : " (rejected by the self-check: " + g_rejected + ")");
+    return full.c_str();
 }
 
 int oracle_set_threads(int n) {

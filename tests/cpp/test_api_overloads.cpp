@@ -18,6 +18,7 @@ MAKE_INDEX(ll)
 extern "C" {
 int oracle_einsum_f64(double cp, ein_tensor_desc const *C, double *c, double abp, ein_tensor_desc const *A,
                       double const *a, ein_tensor_desc const *B, double const *b, int *alg);
+char const *oracle_blas_name();
 int oracle_permute_f64(int rank, int64_t const *dims_A, int const *perm, double alpha, double const *A,
                        int64_t const *strides_A, double beta, double *C, int64_t const *strides_C);
 }
@@ -233,6 +234,7 @@ int main() {
     smart_pointers();
     batched();
     host_permute();
+    std::printf("checker host BLAS: %s\n", oracle_blas_name());
     if (failures == 0) std::printf("PASS test_api_overloads\n");
     return failures == 0 ? 0 : 1;
 }

@@ -30,7 +30,7 @@ def test_graph_survives_workspace_growth(core):
 
     def step():
         core.einsum("ij", C, "ik", A, "kj", B, stream=s)  # GEMM (stream-K partials in the workspace)
-        core.einsum("", E, "i", x, "i", y, stream=s)      # DOT (partials + arrival counter)
+        core.einsum("", E, "i", x, "i", y, stream=s)      # DOT (partials in the workspace)
 
     torch.cuda.synchronize()
     with torch.cuda.stream(s):
@@ -39,12 +39,16 @@ def test_graph_survives_workspace_growth(core):
     C_eager, E_eager = C.clone(), E.clone()
     ws_before = int(core.lib.ein_workspace_size(core.stream_handle(s)))
     g = blocks.Graph(step, s)
-    # a larger eager call on the same stream grows its workspace (split slices)
+    # eager calls on the same stream that grow its workspace: an explicit reservation past
+    # the current size (torch hands out pooled streams, so this one may already hold a large
+    # buffer from earlier work), then a split GEMM that uses the grown buffer
+    h = core.stream_handle(s)
+    assert core.lib.ein_workspace_reserve(h, 2 * ws_before + (1 << 20)) == 0
+    assert int(core.lib.ein_workspace_size(h)) > ws_before
     big = dev(rng.uniform(-1, 1, (2048, 2048)))
     out = torch.empty_like(big)
     core.gemm_split("N", "N", 1.0, big, big, 0.0, out, nslices=5, slice_type="bf16", stream=s)
     s.synchronize()
-    assert int(core.lib.ein_workspace_size(core.stream_handle(s))) > ws_before
     # fresh device allocations written with garbage, then replays
     junk = [torch.full((1 << 20,), float("nan"), dtype=torch.float64, device="cuda") for _ in range(8)]
     for _ in range(3):
