@@ -108,8 +108,9 @@ int ein_gemv_f64(char trans, int64_t m, int64_t n, double alpha, const double *A
                  double beta, double *y, int64_t incy, ein_stream_t stream);
 int ein_gemv_f32(char trans, int64_t m, int64_t n, float alpha, const float *A, int64_t lda, const float *x, int64_t incx,
                  float beta, float *y, int64_t incy, ein_stream_t stream);
-/* result (device pointer) = beta*result + alpha*dot(x,y). One launch; the partial sums are
- * added in a fixed order, so repeated calls give identical bits. */
+/* result (device pointer) = beta*result + alpha*dot(x,y). Partial sums per workgroup, then a
+ * second launch adds them in a fixed order, so repeated calls give identical bits.
+ * Replaces blas::gpu::dot (hipBLAS.hpp:546). */
 int ein_dot_f64(int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy, double beta,
                 double *result, ein_stream_t stream);
 int ein_dot_f32(int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float beta, float *result,
