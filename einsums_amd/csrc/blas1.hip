@@ -321,6 +321,57 @@ int dot_impl(int64_t n, T alpha, T const *x, int64_t incx, T const *y, int64_t i
     return dot_launch<T>(n, alpha, x, incx, y, incy, beta, result, static_cast<T *>(ws), parts, stream);
 }
 
+// ------------------------------------------------------- allocation-free dot
+// ein_dot_f64_noalloc keeps its partial sums in this file-scope device array instead of
+// the per-stream workspace, so it never allocates. Every GPU holds its own copy of the
+// array at its own address; host code must use the address hipGetSymbolAddress returns
+// for the current device (the array's name is not a device address on the host). Calls
+// on one GPU share the array: calls that overlap in time on one GPU race. Nothing in it
+// needs to start at zero (each call writes every partial it reads).
+static __device__ __align__(256) double scratch[EIN_DOT_NOALLOC_MAX_PARTS];
+
+namespace {
+std::mutex g_scratch_mutex;
+constexpr int MAX_DEVICES = 64;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+} // namespace
+
+// The current device's address of `scratch` (looked up once per device).
+int noalloc_scratch(double **out) {
+    int dev = 0;
+    EIN_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(EIN_ERROR_UNSUPPORTED, "dot_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            (void)hipGetLastError();
+            return fail(EIN_ERROR_HIP, std::string("dot_noalloc: hipGetSymbolAddress of the scratch array failed: ") +
+                                           hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return EIN_SUCCESS;
+}
+
+// ein_dot_f64 with the partials in `scratch`: same kernels, so the same bits.
+int dot_noalloc_f64(int64_t n, double alpha, double const *x, int64_t incx, double const *y, int64_t incy, double beta,
+                    double *result, hipStream_t stream) {
+    EIN_TRY(dot_check(n, result, incx, incy));
+    int64_t const parts = dot_parts(n, incx, incy);
+    if (parts > EIN_DOT_NOALLOC_MAX_PARTS)
+        return fail(EIN_ERROR_UNSUPPORTED, "dot_noalloc: n = " + std::to_string(n) + " needs " + std::to_string(parts) +
+                                               " partial sums, more than the " +
+                                               std::to_string(EIN_DOT_NOALLOC_MAX_PARTS) + " of its static scratch");
+    double *partial = nullptr;
+    EIN_TRY(noalloc_scratch(&partial));
+    if (n > 0 && incx < 0) x = x - (n - 1) * incx;
+    if (n > 0 && incy < 0) y = y - (n - 1) * incy;
+    return dot_launch<double>(n, alpha, x, incx, y, incy, beta, result, partial, parts, stream);
+}
+
 // -------------------------------------------------------------------- ger
 template <typename T>
 __global__ __launch_bounds__(256) void ger_kernel(int64_t m, int64_t n, T alpha, T const *__restrict__ x, int64_t incx,
@@ -515,6 +566,10 @@ int ein_dot_f64(int64_t n, double alpha, const double *x, int64_t incx, const do
 int ein_dot_f32(int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float beta, float *result,
                 ein_stream_t stream) {
     return ein::dot_impl<float>(n, alpha, x, incx, y, incy, beta, result, ein::as_stream(stream));
+}
+int ein_dot_f64_noalloc(int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy,
+                        double beta, double *result, ein_stream_t stream) {
+    return ein::dot_noalloc_f64(n, alpha, x, incx, y, incy, beta, result, ein::as_stream(stream));
 }
 int ein_ger_f64(int64_t m, int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy, double *A,
                 int64_t lda, ein_stream_t stream) {

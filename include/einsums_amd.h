@@ -115,6 +115,18 @@ int ein_dot_f64(int64_t n, double alpha, const double *x, int64_t incx, const do
                 double *result, ein_stream_t stream);
 int ein_dot_f32(int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float beta, float *result,
                 ein_stream_t stream);
+/* ein_dot_f64 for callers that cannot allocate device memory (code under hipGraph capture,
+ * serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. Same
+ * arguments, kernels and bits as ein_dot_f64; the partial sums live in a static device
+ * array of EIN_DOT_NOALLOC_MAX_PARTS doubles (512 KiB) that each GPU the library runs on
+ * holds until the process ends, so the call allocates nothing. Limit: unit strides up to
+ * EIN_DOT_NOALLOC_MAX_N elements (longer: EIN_ERROR_UNSUPPORTED, nothing launched); other
+ * strides any n. All calls on one GPU share that array: calls that overlap in time on one
+ * GPU race (serialize them, e.g. on one stream). */
+#define EIN_DOT_NOALLOC_MAX_PARTS 65536
+#define EIN_DOT_NOALLOC_MAX_N (65536LL * 8192LL)
+int ein_dot_f64_noalloc(int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy,
+                        double beta, double *result, ein_stream_t stream);
 int ein_ger_f64(int64_t m, int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy, double *A,
                 int64_t lda, ein_stream_t stream);
 int ein_ger_f32(int64_t m, int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy, float *A,

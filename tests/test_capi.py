@@ -123,3 +123,18 @@ def test_contraction_route_setter_without_gpu(core):
     with pytest.raises(ValueError):
         with core.contraction_route("fast"):
             pass
+
+
+def test_noalloc_dot_limit_without_gpu():
+    """ein_dot_f64_noalloc rejects a unit-stride length whose partial sums would not fit its
+    static scratch on the host, before any device call (so this runs without a GPU)."""
+    from einsums_amd import _lib
+
+    f = _lib.lib.ein_dot_f64_noalloc
+    max_n = 65536 * 8192  # EIN_DOT_NOALLOC_MAX_N
+    fake = 4096  # never dereferenced: the call returns before any launch
+    assert f(max_n + 1, 1.0, fake, 1, fake, 1, 0.0, fake, None) == 1005  # EIN_ERROR_UNSUPPORTED
+    assert b"static scratch" in _lib.lib.ein_last_error_string()
+    assert f(-1, 1.0, fake, 1, fake, 1, 0.0, fake, None) == -1
+    assert f(10, 1.0, fake, 1, fake, 1, 0.0, None, None) == -8
+
