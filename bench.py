@@ -104,7 +104,7 @@ def host_cpu_info():
     return cores, model
 
 
-def cpu_leg(I, o, v, flops, large, budget_s=20.0, max_reps=20, warmups=3):
+def cpu_leg(I, o, v, flops, large, hf=None, budget_s=20.0, max_reps=20, warmups=3):
     """The bench's CPU leg: everything here runs the oracle (the restated reference CPU
     einsum, oracle/), which the GPU legs never touch.
 
@@ -114,7 +114,8 @@ def cpu_leg(I, o, v, flops, large, budget_s=20.0, max_reps=20, warmups=3):
        until `max_reps` or `budget_s` seconds of timed work, whichever comes first.
        Reports GFLOP/s of the median step, with the minimum alongside.
     2. Checker of the split-precision block: the oracle's contraction on the sampled rows of
-       the o=48, v=256 W intermediates, against which every GPU mode's rows are compared."""
+       the o=48, v=256 W intermediates, against which every GPU mode's rows are compared.
+    3. Checker of the HF block: the oracle's G = 2J - K at nbf = 48."""
     cores, model = host_cpu_info()
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
@@ -156,6 +157,14 @@ def cpu_leg(I, o, v, flops, large, budget_s=20.0, max_reps=20, warmups=3):
                       f"{len(times)} timed reps, median {med:.3f} s (min {mn:.3f} s, "
                       f"{total / mn / 1e9:.1f} GFLOP/s); CPU {model}; host BLAS {blas}; "
                       f"OMP_PROC_BIND={os.environ['OMP_PROC_BIND']}"}
+    if hf is not None:
+        g_h, D_h = hf.pop("_inputs")
+        ref = np.zeros_like(D_h)
+        pyoracle.einsum(0.0, "pq", ref, 2.0, "pqrs", g_h, "rs", D_h)
+        pyoracle.einsum(1.0, "pq", ref, -1.0, "prqs", g_h, "rs", D_h)
+        for key in ("two_einsum_calls", "fused_jk"):
+            got = hf[key].pop("_got")
+            hf[key]["rel_err_vs_oracle"] = float(f"{sampled.rel_err(got, ref):.3e}")
     if large is not None:
         t_h, g_h = large.pop("t_host"), large.pop("g_host")
         for name, row in large["contractions"].items():
@@ -167,6 +176,70 @@ def cpu_leg(I, o, v, flops, large, budget_s=20.0, max_reps=20, warmups=3):
             large["total"][key]["max_rel_err_vs_oracle"] = max(
                 row[key]["rel_err_vs_oracle"] for row in large["contractions"].values())
     return base
+
+
+def hf_block(core, nbf=48, nocc=12, reps=20):
+    """Hartree-Fock G = 2J - K at nbf = 48 (the HF workload: 8-fold symmetric g of 42.5 MB,
+    D from a QR): the reference's two einsum calls (GEMV + generic, g read twice) and the
+    fused J/K build (g read once), each timed cold (a 512 MB scratch read before every
+    call) and warm (back to back) with HIP events. HBM fraction = algorithmic bytes of the
+    fused build (g + D + G) / time / 8 TB/s. The result is compared with the oracle in
+    the CPU leg."""
+    import numpy as np
+    import torch
+
+    from einsums_amd import workloads
+
+    g_h, D_h = workloads.hf_inputs(nbf, nocc, seed=2024)
+    g = torch.from_numpy(g_h).cuda()
+    D = torch.from_numpy(D_h).cuda()
+    G = torch.zeros((nbf, nbf), dtype=torch.float64, device="cuda")
+    pj = core.compile_plan("pq", "pqrs", "rs")
+    pk = core.compile_plan("pq", "prqs", "rs")
+    scratch = torch.zeros(64 << 20, dtype=torch.float64, device="cuda")
+
+    def two_calls():
+        pj.execute(0.0, G, 2.0, g, D)
+        pk.execute(1.0, G, -1.0, g, D)
+
+    def fused():
+        core.fock_jk(g, D, G)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        warm = e0.elapsed_time(e1) / reps
+        cold = []
+        for _ in range(reps):
+            scratch.sum()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            cold.append(a.elapsed_time(b))
+        return warm, float(np.median(cold))
+
+    nbytes = 8 * (nbf ** 4 + 2 * nbf ** 2)
+    out = {"workload": f"HF G = 2J - K, nbf={nbf} (J: pq <- pqrs,rs GEMV; K: pq <- prqs,rs generic)",
+           "bytes_fused": nbytes}
+    for name, fn in (("two_einsum_calls", two_calls), ("fused_jk", fused)):
+        warm, cold = timed(fn)
+        fn()
+        out[name] = {"cold_us": round(cold * 1e3, 2), "warm_us": round(warm * 1e3, 2),
+                     "cold_hbm_frac_of_fused_bytes": round(nbytes / (cold * 1e-3) / 1e12 / 8.0, 3),
+                     "_got": G.cpu().numpy().copy()}
+    out["speedup_cold"] = round(out["two_einsum_calls"]["cold_us"] / out["fused_jk"]["cold_us"], 3)
+    out["_inputs"] = (g_h, D_h)
+    del g, D, G, scratch
+    torch.cuda.empty_cache()
+    return out
 
 
 SPLIT_MODES = (("int8", 6), ("bf16", 5), ("fp16", 5))
@@ -316,7 +389,7 @@ def precision_sweep(sizes=(1024, 2048, 4096, 8192, 16384), reps=3, cases=("dgemm
 
 def bandwidth_bench(reps=20):
     """HBM-bound kernels against the 8 TB/s peak: the HF J (GEMV) and K (generic) builds at
-    nbf = 48 and 96, the CCD energy dot and the ijab->iajb permute at o=24, v=96 and
+    nbf = 48 and 96, G = 2J - K both as the two einsum calls and as the fused J/K build, the CCD energy dot and the ijab->iajb permute at o=24, v=96 and
     o=48, v=256, and a plain copy. Each is timed warm (back-to-back calls, operands may sit
     in the 256 MB infinity cache) and cold (a 512 MB scratch read between calls, timed
     separately: a scratch write would leave dirty lines whose write-back then competes with
@@ -378,6 +451,14 @@ def bandwidth_bench(reps=20):
         out[-1]["algorithm"] = core.algorithm_name(pj.last_algorithm)
         report(f"HF K nbf={nbf} (pq <- prqs, rs)", nb, lambda: pk.execute(0.0, K, 1.0, g, D))
         out[-1]["algorithm"] = core.algorithm_name(pk.last_algorithm)
+
+        def two_calls():
+            pj.execute(0.0, J, 2.0, g, D)
+            pk.execute(1.0, J, -1.0, g, D)
+
+        # G = 2J - K: the reference's two einsum calls (g read twice), then the fused build
+        report(f"HF G = 2J - K nbf={nbf}, two einsum calls", 2 * nb + 8 * nbf ** 2, two_calls, "GEMV + GENERIC")
+        report(f"HF G = 2J - K nbf={nbf}, fused J/K build", nb, lambda: core.fock_jk(g, D, K), "FOCK_JK (one pass over g)")
         del g, D, J, K
     for (o, v) in ((24, 96), (48, 256)):
         t = torch.rand((o, o, v, v), dtype=torch.float64, device="cuda")
@@ -653,18 +734,24 @@ def main():
     prof = rocprof_figure(args.kernel_trace, flops[dom]) if (o, v) == (24, 96) else None
     if prof is not None:
         out["roofline"]["rocprof"] = prof
+    hf = hf_block(core)
+    out["hf"] = hf
     large = None
     if args.split:
         large = split_large(core)
         out["split"] = large
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_leg(I, o, v, flops, large)
-    elif large is not None:
-        for k in ("t_host", "g_host"):
-            large.pop(k)
-        for row in large["contractions"].values():
-            row.pop("_rows")
-            row.pop("_got")
+        out["cpu_baseline"] = cpu_leg(I, o, v, flops, large, hf)
+    else:
+        hf.pop("_inputs")
+        for key in ("two_einsum_calls", "fused_jk"):
+            hf[key].pop("_got")
+        if large is not None:
+            for k in ("t_host", "g_host"):
+                large.pop(k)
+            for row in large["contractions"].values():
+                row.pop("_rows")
+                row.pop("_got")
     print(json.dumps(out))
 
 

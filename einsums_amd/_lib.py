@@ -114,6 +114,8 @@ def _load():
     sig["ein_graph_end"] = (_c_int, [_vp, ctypes.POINTER(ctypes.c_void_p)])
     sig["ein_graph_launch"] = (_c_int, [_vp, _vp])
     sig["ein_graph_destroy"] = (_c_int, [_vp])
+    sig["ein_fock_jk_f64"] = (_c_int, [_i64, _dp, _dp, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                                       _i64, _vp])
     sig["ein_dot_f64_noalloc"] = (_c_int, [_i64, ctypes.c_double, _dp, _i64, _dp, _i64, ctypes.c_double, _dp, _vp])
     for sfx, T in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
         sig[f"ein_gemm_{sfx}"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, T, _dp, _i64, _dp, _i64, T,

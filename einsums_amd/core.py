@@ -27,7 +27,7 @@ __all__ = [
     "compile_plan", "EinsumGenericPlan", "EinsumDotPlan", "EinsumDirectProductPlan", "EinsumGerPlan",
     "EinsumGemvPlan", "EinsumGemmPlan", "GPUView", "COPY", "MAP", "DEVICE_TENSOR", "gpu_enabled", "einsum",
     "permute", "stream_handle", "DimensionError", "RankError", "EinsumsError", "contraction_route", "syev", "gesv",
-    "gemm_host_c",
+    "gemm_host_c", "fock_jk",
 ]
 
 # GPUView modes, numbered as the reference's PyViewMode
@@ -402,6 +402,28 @@ def _gemm_split_body(transA, transB, alpha, A, B, beta, C, nslices, slice_type):
     status = lib.ein_gemm_split_f64(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0), _ptr(A),
                                     A.stride(0), float(beta), _ptr(C), C.stride(0), int(nslices), st, stream_handle(stream))
     check(status, "gemm_split")
+
+
+def fock_jk(g: torch.Tensor, D: torch.Tensor, F: torch.Tensor, alpha_j: float = 2.0, alpha_k: float = -1.0,
+            beta: float = 0.0, stream=None) -> None:
+    """F = beta*F + alpha_j*J + alpha_k*K with J(a,b) = sum_cd g(a,b,c,d) D(c,d) and
+    K(a,c) = sum_bd g(a,b,c,d) D(b,d), both from one pass over g (n^4, contiguous, FP64).
+    Equals the reference's pair of calls
+    ``einsum(1, "ij", F, alpha_j, "kl", D, "ijkl", g)`` and ``einsum(1, "ij", F, alpha_k, "kl", D, "ikjl", g)``
+    (QChemTests.cpp:299-304) after ``F *= beta``; the defaults give G = 2J - K. n <= 128."""
+    with torch.cuda.stream(stream):
+        n = int(g.shape[0]) if g.dim() == 4 else -1
+        if g.dim() != 4 or tuple(g.shape) != (n,) * 4 or not g.is_contiguous():
+            raise ValueError("fock_jk: g must be a contiguous n x n x n x n tensor")
+        for name, t in (("D", D), ("F", F)):
+            if t.dim() != 2 or tuple(t.shape) != (n, n) or t.stride(1) != 1:
+                raise ValueError(f"fock_jk: {name} must be n x n with unit column stride")
+        for t in (g, D, F):
+            if t.dtype != torch.float64 or t.device.type != "cuda":
+                raise TypeError("fock_jk: float64 CUDA tensors only")
+        status = lib.ein_fock_jk_f64(n, _ptr(g), _ptr(D), D.stride(0), float(alpha_j), float(alpha_k), float(beta),
+                                     _ptr(F), F.stride(0), stream_handle(None))
+        check(status, "fock_jk")
 
 
 def gemm_host_c(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch.Tensor, beta: float, C: torch.Tensor,

@@ -177,6 +177,14 @@ int ein_einsum_split_f64(int32_t nslices, int32_t slice_type, double c_prefactor
                          double ab_prefactor, const ein_tensor_desc *A, const double *a, const ein_tensor_desc *B,
                          const double *b, int32_t *algorithm_used, ein_stream_t stream);
 
+/* Fock build: F = beta*F + alpha_j*J + alpha_k*K with J(a,b) = sum_cd g(a,b,c,d) D(c,d) and
+ * K(a,c) = sum_bd g(a,b,c,d) D(b,d), g row-major n^4, D and F row-major n x n (ld >= n),
+ * n <= 128. Both matrices come from one pass over g, where the reference makes two einsum
+ * calls that each read it (QChemTests.cpp:299-304, profile_einsums.cpp:23-30); no single
+ * reference function corresponds. Deterministic (fixed summation order). */
+int ein_fock_jk_f64(int64_t n, const double *g, const double *D, int64_t ldd, double alpha_j, double alpha_k,
+                    double beta, double *F, int64_t ldf, ein_stream_t stream);
+
 /* ------------------------------------------------------------- tensors */
 /* Whole-view ops on a strided tensor: C = alpha*C (alpha==0 writes zeros), C = value. */
 int ein_tensor_scale_f64(const ein_tensor_desc *C, double *c, double alpha, ein_stream_t stream);

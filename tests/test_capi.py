@@ -138,3 +138,16 @@ def test_noalloc_dot_limit_without_gpu():
     assert f(-1, 1.0, fake, 1, fake, 1, 0.0, fake, None) == -1
     assert f(10, 1.0, fake, 1, fake, 1, 0.0, None, None) == -8
 
+
+def test_fock_argument_errors_without_gpu():
+    """ein_fock_jk_f64 checks its arguments on the host before any device call."""
+    from einsums_amd import _lib
+
+    f = _lib.lib.ein_fock_jk_f64
+    fake = 4096  # never dereferenced
+    assert f(-1, fake, fake, 1, 2.0, -1.0, 0.0, fake, 1, None) == -1
+    assert f(8, fake, fake, 7, 2.0, -1.0, 0.0, fake, 8, None) == -4
+    assert f(8, fake, fake, 8, 2.0, -1.0, 0.0, fake, 7, None) == -9
+    assert f(129, fake, fake, 129, 2.0, -1.0, 0.0, fake, 129, None) == 1005
+    assert b"exceeds 128" in _lib.lib.ein_last_error_string()
+    assert f(0, None, None, 1, 2.0, -1.0, 0.0, None, 1, None) == 0  # empty: nothing to do
