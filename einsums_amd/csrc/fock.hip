@@ -12,11 +12,11 @@
 // and K(a,c) with D(b,d), so both can be accumulated from one read of g: half the HBM traffic
 // of the two calls, which is what bounds them (0.25 flop per byte).
 //
-// Kernel 1: workgroup (a, chunk of b) streams the slabs g(a,b,:,:) of its b's. Wave w takes
-// the rows c = w, w+4, ...; its lanes run over d. Per lane it keeps J partials per b and K
-// partials per row; J(a,b) is complete inside the workgroup and is written directly, K is
+// Kernel 1: workgroup (a, one or four b's) reads the slabs g(a,b,:,:) of its b's. Wave w
+// takes the rows c = w, w+W, ... (W = 4 or 16 waves); its lanes run over d. Per lane it keeps
+// K partials per row; J(a,b) is complete inside the workgroup and is written directly, K is
 // written as one partial row per chunk. Kernel 2 adds the chunks' K partials in chunk order
-// and forms F (deterministic: fixed summation trees, no atomics).
+// and forms F (deterministic: fixed summation trees, no atomics). n <= 128.
 #include <algorithm>
 #include <string>
 
@@ -26,70 +26,69 @@ namespace ein {
 
 namespace {
 
-constexpr int FOCK_MAX_N  = 128; // D staged in LDS (128 KiB at n = 128)
-constexpr int FOCK_MAX_BC = 4;   // b's per workgroup
+constexpr int FOCK_MAX_N = 128;
 
-// NPL = elements of a row per lane (1 for n <= 64, 2 for n <= 128); MR = rows c per wave.
-template <int NPL, int MR = 16 * NPL>
-__global__ __launch_bounds__(256) void fock_jk_kernel(int n, int bc, double const *__restrict__ g,
-                                                      double const *__restrict__ D, int64_t ldd, double *__restrict__ J,
-                                                      double *__restrict__ Kp) {
-    extern __shared__ double Ds[]; // n x n, row stride n
-    __shared__ double red[4][FOCK_MAX_BC];
+// NPL = elements of a row per lane (1 for n <= 64, 2 for n <= 128); WAVES = 4 or 16;
+// MR >= ceil(n / WAVES) rows c per wave; BC = b's per workgroup. Each lane keeps the
+// D(c, d) of its own (c, d) positions in registers (the same for every slab), so no LDS
+// staging or barrier precedes the slab loads: all of a workgroup's slab reads are in
+// flight at once, and the many resident workgroups keep HBM busy.
+template <int NPL, int WAVES, int MR, int BC>
+__global__ __launch_bounds__(64 * WAVES) void fock_jk_kernel(int n, double const *__restrict__ g,
+                                                             double const *__restrict__ D, int64_t ldd,
+                                                             double *__restrict__ J, double *__restrict__ Kp) {
+    __shared__ double red[WAVES][BC];
     int const a    = int(blockIdx.x);
     int const ch   = int(blockIdx.y);
     int const wave = int(threadIdx.x) >> 6;
     int const lane = int(threadIdx.x) & 63;
-    for (int e = int(threadIdx.x); e < n * n; e += 256) Ds[e] = D[int64_t(e / n) * ldd + e % n];
-    __syncthreads();
-
-    int const b0   = ch * bc;
-    int const b1   = b0 + bc < n ? b0 + bc : n;
-    int const rows = (n - wave + 3) / 4; // rows c = wave + 4*i, i < rows
-    double kacc[MR];
-#pragma unroll
-    for (int i = 0; i < MR; ++i) kacc[i] = 0.0;
+    int const b0   = ch * BC;
+    int const rows = (n - wave + WAVES - 1) / WAVES; // rows c = wave + WAVES*i, i < rows
     int64_t const nn = int64_t(n) * n;
-    // one b at a time (not unrolled, so the loads of one slab stay in flight together
-    // without the compiler hoisting every slab's loads at once)
-#pragma unroll 1
-    for (int q = 0; q < FOCK_MAX_BC; ++q) {
-        double jsum = 0.0;
-        int const b      = b0 + q;
-        bool const b_in  = b < b1;
-        double const *__restrict__ slab = g + (int64_t(a) * n + (b_in ? b : b0)) * nn;
-        // D(b, d) of this lane's d's
-        double db[NPL];
+
+    double x[BC][MR][NPL];
+#pragma unroll
+    for (int q = 0; q < BC; ++q) {
+        bool const b_in = b0 + q < n;
+        double const *__restrict__ slab = g + (int64_t(a) * n + (b_in ? b0 + q : b0)) * nn;
+#pragma unroll
+        for (int i = 0; i < MR; ++i)
+#pragma unroll
+            for (int u = 0; u < NPL; ++u) {
+                int const c = wave + WAVES * i, d = lane + 64 * u;
+                x[q][i][u]  = (b_in && i < rows && d < n) ? slab[int64_t(c) * n + d] : 0.0;
+            }
+    }
+    double dc[MR][NPL], db[BC][NPL];
+#pragma unroll
+    for (int i = 0; i < MR; ++i)
+#pragma unroll
+        for (int u = 0; u < NPL; ++u) {
+            int const c = wave + WAVES * i, d = lane + 64 * u;
+            dc[i][u]    = (i < rows && d < n) ? D[int64_t(c) * ldd + d] : 0.0;
+        }
+#pragma unroll
+    for (int q = 0; q < BC; ++q)
 #pragma unroll
         for (int u = 0; u < NPL; ++u) {
             int const d = lane + 64 * u;
-            db[u]       = (b_in && d < n) ? Ds[b * n + d] : 0.0;
+            db[q][u]    = (b0 + q < n && d < n) ? D[int64_t(b0 + q) * ldd + d] : 0.0;
         }
-        // rows in groups of RG: every load of a group is issued before any is used
-        constexpr int RG = 8;
+
+    double kacc[MR];
 #pragma unroll
-        for (int i0 = 0; i0 < MR; i0 += RG) {
-            double x[RG][NPL];
+    for (int i = 0; i < MR; ++i) kacc[i] = 0.0;
 #pragma unroll
-            for (int i = 0; i < RG; ++i)
+    for (int q = 0; q < BC; ++q) {
+        double jsum = 0.0;
 #pragma unroll
-                for (int u = 0; u < NPL; ++u) {
-                    int const c = wave + 4 * (i0 + i), d = lane + 64 * u;
-                    x[i][u]     = (b_in && i0 + i < rows && d < n) ? slab[int64_t(c) * n + d] : 0.0;
-                }
+        for (int i = 0; i < MR; ++i)
 #pragma unroll
-            for (int i = 0; i < RG; ++i) {
-                int const c = wave + 4 * (i0 + i);
-#pragma unroll
-                for (int u = 0; u < NPL; ++u) {
-                    int const d     = lane + 64 * u;
-                    double const dc = (i0 + i < rows && d < n) ? Ds[c * n + d] : 0.0;
-                    jsum += x[i][u] * dc;
-                    kacc[i0 + i] += x[i][u] * db[u];
-                }
+            for (int u = 0; u < NPL; ++u) {
+                jsum += x[q][i][u] * dc[i][u];
+                kacc[i] += x[q][i][u] * db[q][u];
             }
-        }
-        // J(a, b): lanes here, the four waves after the loop
+        // J(a, b): lanes here, the waves after the barrier
         jsum = wave_reduce_sum(jsum);
         if (lane == 0) red[wave][q] = jsum;
     }
@@ -98,12 +97,15 @@ __global__ __launch_bounds__(256) void fock_jk_kernel(int n, int bc, double cons
 #pragma unroll
     for (int i = 0; i < MR; ++i) {
         double const s = wave_reduce_sum(kacc[i]);
-        if (lane == 0 && i < rows) kp[wave + 4 * i] = s;
+        if (lane == 0 && i < rows) kp[wave + WAVES * i] = s;
     }
     __syncthreads();
-    if (threadIdx.x < unsigned(b1 - b0))
-        J[int64_t(a) * n + b0 + int(threadIdx.x)] =
-            (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+    if (int(threadIdx.x) < BC && b0 + int(threadIdx.x) < n) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) s += red[w][threadIdx.x];
+        J[int64_t(a) * n + b0 + int(threadIdx.x)] = s;
+    }
 }
 
 // F(a,c) = beta*F(a,c) + alpha_j*J(a,c) + alpha_k*sum_chunk Kp(a,chunk,c), chunks in order.
@@ -113,8 +115,15 @@ __global__ __launch_bounds__(256) void fock_finish_kernel(int n, int chunks, dou
     int64_t const e = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (e >= int64_t(n) * n) return;
     int const a = int(e / n), c = int(e % n);
-    double k = 0.0;
-    for (int ch = 0; ch < chunks; ++ch) k += Kp[(int64_t(a) * chunks + ch) * n + c];
+    // four independent partial sums so the chunk loads are in flight together
+    double const *__restrict__ kp = Kp + int64_t(a) * chunks * n + c;
+    double k4[4] = {0.0, 0.0, 0.0, 0.0};
+    int ch = 0;
+    for (; ch + 3 < chunks; ch += 4)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) k4[u] += kp[int64_t(ch + u) * n];
+    for (; ch < chunks; ++ch) k4[0] += kp[int64_t(ch) * n];
+    double const k = (k4[0] + k4[1]) + (k4[2] + k4[3]);
     double *fp = F + int64_t(a) * ldf + c;
     double v   = alpha_j * J[e] + alpha_k * k;
     if (beta != 0.0) v += beta * *fp;
@@ -133,10 +142,7 @@ int fock_jk_f64(int64_t n, double const *g, double const *D, int64_t ldd, double
                                                " (use the two einsum calls)");
     if (n == 0) return EIN_SUCCESS;
     if (g == nullptr || D == nullptr || F == nullptr) return fail(-2, "fock_jk: null pointer");
-    // b's per workgroup: enough workgroups for four per CU where n allows it
-    int const cus = num_cus();
-    int bc        = 1;
-    while (bc < FOCK_MAX_BC && n * ceil_div(n, bc * 2) >= int64_t(cus) * 4) bc *= 2;
+    int const bc     = n <= 64 ? 4 : 1; // the kernels' BC
     int const chunks = int(ceil_div(n, bc));
     void *ws = nullptr;
     size_t const j_bytes = size_t(n) * size_t(n) * sizeof(double);
@@ -144,18 +150,18 @@ int fock_jk_f64(int64_t n, double const *g, double const *D, int64_t ldd, double
     EIN_TRY(workspace(stream, j_bytes + k_bytes, &ws));
     double *J  = static_cast<double *>(ws);
     double *Kp = J + size_t(n) * size_t(n);
-    size_t const lds = size_t(n) * size_t(n) * sizeof(double);
     dim3 grid{uint32_t(n), uint32_t(chunks)};
-    if (n <= 64) {
-        fock_jk_kernel<1><<<grid, 256, lds, stream>>>(int(n), bc, g, D, ldd, J, Kp);
-    } else {
-        // more than 64 KiB of dynamic LDS has to be allowed explicitly
-        static hipError_t const attr = hipFuncSetAttribute(reinterpret_cast<void const *>(&fock_jk_kernel<2>),
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                           int(FOCK_MAX_N * FOCK_MAX_N * sizeof(double)));
-        if (attr != hipSuccess) return fail(EIN_ERROR_HIP, "fock_jk: cannot enable 128 KiB of LDS");
-        fock_jk_kernel<2><<<grid, 256, lds, stream>>>(int(n), bc, g, D, ldd, J, Kp);
-    }
+    int const ni = int(n);
+    if (n <= 32)
+        fock_jk_kernel<1, 4, 8, 4><<<grid, 256, 0, stream>>>(ni, g, D, ldd, J, Kp);
+    else if (n <= 48)
+        fock_jk_kernel<1, 4, 12, 4><<<grid, 256, 0, stream>>>(ni, g, D, ldd, J, Kp);
+    else if (n <= 64)
+        fock_jk_kernel<1, 4, 16, 4><<<grid, 256, 0, stream>>>(ni, g, D, ldd, J, Kp);
+    else if (n <= 96)
+        fock_jk_kernel<2, 16, 6, 1><<<grid, 1024, 0, stream>>>(ni, g, D, ldd, J, Kp);
+    else
+        fock_jk_kernel<2, 16, 8, 1><<<grid, 1024, 0, stream>>>(ni, g, D, ldd, J, Kp);
     EIN_LAUNCH_CHECK();
     fock_finish_kernel<<<uint32_t(ceil_div(n * n, 256)), 256, 0, stream>>>(int(n), chunks, J, Kp, alpha_j, alpha_k, beta,
                                                                          F, ldf);
