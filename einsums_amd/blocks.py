@@ -144,60 +144,164 @@ class TiledTensor:
         return out
 
 
-def block_terms(C_prefactor, C_indices: str, C: BlockTensor, AB_prefactor, A_indices: str, A: BlockTensor,
-                B_indices: str, B: BlockTensor) -> list:
-    """Per-block contraction terms of a BlockTensor einsum (BlockAlgebra.hpp:23-52)."""
-    if not (C.num_blocks() == A.num_blocks() == B.num_blocks()):
-        raise ValueError("block counts differ")
+def _diag_view(t: torch.Tensor, off: int, n: int) -> torch.Tensor:
+    """The sub-block [off, off+n) on every axis of a dense tensor (the reference's
+    ``std::apply(T, view_index)`` with every Range equal to one block's range)."""
+    return t[tuple(slice(off, off + n) for _ in range(t.dim()))]
+
+
+def _kind(x) -> str:
+    if x is None:  # a scalar C whose partials the caller places
+        return "scalar"
+    if isinstance(x, BlockTensor):
+        return "block"
+    if isinstance(x, TiledTensor):
+        return "tiled"
+    if isinstance(x, torch.Tensor):
+        return "scalar" if x.dim() == 0 else "dense"
+    raise TypeError(f"blocks.einsum: unsupported operand type {type(x).__name__}")
+
+
+def block_terms(C_prefactor, C_indices: str, C, AB_prefactor, A_indices: str, A, B_indices: str, B) -> list:
+    """Per-block contraction terms of an einsum with BlockTensor operands
+    (BlockAlgebra.hpp:23-428). A and B are BlockTensors, or one of them is a dense tensor
+    (the reference's vector case) that is cut into the diagonal sub-block of every block's
+    range; C is a BlockTensor (block i of C), a dense tensor (its diagonal sub-block i, the
+    rest untouched) or -- handled by ``einsum`` -- a scalar. Blocks of extent 0 are skipped."""
+    kinds = (_kind(C), _kind(A), _kind(B))
+    blk = A if kinds[1] == "block" else B
+    if kinds[1] != "block" and kinds[2] != "block":
+        raise TypeError("block_terms: A or B must be a BlockTensor")
+    nb = blk.num_blocks()
+    for X, k in zip((C, A, B), kinds):
+        if k == "block" and X.num_blocks() != nb:
+            raise ValueError("block counts differ")  # tensor_compat_error
+    offs = [0] + list(itertools.accumulate(blk.block_dims))
     terms = []
-    for i in range(C.num_blocks()):
-        if C.block_dims[i] != A.block_dims[i] or C.block_dims[i] != B.block_dims[i]:
-            raise _lib.DimensionError(f"block {i} extents differ")
-        if C.block_dims[i] == 0:
+    for i in range(nb):
+        n = blk.block_dims[i]
+        for X, k in zip((C, A, B), kinds):
+            if k == "block" and X.block_dims[i] != n:
+                raise _lib.DimensionError(f"block {i} extents differ")
+        if n == 0:
             continue
-        terms.append((C_indices, C[i], A_indices, A[i], B_indices, B[i], C_prefactor, AB_prefactor))
+
+        def part(X, k):
+            return X[i] if k == "block" else _diag_view(X, offs[i], n)
+
+        terms.append((C_indices, part(C, kinds[0]) if kinds[0] != "scalar" else None, A_indices, part(A, kinds[1]),
+                      B_indices, part(B, kinds[2]), C_prefactor, AB_prefactor))
     return terms
 
 
-def tiled_terms(C_prefactor, C_indices: str, C: TiledTensor, AB_prefactor, A_indices: str, A: TiledTensor,
-                B_indices: str, B: TiledTensor) -> tuple[list, list]:
-    """Tile contraction terms of a TiledTensor einsum (TileAlgebra.hpp:18-93): the tile grid of
-    every distinct label is walked; triples with a missing or empty tile are skipped. Returns
-    (terms accumulating with prefactor 1, C tiles to scale by C_prefactor first)."""
+def _tile_views(X, kind: str, rank: int):
+    """(grid extents per axis, {coord: tile}) of a TiledTensor, or of a BlockTensor seen as a
+    tiled tensor whose only tiles are the diagonal ones (BlockTileAlgebra.hpp:18-173)."""
+    if kind == "tiled":
+        return [X.grid_size(d) for d in range(X.rank)], {c: t for c, t in X.tiles.items() if t.numel() > 0}
+    nb = X.num_blocks()
+    return [nb] * rank, {(i,) * rank: X[i] for i in range(nb) if X.block_dims[i] > 0}
+
+
+def tiled_terms(C_prefactor, C_indices: str, C, AB_prefactor, A_indices: str, A, B_indices: str, B) -> tuple[list, list]:
+    """Tile contraction terms of an einsum over TiledTensors, or over a mix of TiledTensors
+    and BlockTensors (TileAlgebra.hpp:18-158, BlockTileAlgebra.hpp:18-173). The tile grid of
+    every distinct label is walked; a triple whose A or B tile is absent or empty is skipped,
+    and a TiledTensor C gets a zero tile where a triple lands on an absent one (the
+    reference's ``C->tile(index)``). Returns (terms accumulating with prefactor 1, C tiles to
+    scale by C_prefactor first); a scalar C is handled by ``einsum``."""
+    kinds = (_kind(C), _kind(A), _kind(B))
+    views = {}
+    for name, X, k, idx in (("C", C, kinds[0], C_indices), ("A", A, kinds[1], A_indices), ("B", B, kinds[2], B_indices)):
+        if k in ("tiled", "block"):
+            views[name] = _tile_views(X, k, len(idx))
     labels = list(dict.fromkeys(C_indices + A_indices + B_indices))
     grid = {}
-    for T, idx in ((C, C_indices), (A, A_indices), (B, B_indices)):
+    for name, idx in (("C", C_indices), ("A", A_indices), ("B", B_indices)):
+        if name not in views:
+            continue
         for d, ch in enumerate(idx):
-            n = T.grid_size(d)
+            n = views[name][0][d]
             if grid.setdefault(ch, n) != n:
                 raise _lib.DimensionError(f"tile grids disagree on index {ch!r}")
-    scale = list(C.tiles.values())
+    scale = list(C.tiles.values()) if kinds[0] == "tiled" else (C.blocks if kinds[0] == "block" else [])
     terms = []
     for combo in itertools.product(*[range(grid[ch]) for ch in labels]):
         pos = dict(zip(labels, combo))
-        cc = tuple(pos[ch] for ch in C_indices)
         ac = tuple(pos[ch] for ch in A_indices)
         bc = tuple(pos[ch] for ch in B_indices)
-        if not (C.has_tile(cc) and A.has_tile(ac) and B.has_tile(bc)):
+        ta, tb = views["A"][1].get(ac), views["B"][1].get(bc)
+        if ta is None or tb is None:
             continue
-        terms.append((C_indices, C.tiles[cc], A_indices, A.tiles[ac], B_indices, B.tiles[bc], 1.0, AB_prefactor))
+        tc = None
+        if kinds[0] != "scalar":
+            cc = tuple(pos[ch] for ch in C_indices)
+            tc = views["C"][1].get(cc)
+            if tc is None:
+                if kinds[0] == "block":
+                    continue  # off-diagonal position of a block C: no such block
+                if any(e == 0 for e in C.tile_shape(cc)):
+                    continue
+                tc = C.add_tile(cc)  # zero tile, created where a contribution lands
+                views["C"][1][cc] = tc  # later triples on this position use the same tile
+        terms.append((C_indices, tc, A_indices, ta, B_indices, tb, 1.0, AB_prefactor))
     return terms, scale
 
 
-def einsum(C_prefactor, C_indices: str, C, AB_prefactor, A_indices: str, A, B_indices: str, B,
-           stream=None, autotune: bool = False) -> GroupedEinsum:
-    """Block/tiled einsum in one grouped plan; returns the plan so it can be replayed."""
-    if isinstance(C, BlockTensor):
-        plan = GroupedEinsum(block_terms(C_prefactor, C_indices, C, AB_prefactor, A_indices, A, B_indices, B),
+def _scalar_einsum(C_prefactor, C: torch.Tensor, terms: list, autotune, stream) -> GroupedEinsum | None:
+    """C = C_prefactor*C + sum over the terms' full contractions: each term writes its own
+    partial, then the partials are added in term order by the deterministic dot (the
+    reference's scalar-C block/tile dispatch, BlockAlgebra.hpp:105-180, TileAlgebra.hpp:95-158:
+    C zeroed when C_prefactor == 0, scaled otherwise). The returned plan recomputes the
+    partials only; capture the whole call in a ``Graph`` to replay the sum as well."""
+    nt = len(terms)
+    partial = torch.zeros(max(nt, 1), dtype=C.dtype, device=C.device)
+    ones = torch.ones_like(partial)
+    if stream is not None:  # allocated on the current stream, used on `stream`
+        partial.record_stream(stream)
+        ones.record_stream(stream)
+    plan = None
+    if nt:
+        plan = GroupedEinsum([(t[0], partial[k], t[2], t[3], t[4], t[5], 0.0, t[7]) for k, t in enumerate(terms)],
                              autotune, stream)
         plan.execute(stream)
-        return plan
-    if isinstance(C, TiledTensor):
+        plan._keep.extend((partial, ones))
+    from .core import _fn
+    check(_fn("ein_dot", C.dtype)(nt, 1.0, _ptr(partial), 1, _ptr(ones), 1, float(C_prefactor), _ptr(C),
+                                  stream_handle(stream)), "block scalar sum")
+    return plan
+
+
+def einsum(C_prefactor, C_indices: str, C, AB_prefactor, A_indices: str, A, B_indices: str, B,
+           stream=None, autotune: bool = False) -> GroupedEinsum | None:
+    """Block/tiled einsum in one grouped plan; returns the plan so it can be replayed (None
+    when nothing is left to contract). Operand kinds, as the reference's special dispatch:
+      * BlockTensor x BlockTensor -> BlockTensor, dense tensor or scalar (BlockAlgebra.hpp:23-180);
+      * dense tensor x BlockTensor, BlockTensor x dense tensor -> BlockTensor, dense or scalar
+        (the dense operand cut by every block's range; BlockAlgebra.hpp:182-428);
+      * TiledTensor x TiledTensor -> TiledTensor or scalar (TileAlgebra.hpp:18-158);
+      * any mix of BlockTensor and TiledTensor (BlockTileAlgebra.hpp:18-173).
+    A scalar C is a 0-dimensional device tensor."""
+    kinds = (_kind(C), _kind(A), _kind(B))
+    if "tiled" in kinds:
+        if "dense" in kinds:
+            raise TypeError("blocks.einsum: dense operands combine with BlockTensors only")
         terms, scale = tiled_terms(C_prefactor, C_indices, C, AB_prefactor, A_indices, A, B_indices, B)
+        if kinds[0] == "scalar":
+            return _scalar_einsum(C_prefactor, C, terms, autotune, stream)
         from .core import _fn
         for t in scale:  # C scaled once by C_prefactor, then every tile triple accumulates
             d = make_desc("".join(chr(97 + i) for i in range(t.dim())), t.shape, t.stride())
             check(_fn("ein_tensor_scale", t.dtype)(ctypes.byref(d), _ptr(t), float(C_prefactor), stream_handle(stream)))
+        if not terms:
+            return None
+        plan = GroupedEinsum(terms, autotune, stream)
+        plan.execute(stream)
+        return plan
+    if kinds[1] == "block" or kinds[2] == "block":
+        terms = block_terms(C_prefactor, C_indices, C, AB_prefactor, A_indices, A, B_indices, B)
+        if kinds[0] == "scalar":
+            return _scalar_einsum(C_prefactor, C, terms, autotune, stream)
         if not terms:
             return None
         plan = GroupedEinsum(terms, autotune, stream)

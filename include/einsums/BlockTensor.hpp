@@ -113,9 +113,151 @@ class GroupedPlan {
 namespace tensor_algebra {
 namespace detail {
 
+template <typename X>
+inline constexpr bool is_block_v = false;
+template <typename T, size_t R>
+inline constexpr bool is_block_v<BlockDeviceTensor<T, R>> = true;
+template <typename X>
+inline constexpr bool is_tiled_v = false;
+template <typename T, size_t R>
+inline constexpr bool is_tiled_v<TiledDeviceTensor<T, R>> = true;
+
 template <typename TT, size_t R>
 ein_tensor_desc block_desc(DeviceTensor<TT, R> const &t, int32_t const *labels) {
     return t.desc(labels);
+}
+
+// The sub-block [off, off+n) on every axis of a dense device tensor: the reference's
+// std::apply(T, view_index) with every Range equal to one block's range.
+template <typename T, size_t R>
+DeviceTensorView<T, R> diag_view(DeviceTensorBase<T, R> const &t, int64_t off, int64_t n) {
+    std::array<int64_t, R> d;
+    d.fill(n);
+    int64_t o = 0;
+    for (size_t i = 0; i < R; ++i) o += off * t.stride(i);
+    return DeviceTensorView<T, R>(t.name() + " block", t.data() + o, d, t.strides());
+}
+
+inline ein_contraction_item make_item(ein_tensor_desc const &C, void *c, ein_tensor_desc const &A, void const *a,
+                                      ein_tensor_desc const &B, void const *b, double cp, double abp) {
+    ein_contraction_item it{};
+    it.C            = C;
+    it.A            = A;
+    it.B            = B;
+    it.c            = c;
+    it.a            = a;
+    it.b            = b;
+    it.c_prefactor  = cp;
+    it.ab_prefactor = abp;
+    return it;
+}
+
+// Scalar C of a block or tiled einsum (BlockAlgebra.hpp:105-180, TileAlgebra.hpp:95-158):
+// every item writes its own partial sum, then C = C_prefactor*C + (partials added in item
+// order) by the deterministic dot. The result is read back (a synchronisation).
+template <typename T, typename CType>
+void scalar_from_items(T C_prefactor, CType *C, std::vector<ein_contraction_item> items) {
+    size_t const nt = items.size();
+    DeviceTensor<T, 1> partial("block partial sums", int64_t(std::max<size_t>(nt, 1)));
+    DeviceTensor<T, 1> ones("ones", std::array<int64_t, 1>{int64_t(std::max<size_t>(nt, 1))},
+                            std::vector<T>(std::max<size_t>(nt, 1), T(1)));
+    for (size_t k = 0; k < nt; ++k) {
+        items[k].C           = ein_tensor_desc{};
+        items[k].c           = partial.data() + k;
+        items[k].c_prefactor = 0.0;
+    }
+    if (nt) GroupedPlan(int(sizeof(T)), items).execute();
+    DeviceTensor<T, 0> c("scalar C");
+    if (C_prefactor != T(0)) c = T(*C);
+    int st;
+    if constexpr (std::is_same_v<T, double>)
+        st = ein_dot_f64(int64_t(nt), 1.0, partial.data(), 1, ones.data(), 1, double(C_prefactor), c.data(), gpu::stream());
+    else
+        st = ein_dot_f32(int64_t(nt), 1.0f, partial.data(), 1, ones.data(), 1, float(C_prefactor), c.data(), gpu::stream());
+    check(st, "block scalar sum");
+    *C = CType(T(c));
+}
+
+// Items of an einsum over BlockDeviceTensors where A and/or B is a BlockDeviceTensor and the
+// other may be a dense device tensor cut into the diagonal sub-block of every block's range;
+// C is a BlockDeviceTensor (block i), a dense device tensor (its diagonal sub-block i; the
+// rest untouched) or a scalar (placeholder items). Blocks of extent 0 are skipped
+// (BlockAlgebra.hpp:23-428).
+template <typename T, typename CType, typename AType, typename BType, size_t NC, size_t NA, size_t NB>
+std::vector<ein_contraction_item> block_items(T C_prefactor, std::array<int32_t, NC> const &c_ids, CType *C,
+                                              T AB_prefactor, std::array<int32_t, NA> const &a_ids, AType const &A,
+                                              std::array<int32_t, NB> const &b_ids, BType const &B) {
+    static_assert(is_block_v<AType> || is_block_v<BType>, "A or B must be a BlockDeviceTensor");
+    auto const &blk = [&]() -> auto const & {
+        if constexpr (is_block_v<AType>) return A;
+        else return B;
+    }();
+    size_t const nb = blk.num_blocks();
+    auto check_blocks = [&](auto const &X) {
+        if constexpr (is_block_v<std::remove_cvref_t<decltype(X)>>) {
+            if (X.num_blocks() != nb) throw tensor_compat_error("block einsum: block counts differ");
+            for (size_t i = 0; i < nb; ++i)
+                if (X.block_dim(i) != blk.block_dim(i)) throw dimension_error("block einsum: block extents differ");
+        }
+    };
+    check_blocks(A);
+    check_blocks(B);
+    if constexpr (is_block_v<CType>) check_blocks(*C);
+    std::vector<ein_contraction_item> items;
+    int64_t off = 0;
+    for (size_t i = 0; i < nb; ++i) {
+        int64_t const n = blk.block_dim(i);
+        if (n == 0) continue;
+        auto part = [&](auto const &X, auto const &ids) {
+            using X_t = std::remove_cvref_t<decltype(X)>;
+            if constexpr (is_block_v<X_t>) return X.block(i).view();
+            else return diag_view(X, off, n);
+        };
+        auto av = part(A, a_ids);
+        auto bv = part(B, b_ids);
+        ein_tensor_desc cd{};
+        void *cptr = nullptr;
+        if constexpr (is_block_v<CType>) {
+            cd   = C->block(i).desc(c_ids.data());
+            cptr = C->block(i).data();
+        } else if constexpr (!std::is_arithmetic_v<CType>) {
+            auto cv = diag_view(*C, off, n);
+            cd      = cv.desc(c_ids.data());
+            cptr    = cv.data();
+        }
+        items.push_back(make_item(cd, cptr, av.desc(a_ids.data()), av.data(), bv.desc(b_ids.data()), bv.data(),
+                                  double(C_prefactor), double(AB_prefactor)));
+        off += n;
+    }
+    return items;
+}
+
+// A BlockDeviceTensor seen as a tiled tensor whose tiles are its diagonal blocks
+// (BlockTileAlgebra.hpp:18-173).
+template <typename T, size_t R>
+struct BlockTiles {
+    BlockDeviceTensor<T, R> const *b;
+    using Coord = std::array<int, R>;
+    size_t grid_size(size_t) const { return b->num_blocks(); }
+    bool has_tile(Coord const &c) const {
+        for (size_t d = 1; d < R; ++d)
+            if (c[d] != c[0]) return false;
+        return b->block_dim(size_t(c[0])) > 0;
+    }
+    DeviceTensor<T, R> const &tile(Coord const &c) const { return b->block(size_t(c[0])); }
+};
+
+template <typename T, size_t R>
+auto as_tiles(TiledDeviceTensor<T, R> &t) -> TiledDeviceTensor<T, R> & {
+    return t;
+}
+template <typename T, size_t R>
+auto as_tiles(TiledDeviceTensor<T, R> const &t) -> TiledDeviceTensor<T, R> & {
+    return const_cast<TiledDeviceTensor<T, R> &>(t);
+}
+template <typename T, size_t R>
+BlockTiles<T, R> as_tiles(BlockDeviceTensor<T, R> const &b) {
+    return BlockTiles<T, R>{&b};
 }
 
 } // namespace detail
@@ -126,49 +268,50 @@ GroupedPlan make_block_plan(T C_prefactor, std::tuple<CIndices...> const &, Bloc
                             std::tuple<AIndices...> const &, BlockDeviceTensor<T, AR> const &A,
                             std::tuple<BIndices...> const &, BlockDeviceTensor<T, BR> const &B) {
     static_assert(sizeof...(CIndices) == CR && sizeof...(AIndices) == AR && sizeof...(BIndices) == BR, "rank mismatch");
-    if (C->num_blocks() != A.num_blocks() || C->num_blocks() != B.num_blocks())
-        throw tensor_compat_error("block einsum: block counts differ");
     constexpr auto c_ids = detail::label_ids(std::tuple<CIndices...>{});
     constexpr auto a_ids = detail::label_ids(std::tuple<AIndices...>{});
     constexpr auto b_ids = detail::label_ids(std::tuple<BIndices...>{});
-    std::vector<ein_contraction_item> items;
-    for (size_t i = 0; i < C->num_blocks(); ++i) {
-        if (C->block_dim(i) != A.block_dim(i) || C->block_dim(i) != B.block_dim(i))
-            throw dimension_error("block einsum: block extents differ");
-        if (C->block_dim(i) == 0) continue;
-        ein_contraction_item it{};
-        it.C            = C->block(i).desc(c_ids.data());
-        it.A            = A.block(i).desc(a_ids.data());
-        it.B            = B.block(i).desc(b_ids.data());
-        it.c            = C->block(i).data();
-        it.a            = A.block(i).data();
-        it.b            = B.block(i).data();
-        it.c_prefactor  = double(C_prefactor);
-        it.ab_prefactor = double(AB_prefactor);
-        items.push_back(it);
-    }
-    return GroupedPlan(int(sizeof(T)), items);
+    return GroupedPlan(int(sizeof(T)), detail::block_items(C_prefactor, c_ids, C, AB_prefactor, a_ids, A, b_ids, B));
 }
 
-template <typename T, size_t CR, size_t AR, size_t BR, typename U, typename... CIndices, typename... AIndices,
+// Einsums with BlockDeviceTensor operands, every combination of the reference's special
+// dispatch (BlockAlgebra.hpp:23-428): A and B blocks, or one of them a dense device tensor
+// (cut by every block's range); C a BlockDeviceTensor, a dense device tensor (its diagonal
+// blocks) or a scalar. One grouped plan per call.
+template <typename U, typename CType, typename AType, typename BType, typename... CIndices, typename... AIndices,
           typename... BIndices>
-void einsum(U const C_prefactor, std::tuple<CIndices...> const &Ci, BlockDeviceTensor<T, CR> *C, U const AB_prefactor,
-            std::tuple<AIndices...> const &Ai, BlockDeviceTensor<T, AR> const &A, std::tuple<BIndices...> const &Bi,
-            BlockDeviceTensor<T, BR> const &B) {
-    make_block_plan(T(C_prefactor), Ci, C, T(AB_prefactor), Ai, A, Bi, B).execute();
-}
-
-// TiledDeviceTensor einsum (TileAlgebra.hpp:18-93): C scaled once by C_prefactor, then every
-// tile triple over the tile grid of the distinct labels accumulates with prefactor 1;
-// triples with an absent or empty tile are skipped. Triples writing the same C tile run in
-// order (separate waves of the plan).
-template <typename T, size_t CR, size_t AR, size_t BR, typename... CIndices, typename... AIndices, typename... BIndices>
-GroupedPlan make_tiled_plan(T AB_prefactor, std::tuple<CIndices...> const &, TiledDeviceTensor<T, CR> *C,
-                            std::tuple<AIndices...> const &, TiledDeviceTensor<T, AR> &A, std::tuple<BIndices...> const &,
-                            TiledDeviceTensor<T, BR> &B) {
+    requires((detail::is_block_v<AType> || detail::is_block_v<BType>) &&
+             (detail::is_block_v<AType> || detail::DeviceTensorConcept<AType>) &&
+             (detail::is_block_v<BType> || detail::DeviceTensorConcept<BType>) &&
+             (detail::is_block_v<CType> || detail::DeviceTensorConcept<CType> || std::is_arithmetic_v<CType>))
+void einsum(U const C_prefactor, std::tuple<CIndices...> const &, CType *C, U const AB_prefactor,
+            std::tuple<AIndices...> const &, AType const &A, std::tuple<BIndices...> const &, BType const &B) {
+    using T = std::conditional_t<detail::is_block_v<AType>, typename AType::ValueType, typename BType::ValueType>;
     constexpr auto c_ids = detail::label_ids(std::tuple<CIndices...>{});
     constexpr auto a_ids = detail::label_ids(std::tuple<AIndices...>{});
     constexpr auto b_ids = detail::label_ids(std::tuple<BIndices...>{});
+    auto items = detail::block_items(T(C_prefactor), c_ids, C, T(AB_prefactor), a_ids, A, b_ids, B);
+    if constexpr (std::is_arithmetic_v<CType>) {
+        detail::scalar_from_items(T(C_prefactor), C, std::move(items));
+    } else if (!items.empty()) {
+        GroupedPlan(int(sizeof(T)), items).execute();
+    }
+}
+
+template <typename X>
+struct is_block_v_helper : std::bool_constant<detail::is_block_v<X>> {};
+
+// Tile-triple plan of an einsum over TiledDeviceTensors, or over a mix of TiledDeviceTensors
+// and BlockDeviceTensors (TileAlgebra.hpp:18-93, BlockTileAlgebra.hpp:18-173): every tile
+// triple over the tile grid of the distinct labels accumulates with prefactor 1; triples with
+// an absent or empty A or B tile are skipped. A TiledDeviceTensor C gets a zero tile where a
+// triple lands on an absent one (the reference's C->tile(index)); a BlockDeviceTensor C has
+// only its diagonal positions. Triples writing the same C tile run in order (separate waves
+// of the plan). With C == nullptr the items carry no C (a scalar C, placed by the caller).
+template <typename T, typename CT, typename AT, typename BT, size_t NC, size_t NA, size_t NB>
+std::vector<ein_contraction_item> tile_items(T AB_prefactor, std::array<int32_t, NC> const &c_ids, CT *C,
+                                             std::array<int32_t, NA> const &a_ids, AT const &A,
+                                             std::array<int32_t, NB> const &b_ids, BT const &B) {
     std::vector<int32_t> labels;
     std::vector<size_t> grid;
     auto add = [&](int32_t id, size_t n) {
@@ -180,9 +323,10 @@ GroupedPlan make_tiled_plan(T AB_prefactor, std::tuple<CIndices...> const &, Til
             throw dimension_error("tiled einsum: tile grids disagree");
         }
     };
-    for (size_t d = 0; d < CR; ++d) add(c_ids[d], C->grid_size(d));
-    for (size_t d = 0; d < AR; ++d) add(a_ids[d], A.grid_size(d));
-    for (size_t d = 0; d < BR; ++d) add(b_ids[d], B.grid_size(d));
+    if (C != nullptr)
+        for (size_t d = 0; d < NC; ++d) add(c_ids[d], C->grid_size(d));
+    for (size_t d = 0; d < NA; ++d) add(a_ids[d], A.grid_size(d));
+    for (size_t d = 0; d < NB; ++d) add(b_ids[d], B.grid_size(d));
     auto pos_of = [&](int32_t id) { return size_t(std::find(labels.begin(), labels.end(), id) - labels.begin()); };
     std::vector<ein_contraction_item> items;
     std::vector<size_t> combo(labels.size(), 0);
@@ -194,34 +338,83 @@ GroupedPlan make_tiled_plan(T AB_prefactor, std::tuple<CIndices...> const &, Til
             combo[q] = rem % grid[q];
             rem /= grid[q];
         }
-        typename TiledDeviceTensor<T, CR>::Coord cc;
-        typename TiledDeviceTensor<T, AR>::Coord ac;
-        typename TiledDeviceTensor<T, BR>::Coord bc;
-        for (size_t d = 0; d < CR; ++d) cc[d] = int(combo[pos_of(c_ids[d])]);
-        for (size_t d = 0; d < AR; ++d) ac[d] = int(combo[pos_of(a_ids[d])]);
-        for (size_t d = 0; d < BR; ++d) bc[d] = int(combo[pos_of(b_ids[d])]);
-        if (!C->has_tile(cc) || !A.has_tile(ac) || !B.has_tile(bc)) continue;
-        ein_contraction_item it{};
-        it.C            = C->tile(cc).desc(c_ids.data());
-        it.A            = A.tile(ac).desc(a_ids.data());
-        it.B            = B.tile(bc).desc(b_ids.data());
-        it.c            = C->tile(cc).data();
-        it.a            = A.tile(ac).data();
-        it.b            = B.tile(bc).data();
-        it.c_prefactor  = 1.0;
-        it.ab_prefactor = double(AB_prefactor);
-        items.push_back(it);
+        std::array<int, NA> ac;
+        std::array<int, NB> bc;
+        for (size_t d = 0; d < NA; ++d) ac[d] = int(combo[pos_of(a_ids[d])]);
+        for (size_t d = 0; d < NB; ++d) bc[d] = int(combo[pos_of(b_ids[d])]);
+        if (!A.has_tile(ac) || !B.has_tile(bc)) continue;
+        auto const &ta = A.tile(ac);
+        auto const &tb = B.tile(bc);
+        ein_tensor_desc cd{};
+        void *cptr = nullptr;
+        if (C != nullptr) {
+            std::array<int, NC> cc;
+            for (size_t d = 0; d < NC; ++d) cc[d] = int(combo[pos_of(c_ids[d])]);
+            if constexpr (is_block_v_helper<CT>::value) {
+                bool diag = true;
+                for (size_t d = 1; d < NC; ++d) diag = diag && cc[d] == cc[0];
+                if (!diag || C->block_dim(size_t(cc[0])) == 0) continue;
+                cd   = C->block(size_t(cc[0])).desc(c_ids.data());
+                cptr = C->block(size_t(cc[0])).data();
+            } else {
+                auto shape = C->tile_shape(cc);
+                bool empty = false;
+                for (auto e : shape) empty = empty || e == 0;
+                if (empty) continue;
+                bool const fresh = !C->has_tile(cc);
+                auto &tc         = C->tile(cc); // created when absent
+                if (fresh) tc.zero();
+                cd = tc.desc(c_ids.data());
+                cptr     = tc.data();
+            }
+        }
+        items.push_back(detail::make_item(cd, cptr, ta.desc(a_ids.data()), ta.data(), tb.desc(b_ids.data()), tb.data(), 1.0,
+                                          double(AB_prefactor)));
     }
-    return GroupedPlan(int(sizeof(T)), items);
+    return items;
 }
 
-template <typename T, size_t CR, size_t AR, size_t BR, typename U, typename... CIndices, typename... AIndices,
+template <typename T, size_t CR, size_t AR, size_t BR, typename... CIndices, typename... AIndices, typename... BIndices>
+GroupedPlan make_tiled_plan(T AB_prefactor, std::tuple<CIndices...> const &, TiledDeviceTensor<T, CR> *C,
+                            std::tuple<AIndices...> const &, TiledDeviceTensor<T, AR> &A, std::tuple<BIndices...> const &,
+                            TiledDeviceTensor<T, BR> &B) {
+    constexpr auto c_ids = detail::label_ids(std::tuple<CIndices...>{});
+    constexpr auto a_ids = detail::label_ids(std::tuple<AIndices...>{});
+    constexpr auto b_ids = detail::label_ids(std::tuple<BIndices...>{});
+    return GroupedPlan(int(sizeof(T)), tile_items(AB_prefactor, c_ids, C, a_ids, A, b_ids, B));
+}
+
+// Tiled and mixed block/tiled einsums (TileAlgebra.hpp:18-158, BlockTileAlgebra.hpp:18-173):
+// C (tiled or block) scaled once by C_prefactor (zeroed when it is 0), then every tile triple
+// accumulates; a scalar C receives C_prefactor*C + the sum of the triples' contractions.
+template <typename U, typename CType, typename AType, typename BType, typename... CIndices, typename... AIndices,
           typename... BIndices>
-void einsum(U const C_prefactor, std::tuple<CIndices...> const &Ci, TiledDeviceTensor<T, CR> *C, U const AB_prefactor,
-            std::tuple<AIndices...> const &Ai, TiledDeviceTensor<T, AR> &A, std::tuple<BIndices...> const &Bi,
-            TiledDeviceTensor<T, BR> &B) {
-    for (auto &kv : C->tiles()) kv.second.scale(T(C_prefactor));
-    make_tiled_plan(T(AB_prefactor), Ci, C, Ai, A, Bi, B).execute();
+    requires((detail::is_tiled_v<AType> || detail::is_tiled_v<BType> || detail::is_tiled_v<CType>) &&
+             (detail::is_tiled_v<AType> || detail::is_block_v<AType>) &&
+             (detail::is_tiled_v<BType> || detail::is_block_v<BType>) &&
+             (detail::is_tiled_v<CType> || detail::is_block_v<CType> || std::is_arithmetic_v<CType>))
+void einsum(U const C_prefactor, std::tuple<CIndices...> const &, CType *C, U const AB_prefactor,
+            std::tuple<AIndices...> const &, AType const &A, std::tuple<BIndices...> const &, BType const &B) {
+    using T = typename AType::ValueType;
+    constexpr auto c_ids = detail::label_ids(std::tuple<CIndices...>{});
+    constexpr auto a_ids = detail::label_ids(std::tuple<AIndices...>{});
+    constexpr auto b_ids = detail::label_ids(std::tuple<BIndices...>{});
+    auto At = detail::as_tiles(A);
+    auto Bt = detail::as_tiles(B);
+    if constexpr (std::is_arithmetic_v<CType>) {
+        using NoC = TiledDeviceTensor<T, 1>;
+        auto items = tile_items<T, NoC>(T(AB_prefactor), std::array<int32_t, 1>{0}, static_cast<NoC *>(nullptr), a_ids, At,
+                                        b_ids, Bt);
+        detail::scalar_from_items(T(C_prefactor), C, std::move(items));
+    } else {
+        if constexpr (detail::is_tiled_v<CType>) {
+            for (auto &kv : C->tiles()) kv.second.scale(T(C_prefactor));
+        } else {
+            for (size_t i = 0; i < C->num_blocks(); ++i) C->block(i).scale(T(C_prefactor));
+        }
+        auto items = tile_items(T(AB_prefactor), c_ids, C, a_ids, At, b_ids, Bt);
+        if (!items.empty()) GroupedPlan(int(sizeof(T)), items).execute();
+    }
 }
 
 } // namespace tensor_algebra

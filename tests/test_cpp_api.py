@@ -74,3 +74,12 @@ def test_api_overloads_program():
     res = _run("test_api_overloads", 120)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "PASS test_api_overloads" in res.stdout
+
+
+@pytest.mark.gpu
+def test_block_executor_program():
+    """Block / tiled einsums with dense, vector and scalar operands and mixed block/tiled
+    operands in C++, against host loop nests."""
+    res = _run("test_block_executor", 120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "PASS test_block_executor" in res.stdout
