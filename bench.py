@@ -51,6 +51,11 @@ def parse():
                    help="cases of the precision sweep: any of dgemm, wmnij, wabef, wmbej (comma separated)")
     p.add_argument("--bandwidth", action="store_true",
                    help="only run the HBM-bound kernels (HF J/K, dot, permute; cold and warm cache) and exit")
+    p.add_argument("--tune", type=str, default="", metavar="PATH",
+                   help="run the CCD workloads once (o=24 v=96 and o=48 v=256, FP64) in GEMM tuning mode, save "
+                        "the tuning table to PATH and exit")
+    p.add_argument("--tuning-table", type=str, default=str(ROOT / "einsums_amd" / "gemm_tuning_mi355x.txt"),
+                   help="GEMM tuning table loaded before the timed steps (skipped when the file is absent)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES bytes (counter calibration) and exit")
     return p.parse_args()
@@ -622,8 +627,35 @@ def grouped_bench(reps=5):
     return res
 
 
+def tune(path: str) -> None:
+    """GEMM tuning pass: the FP64 GEMM-mapped and packed-generic contractions of the CCD step
+    at o=24, v=96 and o=48, v=256, each run once in tuning mode; the table is saved."""
+    import torch
+
+    import einsums_amd.core as core
+    from einsums_amd import workloads
+
+    for o, v in ((24, 96), (48, 256)):
+        I = workloads.ccd_inputs_device(o, v, seed=11, with_vvvv=True)
+        arrs = {"t_oovv": I.t_oovv, "g_oovv": I.g_oovv, "g_oooo": I.g_oooo, "g_vvvv": I.g_vvvv, "g_ovvo": I.g_ovvo}
+        with core.gemm_tuning():
+            for name, Ci, An, Ai, Bn, Bi, cp, abp, init in workloads.CCD_CONTRACTIONS:
+                W = arrs[init]
+                core.einsum(Ci, W, Ai, arrs[An], Bi, arrs[Bn], C_prefactor=cp, AB_prefactor=abp)
+                torch.cuda.synchronize()
+                print(json.dumps({"tuned": name, "o": o, "v": v, "entries": int(core.lib.ein_gemm_tuning_count())}),
+                      flush=True)
+        del I, arrs
+        torch.cuda.empty_cache()
+    core.save_gemm_tuning(path)
+    print(json.dumps({"saved": path, "entries": int(core.lib.ein_gemm_tuning_count())}))
+
+
 def main():
     args = parse()
+    if args.tune:
+        tune(args.tune)
+        return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
@@ -647,6 +679,8 @@ def main():
     import einsums_amd.core as core
     from einsums_amd import workloads
 
+    if args.tuning_table and Path(args.tuning_table).exists():
+        core.load_gemm_tuning(args.tuning_table)
     o, v = args.o, args.v
     I = workloads.ccd_inputs(o, v, seed=7)
     flops = workloads.ccd_flops(o, v)

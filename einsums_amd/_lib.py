@@ -116,6 +116,13 @@ def _load():
     sig["ein_graph_destroy"] = (_c_int, [_vp])
     sig["ein_fock_jk_f64"] = (_c_int, [_i64, _dp, _dp, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                                        _i64, _vp])
+    sig["ein_gemm_tuning_mode"] = (_c_int, [ctypes.c_int32])
+    sig["ein_gemm_tuning_clear"] = (_c_int, [])
+    sig["ein_gemm_tuning_count"] = (_i64, [])
+    sig["ein_gemm_tuning_lookup"] = (_c_int, [ctypes.c_int32, _i64, _i64, _i64, _i64, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.POINTER(ctypes.c_int32)])
+    sig["ein_gemm_tuning_save"] = (_c_int, [ctypes.c_char_p])
+    sig["ein_gemm_tuning_load"] = (_c_int, [ctypes.c_char_p])
     sig["ein_dot_f64_noalloc"] = (_c_int, [_i64, ctypes.c_double, _dp, _i64, _dp, _i64, ctypes.c_double, _dp, _vp])
     for sfx, T in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
         sig[f"ein_gemm_{sfx}"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, T, _dp, _i64, _dp, _i64, T,

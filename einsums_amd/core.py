@@ -27,7 +27,8 @@ __all__ = [
     "compile_plan", "EinsumGenericPlan", "EinsumDotPlan", "EinsumDirectProductPlan", "EinsumGerPlan",
     "EinsumGemvPlan", "EinsumGemmPlan", "GPUView", "COPY", "MAP", "DEVICE_TENSOR", "gpu_enabled", "einsum",
     "permute", "stream_handle", "DimensionError", "RankError", "EinsumsError", "contraction_route", "syev", "gesv",
-    "gemm_host_c", "fock_jk",
+    "gemm_host_c", "fock_jk", "gemm_tuning", "save_gemm_tuning", "load_gemm_tuning", "clear_gemm_tuning",
+    "gemm_tuning_lookup",
 ]
 
 # GPUView modes, numbered as the reference's PyViewMode
@@ -482,6 +483,39 @@ def contraction_route(route: str):
         yield
     finally:
         lib.ein_set_contraction_route(prev)
+
+
+@contextlib.contextmanager
+def gemm_tuning():
+    """Tuning mode for the MFMA GEMM (every GEMM-mapped einsum and ``gemm`` call inside the
+    block): a problem not yet in the tuning table is timed over all candidate
+    configurations on its own operands and the fastest is stored (``save_gemm_tuning``)."""
+    prev = lib.ein_gemm_tuning_mode(1)
+    try:
+        yield
+    finally:
+        lib.ein_gemm_tuning_mode(prev)
+
+
+def save_gemm_tuning(path) -> None:
+    check(lib.ein_gemm_tuning_save(str(path).encode()), "save gemm tuning")
+
+
+def load_gemm_tuning(path) -> None:
+    check(lib.ein_gemm_tuning_load(str(path).encode()), "load gemm tuning")
+
+
+def clear_gemm_tuning() -> None:
+    check(lib.ein_gemm_tuning_clear(), "clear gemm tuning")
+
+
+def gemm_tuning_lookup(elem_bytes: int, m: int, n: int, k: int, batch: int, a_kcontig: bool, b_kcontig: bool):
+    """The stored configuration {bm, bn, data_parallel, blocks_per_cu} of a normalised problem
+    (C contiguous along m), or None."""
+    cfg = (ctypes.c_int32 * 4)()
+    if lib.ein_gemm_tuning_lookup(elem_bytes, m, n, k, batch, int(a_kcontig), int(b_kcontig), cfg) != 0:
+        return None
+    return {"bm": cfg[0], "bn": cfg[1], "data_parallel": cfg[2], "blocks_per_cu": cfg[3]}
 
 
 def algorithm_name(alg: int) -> str:

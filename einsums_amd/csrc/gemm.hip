@@ -8,8 +8,12 @@
 // both land here. Reference leaf replaced: blas::gpu::gemm -> hipblasDgemm_64
 // (libs/Einsums/hipBLAS/include/Einsums/hipBLAS.hpp:64, hipBLASVendor/src/gemm.hip:82-155).
 #include <algorithm>
+#include <cstdio>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
+#include <vector>
 
 #include "gemm.hpp"
 #include "tile_engine.hpp"
@@ -309,7 +313,115 @@ GemmConfig choose_gemm_config(int64_t m, int64_t n, int64_t k, int64_t batch, in
     return cfg;
 }
 
+// ------------------------------------------------------------- tuning table
+// Measured GEMM configurations, keyed by the normalised problem (C's unit-stride direction
+// along i): element size, m, n, k, batch and whether A and B are contiguous along k (their
+// staging class). A call whose key is in the table runs the stored configuration; with tuning
+// mode on, a call whose key is absent first times every candidate on its own operands
+// (writing a scratch C, beta = 0) and stores the fastest. The table can be saved to and
+// loaded from a text file (one "elem m n k batch ak bk : bm bn dp bpc" line per entry).
+struct TuneKey {
+    int eb;
+    int64_t m, n, k, batch;
+    int ak, bk;
+    bool operator<(TuneKey const &o) const {
+        return std::tie(eb, m, n, k, batch, ak, bk) < std::tie(o.eb, o.m, o.n, o.k, o.batch, o.ak, o.bk);
+    }
+};
+
 namespace {
+std::mutex g_tune_mutex;
+std::map<TuneKey, GemmConfig> g_tune;
+bool g_tuning = false;
+
+template <typename T>
+TuneKey tune_key(GemmProblem<T> const &P) {
+    return TuneKey{int(sizeof(T)), P.m, P.n, P.k, P.batch, P.a_sk == 1 ? 1 : 0, P.b_sk == 1 ? 1 : 0};
+}
+
+// Candidate configurations: tile shape x schedule (hybrid stream-K or data-parallel) x
+// workgroups per CU the schedule plans for (all that fit, or one).
+std::vector<GemmConfig> tune_candidates() {
+    std::vector<GemmConfig> c;
+    for (int bm : {128, 64})
+        for (int bn : {128, 64})
+            for (int dp : {0, 1})
+                for (int bpc : {0, 1}) {
+                    if (dp == 1 && bpc == 1) continue; // data-parallel ignores the planned occupancy
+                    GemmConfig g;
+                    g.bm            = bm;
+                    g.bn            = bn;
+                    g.data_parallel = dp;
+                    g.blocks_per_cu = bpc;
+                    c.push_back(g);
+                }
+    return c;
+}
+} // namespace
+
+bool tuned_config(TuneKey const &key, GemmConfig *out) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+namespace {
+
+template <typename T>
+int gemm_strided_impl(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream, size_t *ws_bytes);
+
+// Times every candidate on P's operands and returns the fastest. C is a scratch buffer with
+// P's beta (its values do not matter for the timing, the C traffic does). Each candidate is
+// timed in two passes, the candidate list in forward then reverse order (so clock drift over
+// the tuning run does not favour the early or the late candidates), each pass running it
+// for about 10 ms (3 to 200 launches) after a warm-up; its time is the smaller of the two.
+// P is normalised.
+template <typename T>
+int tune_gemm(GemmProblem<T> const &P, hipStream_t stream, GemmConfig *best) {
+    size_t const c_elems = size_t(P.m) * size_t(P.n) * size_t(P.batch);
+    void *scratch        = nullptr;
+    EIN_TRY(ein_malloc(&scratch, c_elems * sizeof(T)));
+    GemmProblem<T> Q = P;
+    Q.C              = static_cast<T *>(scratch);
+    Q.c_sm           = 1;
+    Q.c_sn           = P.m;
+    Q.c_sb           = P.m * P.n;
+    Q.beta           = P.beta;
+    std::vector<GemmConfig> const cands = tune_candidates();
+    std::vector<float> best_of(cands.size(), 0.f);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = EIN_SUCCESS;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = fail(EIN_ERROR_HIP, "tuning events");
+    auto timed = [&](GemmConfig const &c, int reps, float *ms) {
+        (void)hipEventRecord(e0, stream);
+        for (int r = 0; r < reps && st == EIN_SUCCESS; ++r) st = gemm_strided_impl<T>(Q, &c, stream, nullptr);
+        (void)hipEventRecord(e1, stream);
+        if (st == EIN_SUCCESS && hipEventSynchronize(e1) != hipSuccess) st = fail(EIN_ERROR_HIP, "tuning synchronisation");
+        if (st == EIN_SUCCESS) (void)hipEventElapsedTime(ms, e0, e1);
+    };
+    for (int pass = 0; pass < 2 && st == EIN_SUCCESS; ++pass)
+        for (size_t q = 0; q < cands.size() && st == EIN_SUCCESS; ++q) {
+            size_t const i = pass == 0 ? q : cands.size() - 1 - q;
+            float first = 0.f, ms = 0.f;
+            timed(cands[i], 1, &first); // warm-up, and the length of one launch
+            int const reps = std::max(3, std::min(200, int(10.0f / std::max(first, 1e-3f))));
+            timed(cands[i], reps, &ms);
+            ms /= float(reps);
+            if (best_of[i] == 0.f || ms < best_of[i]) best_of[i] = ms;
+        }
+    if (st == EIN_SUCCESS) {
+        size_t arg = 0;
+        for (size_t i = 1; i < cands.size(); ++i)
+            if (best_of[i] < best_of[arg]) arg = i;
+        *best = cands[arg];
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)ein_free_async(scratch, reinterpret_cast<ein_stream_t>(stream));
+    return st;
+}
 
 template <typename T>
 int gemm_strided_impl(GemmProblem<T> P, GemmConfig const *force, hipStream_t stream, size_t *ws_bytes) {
@@ -342,6 +454,37 @@ int gemm_strided_impl(GemmProblem<T> P, GemmConfig const *force, hipStream_t str
         EIN_LAUNCH_CHECK();
         return EIN_SUCCESS;
     }
+    GemmConfig tuned;
+    bool have_tuned = false;
+    if (!force) {
+        TuneKey const key = tune_key(P);
+        have_tuned        = tuned_config(key, &tuned);
+        bool tuning;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            tuning = g_tuning;
+        }
+        if (!have_tuned && tuning) {
+            if (ws_bytes != nullptr) {
+                // an untuned key in tuning mode may run any candidate: report the largest need
+                size_t most = 0;
+                for (GemmConfig const &c : tune_candidates()) {
+                    size_t b = 0;
+                    EIN_TRY(gemm_strided_impl<T>(P, &c, stream, &b));
+                    most = std::max(most, b);
+                }
+                *ws_bytes = most;
+                return EIN_SUCCESS;
+            }
+            if (!stream_capturing(stream)) {
+                EIN_TRY(tune_gemm<T>(P, stream, &tuned));
+                std::lock_guard<std::mutex> lock(g_tune_mutex);
+                g_tune[key] = tuned;
+                have_tuned  = true;
+            }
+        }
+    }
+    if (have_tuned) force = &tuned;
     GemmConfig cfg = force ? *force : choose_gemm_config(P.m, P.n, P.k, P.batch, int(sizeof(T)));
     if (!force && cfg.bm == 128 && cfg.bn == 64) {
         // few tiles with both operands staged row-major (contiguous along k): 64 x 64
@@ -564,6 +707,83 @@ int ein_gemm_strided_batched_f32(char transa, char transb, int64_t m, int64_t n,
                                  float *C, int64_t ldc, int64_t strideC, int64_t batch, ein_stream_t stream) {
     return ein::gemm_colmajor<float>(transa, transb, m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
                                      strideC, batch, ein::as_stream(stream));
+}
+
+int ein_gemm_tuning_mode(int32_t on) {
+    std::lock_guard<std::mutex> lock(ein::g_tune_mutex);
+    int32_t const prev = ein::g_tuning ? 1 : 0;
+    ein::g_tuning      = on != 0;
+    return prev;
+}
+
+int ein_gemm_tuning_clear(void) {
+    std::lock_guard<std::mutex> lock(ein::g_tune_mutex);
+    ein::g_tune.clear();
+    return EIN_SUCCESS;
+}
+
+int64_t ein_gemm_tuning_count(void) {
+    std::lock_guard<std::mutex> lock(ein::g_tune_mutex);
+    return int64_t(ein::g_tune.size());
+}
+
+int ein_gemm_tuning_lookup(int32_t elem_bytes, int64_t m, int64_t n, int64_t k, int64_t batch, int32_t a_kcontig,
+                           int32_t b_kcontig, int32_t *config) {
+    if (config == nullptr) return -8;
+    ein::GemmConfig c;
+    if (!ein::tuned_config(ein::TuneKey{elem_bytes, m, n, k, batch, a_kcontig ? 1 : 0, b_kcontig ? 1 : 0}, &c))
+        return ein::fail(EIN_ERROR_VALUE, "gemm tuning: no entry for this problem");
+    config[0] = c.bm;
+    config[1] = c.bn;
+    config[2] = c.data_parallel;
+    config[3] = c.blocks_per_cu;
+    return EIN_SUCCESS;
+}
+
+int ein_gemm_tuning_save(const char *path) {
+    if (path == nullptr) return -1;
+    FILE *f = std::fopen(path, "w");
+    if (f == nullptr) return ein::fail(EIN_ERROR_VALUE, std::string("gemm tuning: cannot write ") + path);
+    std::lock_guard<std::mutex> lock(ein::g_tune_mutex);
+    std::fprintf(f, "# einsums_amd GEMM tuning table: elem m n k batch a_kcontig b_kcontig : bm bn data_parallel blocks_per_cu\n");
+    for (auto const &kv : ein::g_tune)
+        std::fprintf(f, "%d %lld %lld %lld %lld %d %d : %d %d %d %d\n", kv.first.eb, (long long)kv.first.m,
+                     (long long)kv.first.n, (long long)kv.first.k, (long long)kv.first.batch, kv.first.ak, kv.first.bk,
+                     kv.second.bm, kv.second.bn, kv.second.data_parallel, kv.second.blocks_per_cu);
+    std::fclose(f);
+    return EIN_SUCCESS;
+}
+
+int ein_gemm_tuning_load(const char *path) {
+    if (path == nullptr) return -1;
+    FILE *f = std::fopen(path, "r");
+    if (f == nullptr) return ein::fail(EIN_ERROR_VALUE, std::string("gemm tuning: cannot read ") + path);
+    std::map<ein::TuneKey, ein::GemmConfig> loaded;
+    char line[512];
+    int lineno = 0;
+    while (std::fgets(line, sizeof line, f) != nullptr) {
+        ++lineno;
+        if (line[0] == '#' || line[0] == '\n') continue;
+        int eb, ak, bk, bm, bn, dp, bpc;
+        long long m, n, k, b;
+        if (std::sscanf(line, "%d %lld %lld %lld %lld %d %d : %d %d %d %d", &eb, &m, &n, &k, &b, &ak, &bk, &bm, &bn, &dp,
+                        &bpc) != 11 ||
+            (eb != 4 && eb != 8) || (bm != 64 && bm != 128) || (bn != 64 && bn != 128) || dp < 0 || dp > 1 || bpc < 0 ||
+            m <= 0 || n <= 0 || k <= 0 || b <= 0) {
+            std::fclose(f);
+            return ein::fail(EIN_ERROR_VALUE, "gemm tuning: malformed line " + std::to_string(lineno) + " in " + path);
+        }
+        ein::GemmConfig c;
+        c.bm            = bm;
+        c.bn            = bn;
+        c.data_parallel = dp;
+        c.blocks_per_cu = bpc;
+        loaded[ein::TuneKey{eb, m, n, k, b, ak ? 1 : 0, bk ? 1 : 0}] = c;
+    }
+    std::fclose(f);
+    std::lock_guard<std::mutex> lock(ein::g_tune_mutex);
+    for (auto const &kv : loaded) ein::g_tune[kv.first] = kv.second;
+    return EIN_SUCCESS;
 }
 
 } // extern "C"

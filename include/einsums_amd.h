@@ -143,6 +143,24 @@ int ein_axpby_f64(int64_t n, double alpha, const double *x, int64_t incx, double
 int ein_axpby_f32(int64_t n, float alpha, const float *x, int64_t incx, float beta, float *y, int64_t incy,
                   ein_stream_t stream);
 
+/* Measured GEMM configurations (the autotuner). The MFMA GEMM behind ein_gemm_* and every
+ * GEMM-mapped einsum looks its problem up in a process-wide table keyed by the normalised
+ * problem (element size, m, n, k, batch, whether A and B are contiguous along k); an entry
+ * overrides the built-in tile heuristic. With tuning mode on, a GEMM whose key is absent
+ * first times every candidate (tile 128/64 x 128/64, hybrid stream-K or data-parallel,
+ * planned workgroups per CU) on its own operands, writing a scratch C, and stores the
+ * fastest (not while its stream is captured). No counterpart in the reference (its GEMMs
+ * are vendor BLAS calls). The table is saved and loaded as text.
+ * ein_gemm_tuning_mode returns the previous mode; ein_gemm_tuning_lookup writes
+ * {bm, bn, data_parallel, blocks_per_cu} to config[0..3]. */
+int ein_gemm_tuning_mode(int32_t on);
+int ein_gemm_tuning_clear(void);
+int64_t ein_gemm_tuning_count(void);
+int ein_gemm_tuning_lookup(int32_t elem_bytes, int64_t m, int64_t n, int64_t k, int64_t batch, int32_t a_kcontig,
+                           int32_t b_kcontig, int32_t *config);
+int ein_gemm_tuning_save(const char *path);
+int ein_gemm_tuning_load(const char *path);
+
 /* Split-precision FP64 GEMM (Ozaki scheme): op(A), op(B) are cut into `nslices` integer
  * slices with per-row / per-column power-of-two scales, multiplied on MFMA with exact
  * accumulation and recombined in FP64.

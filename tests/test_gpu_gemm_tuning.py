@@ -1,0 +1,48 @@
+"""The GEMM autotuner on the GPU: tuning mode times the candidate configurations of a new
+problem on its own operands and stores the fastest; the tuned call and later calls match
+the oracle, repeat bit for bit, and survive a save / clear / load of the table."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from tests.helpers import dev, host, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tuning_mode_on_gemm_and_einsum(core, oracle, tmp_path):
+    rng = np.random.default_rng(55)
+    core.clear_gemm_tuning()
+    # Wmnij-shaped einsum (576 x 576 x 9216 at o=24, v=96 is too slow for the oracle here:
+    # o=12, v=48 gives 144 x 144 x 2304) and a plain GEMM with uneven extents
+    t = rng.uniform(-1, 1, (12, 12, 48, 48))
+    g = rng.uniform(-1, 1, (12, 12, 48, 48))
+    W0 = rng.uniform(-1, 1, (12, 12, 12, 12))
+    A = rng.uniform(-1, 1, (700, 333))
+    B = rng.uniform(-1, 1, (333, 1100))
+    dt, dg, dA, dB = dev(t), dev(g), dev(A), dev(B)
+
+    def run():
+        W = dev(W0)
+        core.einsum("mnij", W, "ijef", dt, "mnef", dg, C_prefactor=1.0, AB_prefactor=0.25)
+        C = torch.zeros((700, 1100), dtype=torch.float64, device="cuda")
+        core.gemm("N", "N", 1.0, dA, dB, 0.0, C)
+        return host(W), host(C)
+
+    with core.gemm_tuning():
+        W1, C1 = run()
+    assert core.lib.ein_gemm_tuning_count() == 2
+    Wr = W0.copy()
+    oracle.einsum(1.0, "mnij", Wr, 0.25, "ijef", t, "mnef", g)
+    assert rel_fro(W1, Wr) < 1e-13 and rel_fro(C1, A @ B) < 1e-13
+    W2, C2 = run()  # the stored configurations, outside tuning mode
+    assert np.array_equal(W1, W2) and np.array_equal(C1, C2)
+    f = tmp_path / "tuned.txt"
+    core.save_gemm_tuning(f)
+    core.clear_gemm_tuning()
+    core.load_gemm_tuning(f)
+    assert core.lib.ein_gemm_tuning_count() == 2
+    W3, C3 = run()
+    assert np.array_equal(W1, W3) and np.array_equal(C1, C3)
+    core.clear_gemm_tuning()
