@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import itertools
+import weakref
 from typing import Iterable, Sequence
 
 import torch
@@ -117,6 +118,7 @@ class TiledTensor:
         self.rank = len(self.tile_dims)
         self.tiles: dict[tuple, torch.Tensor] = {}
         self.dtype, self.device = dtype, device
+        self.version = 0  # bumped whenever the set of tiles changes (plan cache key)
 
     def tile_shape(self, coord: tuple) -> tuple:
         return tuple(self.tile_dims[d][coord[d]] for d in range(self.rank))
@@ -127,6 +129,7 @@ class TiledTensor:
         if tuple(t.shape) != shape:
             raise ValueError(f"tile {coord} must have shape {shape}")
         self.tiles[tuple(coord)] = t
+        self.version += 1
         return t
 
     def has_tile(self, coord: tuple) -> bool:
@@ -225,14 +228,32 @@ def tiled_terms(C_prefactor, C_indices: str, C, AB_prefactor, A_indices: str, A,
             if grid.setdefault(ch, n) != n:
                 raise _lib.DimensionError(f"tile grids disagree on index {ch!r}")
     scale = list(C.tiles.values()) if kinds[0] == "tiled" else (C.blocks if kinds[0] == "block" else [])
+    # Join of the present A and B tiles on their shared labels (instead of a walk over the
+    # whole tile grid: with irrep-blocked tensors most grid positions have no tile), then the
+    # labels found only in C; the triples are ordered as the grid walk of TileAlgebra.hpp:59-92
+    # would meet them (lexicographic over the distinct labels), so the accumulation order into
+    # every C tile is the reference's.
+    shared = [ch for ch in dict.fromkeys(A_indices) if ch in B_indices]
+    c_only = [ch for ch in dict.fromkeys(C_indices) if ch not in A_indices and ch not in B_indices]
+    by_key = {}
+    for bc, tb in views["B"][1].items():
+        by_key.setdefault(tuple(bc[B_indices.index(ch)] for ch in shared), []).append((bc, tb))
+    found = []
+    for ac, ta in views["A"][1].items():
+        if any(ac[d] != ac[A_indices.index(ch)] for d, ch in enumerate(A_indices)):
+            continue  # a repeated label takes one tile index
+        for bc, tb in by_key.get(tuple(ac[A_indices.index(ch)] for ch in shared), ()):
+            if any(bc[d] != bc[B_indices.index(ch)] for d, ch in enumerate(B_indices)):
+                continue
+            pos = {ch: ac[A_indices.index(ch)] for ch in A_indices}
+            pos.update({ch: bc[B_indices.index(ch)] for ch in B_indices})
+            for extra in itertools.product(*[range(grid[ch]) for ch in c_only]):
+                pos.update(zip(c_only, extra))
+                found.append((tuple(pos[ch] for ch in labels), ta, tb))
+    found.sort(key=lambda f: f[0])
     terms = []
-    for combo in itertools.product(*[range(grid[ch]) for ch in labels]):
+    for combo, ta, tb in found:
         pos = dict(zip(labels, combo))
-        ac = tuple(pos[ch] for ch in A_indices)
-        bc = tuple(pos[ch] for ch in B_indices)
-        ta, tb = views["A"][1].get(ac), views["B"][1].get(bc)
-        if ta is None or tb is None:
-            continue
         tc = None
         if kinds[0] != "scalar":
             cc = tuple(pos[ch] for ch in C_indices)
@@ -272,8 +293,26 @@ def _scalar_einsum(C_prefactor, C: torch.Tensor, terms: list, autotune, stream) 
     return plan
 
 
+# Plans of earlier block/tiled einsums, reused while the same operand objects with the same
+# tile sets come back (the reference rebuilds its tile loop on every call).
+_PLAN_CACHE: dict = {}
+_PLAN_CACHE_MAX = 64
+
+
+def _cache_key(C_prefactor, C_indices, C, AB_prefactor, A_indices, A, B_indices, B, autotune):
+    def ident(X):
+        if isinstance(X, TiledTensor):
+            return ("tiled", id(X), X.version)
+        if isinstance(X, BlockTensor):
+            return ("block", id(X), tuple(b.data_ptr() for b in X.blocks))
+        return ("tensor", id(X), X.data_ptr(), tuple(X.shape), tuple(X.stride()), X.dtype)
+
+    return (float(C_prefactor), C_indices, ident(C), float(AB_prefactor), A_indices, ident(A), B_indices, ident(B),
+            bool(autotune))
+
+
 def einsum(C_prefactor, C_indices: str, C, AB_prefactor, A_indices: str, A, B_indices: str, B,
-           stream=None, autotune: bool = False) -> GroupedEinsum | None:
+           stream=None, autotune: bool = False, cache: bool = True) -> GroupedEinsum | None:
     """Block/tiled einsum in one grouped plan; returns the plan so it can be replayed (None
     when nothing is left to contract). Operand kinds, as the reference's special dispatch:
       * BlockTensor x BlockTensor -> BlockTensor, dense tensor or scalar (BlockAlgebra.hpp:23-180);
@@ -281,33 +320,47 @@ def einsum(C_prefactor, C_indices: str, C, AB_prefactor, A_indices: str, A, B_in
         (the dense operand cut by every block's range; BlockAlgebra.hpp:182-428);
       * TiledTensor x TiledTensor -> TiledTensor or scalar (TileAlgebra.hpp:18-158);
       * any mix of BlockTensor and TiledTensor (BlockTileAlgebra.hpp:18-173).
-    A scalar C is a 0-dimensional device tensor."""
+    A scalar C is a 0-dimensional device tensor. With ``cache`` (the default) the triple list
+    and its device-resident plan are built once per operand set and reused by later calls
+    with the same operand objects, labels, prefactors and tile sets."""
     kinds = (_kind(C), _kind(A), _kind(B))
-    if "tiled" in kinds:
-        if "dense" in kinds:
-            raise TypeError("blocks.einsum: dense operands combine with BlockTensors only")
-        terms, scale = tiled_terms(C_prefactor, C_indices, C, AB_prefactor, A_indices, A, B_indices, B)
+    if "tiled" not in kinds and kinds[1] != "block" and kinds[2] != "block":
+        raise TypeError("blocks.einsum expects BlockTensor or TiledTensor operands")
+    if "tiled" in kinds and "dense" in kinds:
+        raise TypeError("blocks.einsum: dense operands combine with BlockTensors only")
+    key = None
+    entry = None
+    if cache and kinds[0] != "scalar":
+        key = _cache_key(C_prefactor, C_indices, C, AB_prefactor, A_indices, A, B_indices, B, autotune)
+        entry = _PLAN_CACHE.get(key)
+        if entry is not None and not all(ref() is X for ref, X in zip(entry[0], (C, A, B))):
+            entry = None  # an id was reused by another object
+    if entry is None:
+        if "tiled" in kinds:
+            terms, scale = tiled_terms(C_prefactor, C_indices, C, AB_prefactor, A_indices, A, B_indices, B)
+        else:
+            terms, scale = block_terms(C_prefactor, C_indices, C, AB_prefactor, A_indices, A, B_indices, B), []
         if kinds[0] == "scalar":
             return _scalar_einsum(C_prefactor, C, terms, autotune, stream)
+        plan = GroupedEinsum(terms, autotune, stream) if terms else None
+        if key is not None:
+            if isinstance(C, TiledTensor):  # tiles created by the triple list are part of the key
+                key = _cache_key(C_prefactor, C_indices, C, AB_prefactor, A_indices, A, B_indices, B, autotune)
+            if len(_PLAN_CACHE) >= _PLAN_CACHE_MAX:
+                _PLAN_CACHE.pop(next(iter(_PLAN_CACHE)))
+            _PLAN_CACHE[key] = ((weakref.ref(C), weakref.ref(A), weakref.ref(B)), plan, bool(scale))
+    else:
+        plan = entry[1]
+        # every C tile or block present now, the ones the first call created included
+        scale = (list(C.tiles.values()) if isinstance(C, TiledTensor) else list(C.blocks)) if entry[2] else []
+    if scale:
         from .core import _fn
         for t in scale:  # C scaled once by C_prefactor, then every tile triple accumulates
             d = make_desc("".join(chr(97 + i) for i in range(t.dim())), t.shape, t.stride())
             check(_fn("ein_tensor_scale", t.dtype)(ctypes.byref(d), _ptr(t), float(C_prefactor), stream_handle(stream)))
-        if not terms:
-            return None
-        plan = GroupedEinsum(terms, autotune, stream)
+    if plan is not None:
         plan.execute(stream)
-        return plan
-    if kinds[1] == "block" or kinds[2] == "block":
-        terms = block_terms(C_prefactor, C_indices, C, AB_prefactor, A_indices, A, B_indices, B)
-        if kinds[0] == "scalar":
-            return _scalar_einsum(C_prefactor, C, terms, autotune, stream)
-        if not terms:
-            return None
-        plan = GroupedEinsum(terms, autotune, stream)
-        plan.execute(stream)
-        return plan
-    raise TypeError("blocks.einsum expects BlockTensor or TiledTensor operands")
+    return plan
 
 
 class Graph:

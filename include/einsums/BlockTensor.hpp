@@ -11,6 +11,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cstdint>
 #include <map>
 #include <vector>
 
@@ -73,6 +74,12 @@ class TiledDeviceTensor {
     }
     DeviceTensor<T, Rank> const &tile(Coord const &c) const { return tiles_.at(c); }
     std::map<Coord, DeviceTensor<T, Rank>> &tiles() { return tiles_; }
+    // Calls f(coordinate) for every present tile of non-zero size.
+    template <typename F>
+    void for_each_tile(F &&f) const {
+        for (auto const &kv : tiles_)
+            if (kv.second.size() > 0) f(kv.first);
+    }
 
   private:
     std::string name_;
@@ -245,6 +252,15 @@ struct BlockTiles {
         return b->block_dim(size_t(c[0])) > 0;
     }
     DeviceTensor<T, R> const &tile(Coord const &c) const { return b->block(size_t(c[0])); }
+    template <typename F>
+    void for_each_tile(F &&f) const {
+        for (size_t i = 0; i < b->num_blocks(); ++i)
+            if (b->block_dim(i) > 0) {
+                Coord c;
+                c.fill(int(i));
+                f(c);
+            }
+    }
 };
 
 template <typename T, size_t R>
@@ -328,21 +344,58 @@ std::vector<ein_contraction_item> tile_items(T AB_prefactor, std::array<int32_t,
     for (size_t d = 0; d < NA; ++d) add(a_ids[d], A.grid_size(d));
     for (size_t d = 0; d < NB; ++d) add(b_ids[d], B.grid_size(d));
     auto pos_of = [&](int32_t id) { return size_t(std::find(labels.begin(), labels.end(), id) - labels.begin()); };
-    std::vector<ein_contraction_item> items;
-    std::vector<size_t> combo(labels.size(), 0);
-    size_t total = 1;
-    for (auto n : grid) total *= n;
-    for (size_t lin = 0; lin < total; ++lin) {
-        size_t rem = lin;
-        for (size_t q = labels.size(); q-- > 0;) {
-            combo[q] = rem % grid[q];
-            rem /= grid[q];
+    // Join of the present A and B tiles on their shared labels (no walk over the whole tile
+    // grid: with irrep-blocked tensors most positions have no tile), then the labels found only
+    // in C; the triples are sorted into the order in which the grid walk of
+    // TileAlgebra.hpp:59-92 meets them, so every C tile accumulates in the reference's order.
+    std::vector<std::array<int, NA>> a_tiles;
+    std::vector<std::array<int, NB>> b_tiles;
+    A.for_each_tile([&](std::array<int, NA> const &c) { a_tiles.push_back(c); });
+    B.for_each_tile([&](std::array<int, NB> const &c) { b_tiles.push_back(c); });
+    std::vector<size_t> c_only;
+    for (size_t d = 0; d < NC; ++d)
+        if (C != nullptr && std::find(a_ids.begin(), a_ids.end(), c_ids[d]) == a_ids.end() &&
+            std::find(b_ids.begin(), b_ids.end(), c_ids[d]) == b_ids.end() &&
+            std::find(c_only.begin(), c_only.end(), pos_of(c_ids[d])) == c_only.end())
+            c_only.push_back(pos_of(c_ids[d]));
+    std::vector<std::vector<size_t>> combos;
+    for (auto const &ac : a_tiles) {
+        std::vector<size_t> combo(labels.size(), SIZE_MAX);
+        bool ok = true;
+        auto set = [&](size_t q, size_t v) {
+            if (combo[q] != SIZE_MAX && combo[q] != v) ok = false;
+            combo[q] = v;
+        };
+        for (size_t d = 0; d < NA; ++d) set(pos_of(a_ids[d]), size_t(ac[d]));
+        if (!ok) continue;
+        for (auto const &bc : b_tiles) {
+            std::vector<size_t> cb = combo;
+            bool okb = true;
+            for (size_t d = 0; d < NB; ++d) {
+                size_t const q = pos_of(b_ids[d]);
+                if (cb[q] != SIZE_MAX && cb[q] != size_t(bc[d])) okb = false;
+                cb[q] = size_t(bc[d]);
+            }
+            if (!okb) continue;
+            size_t extra = 1;
+            for (size_t q : c_only) extra *= grid[q];
+            for (size_t e = 0; e < extra; ++e) {
+                size_t rem = e;
+                for (size_t x = c_only.size(); x-- > 0;) {
+                    cb[c_only[x]] = rem % grid[c_only[x]];
+                    rem /= grid[c_only[x]];
+                }
+                combos.push_back(cb);
+            }
         }
+    }
+    std::sort(combos.begin(), combos.end());
+    std::vector<ein_contraction_item> items;
+    for (auto const &combo : combos) {
         std::array<int, NA> ac;
         std::array<int, NB> bc;
         for (size_t d = 0; d < NA; ++d) ac[d] = int(combo[pos_of(a_ids[d])]);
         for (size_t d = 0; d < NB; ++d) bc[d] = int(combo[pos_of(b_ids[d])]);
-        if (!A.has_tile(ac) || !B.has_tile(bc)) continue;
         auto const &ta = A.tile(ac);
         auto const &tb = B.tile(bc);
         ein_tensor_desc cd{};
@@ -365,7 +418,7 @@ std::vector<ein_contraction_item> tile_items(T AB_prefactor, std::array<int32_t,
                 auto &tc         = C->tile(cc); // created when absent
                 if (fresh) tc.zero();
                 cd = tc.desc(c_ids.data());
-                cptr     = tc.data();
+                cptr = tc.data();
             }
         }
         items.push_back(detail::make_item(cd, cptr, ta.desc(a_ids.data()), ta.data(), tb.desc(b_ids.data()), tb.data(), 1.0,

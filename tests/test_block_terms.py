@@ -3,6 +3,7 @@ an einsum over BlockTensors, TiledTensors and dense tensors becomes, checked by 
 each einsum from its terms with numpy.einsum and comparing with numpy.einsum of the dense
 tensors (absent tiles zero, a dense C changed only on its diagonal blocks)."""
 import itertools
+import zlib
 
 import numpy as np
 import pytest
@@ -84,3 +85,47 @@ def test_tiled_terms_create_missing_c_tiles_and_mix_blocks():
     E = blocks.TiledTensor("E", [dims, dims], device="cpu")
     terms, scale = blocks.tiled_terms(0.0, "ij", E, 1.0, "ik", D, "kj", B if False else D)
     assert len(terms) == 2 and all(t[3].shape == t[5].shape for t in terms)
+
+
+def brute_force_triples(C_idx, C, A_idx, A, B_idx, B):
+    """The reference's walk over the whole tile grid (TileAlgebra.hpp:59-92)."""
+    labels = list(dict.fromkeys(C_idx + A_idx + B_idx))
+    grid = {}
+    for T, idx in ((C, C_idx), (A, A_idx), (B, B_idx)):
+        for d, ch in enumerate(idx):
+            grid[ch] = T.grid_size(d)
+    out = []
+    for combo in itertools.product(*[range(grid[ch]) for ch in labels]):
+        pos = dict(zip(labels, combo))
+        ac = tuple(pos[ch] for ch in A_idx)
+        bc = tuple(pos[ch] for ch in B_idx)
+        if A.has_tile(ac) and B.has_tile(bc):
+            cc = tuple(pos[ch] for ch in C_idx)
+            if all(e > 0 for e in C.tile_shape(cc)):
+                out.append((cc, ac, bc))
+    return out
+
+
+@pytest.mark.parametrize("pattern", [("ij", "ik", "kj"), ("abij", "abef", "efij"), ("ijc", "iik", "kjj"),
+                                     ("ijc", "ik", "kj"), ("ab", "ab", "ab")])
+def test_tile_join_matches_grid_walk(pattern):
+    """The join of present tiles gives the grid walk's triples in the grid walk's order, for
+    sparse (irrep-like) tilings, repeated labels and labels found only in C."""
+    rng = np.random.default_rng(zlib.crc32("|".join(pattern).encode()))
+    ci, ai, bi = pattern
+    ext = {ch: [int(x) for x in rng.integers(0, 4, 3)] for ch in set(ci + ai + bi)}
+
+    def make(name, idx, density):
+        T = blocks.TiledTensor(name, [ext[ch] for ch in idx], device="cpu")
+        for c in itertools.product(*[range(3)] * len(idx)):
+            if rng.uniform() < density:
+                T.add_tile(c, torch.zeros(T.tile_shape(c), dtype=torch.float64))
+        return T
+
+    A, B = make("A", ai, 0.4), make("B", bi, 0.4)
+    C = make("C", ci, 1.0)  # every C tile present: the walk and the join meet the same tiles
+    want = brute_force_triples(ci, C, ai, A, bi, B)
+    terms, _ = blocks.tiled_terms(1.0, ci, C, 1.0, ai, A, bi, B)
+    ident = {id(t): c for T in (A, B, C) for c, t in T.tiles.items()}
+    got = [(ident[id(t[1])], ident[id(t[3])], ident[id(t[5])]) for t in terms]
+    assert got == want

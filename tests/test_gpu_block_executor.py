@@ -196,3 +196,29 @@ def test_mixed_block_and_tiled(oracle):
         if n:
             r = hD[i] + full[offs[i]:offs[i + 1], offs[i]:offs[i + 1]]
             assert rel_fro(host(D.blocks[i]), r) < 1e-13, i
+
+
+def test_plan_cache_reuse_and_invalidation(oracle):
+    """A repeated tiled einsum reuses its triple list and device plan; adding a tile
+    invalidates it. Every call's result is checked against the oracle."""
+    from einsums_amd import blocks
+
+    rng = seeded("cache")
+    dims = [[9, 14, 6], [11, 7]]
+    A = tiled(blocks, "A", [dims[0], dims[1]], [(0, 0), (1, 1), (2, 0)], rng)
+    B = tiled(blocks, "B", [dims[1], dims[0]], [(0, 0), (1, 2), (0, 1)], rng)
+    C = tiled(blocks, "C", [dims[0], dims[0]], [(0, 0)], rng)
+    p1 = blocks.einsum(0.5, "ij", C, 1.0, "ik", A, "kj", B)
+    hC = host(C.to_dense())
+    p2 = blocks.einsum(0.5, "ij", C, 1.0, "ik", A, "kj", B)
+    assert p2 is p1  # the cached plan
+    ref = hC.copy()
+    oracle.einsum(0.5, "ij", ref, 1.0, "ik", host(A.to_dense()), "kj", host(B.to_dense()))
+    assert rel_fro(host(C.to_dense()), ref) < 1e-13
+    A.add_tile((2, 1), dev(rng.uniform(-1, 1, A.tile_shape((2, 1)))))
+    hC = host(C.to_dense())
+    p3 = blocks.einsum(0.5, "ij", C, 1.0, "ik", A, "kj", B)
+    assert p3 is not p1
+    ref = hC.copy()
+    oracle.einsum(0.5, "ij", ref, 1.0, "ik", host(A.to_dense()), "kj", host(B.to_dense()))
+    assert rel_fro(host(C.to_dense()), ref) < 1e-13
