@@ -247,7 +247,7 @@ def hf_block(core, nbf=48, nocc=12, reps=20):
     return out
 
 
-SPLIT_MODES = (("int8", 6), ("bf16", 5), ("fp16", 5))
+SPLIT_MODES = (("crt", 13), ("int8", 6), ("bf16", 5), ("fp16", 5))
 
 
 def split_large(core, o=48, v=256, reps=3):

@@ -25,7 +25,7 @@ EIN_ERROR_VALUE = 1007
 
 # detail::AlgorithmChoice numbering (TensorAlgebra/Detail/Utilities.hpp:26-34)
 GENERIC, DOT, DIRECT, GER, GEMV, GEMM, INDETERMINATE = range(7)
-SLICE_BF16, SLICE_FP16, SLICE_INT8 = 0, 1, 2
+SLICE_BF16, SLICE_FP16, SLICE_INT8, SLICE_CRT = 0, 1, 2, 3
 ROUTE_AUTO, ROUTE_GATHER, ROUTE_TTGT = 0, 1, 2
 ALGORITHM_NAMES = {GENERIC: "GENERIC", DOT: "DOT", DIRECT: "DIRECT", GER: "GER", GEMV: "GEMV", GEMM: "GEMM",
                    INDETERMINATE: "INDETERMINATE"}
@@ -150,6 +150,7 @@ def _load():
         sig[f"ein_gesv_{sfx}"] = (_c_int, [_i64, _i64, _dp, _i64, _vp, _dp, _i64, _vp, _vp])
     sig["ein_gemm_split_f64"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64, _dp,
                                           _i64, ctypes.c_double, _dp, _i64, ctypes.c_int32, ctypes.c_int32, _vp])
+    sig["ein_split_crt_bits"] = (_c_int, [ctypes.c_int32, _i64])
     sig["ein_gemm_host_c_f64"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64,
                                            _dp, _i64, ctypes.c_double, _vp, _i64, ctypes.c_size_t, ctypes.c_int32,
                                            ctypes.c_int32, _vp])

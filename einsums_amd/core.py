@@ -160,7 +160,8 @@ class EinsumGenericPlan:
                 split: int = 0, slice_type: str = "bf16") -> None:
         """Run the plan on ``stream`` (torch's current stream when None). ``split=s`` (3..6)
         runs an FP64 GEMM-mapped contraction with the split-precision (Ozaki) GEMM, cutting
-        each operand into s slices of ``slice_type`` "bf16", "fp16" or "int8"."""
+        each operand into s slices of ``slice_type`` "bf16", "fp16" or "int8", or, with
+        ``slice_type="crt"``, uses s moduli (8..16) of the integer modular scheme."""
         if isinstance(C, (list, tuple)):
             if not (isinstance(A, (list, tuple)) and isinstance(B, (list, tuple))) or not (len(C) == len(A) == len(B)):
                 raise ValueError("The lengths of the collections need to be the same!")
@@ -186,7 +187,7 @@ class EinsumGenericPlan:
         if split:
             if dtype != torch.float64:
                 raise TypeError("split precision applies to float64 operands")
-            st = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16, "int8": _lib.SLICE_INT8}[slice_type]
+            st = _slice_code(slice_type)
             status = lib.ein_einsum_split_f64(int(split), st, float(C_prefactor), ctypes.byref(cd), _ptr(dC),
                                               float(AB_prefactor), ctypes.byref(ad), _ptr(dA), ctypes.byref(bd), _ptr(dB),
                                               ctypes.byref(used), stream_handle(stream))
@@ -390,6 +391,27 @@ def gemm_split(transA: str, transB: str, alpha: float, A: torch.Tensor, B: torch
         return _gemm_split_body(transA, transB, alpha, A, B, beta, C, nslices, slice_type)
 
 
+SLICE_TYPES = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16, "int8": _lib.SLICE_INT8, "crt": _lib.SLICE_CRT}
+
+
+def _slice_code(slice_type: str) -> int:
+    """ABI code of a split-precision slice type: "bf16", "fp16" (nslices 3..6 slices),
+    "int8" (3..6 slices) or "crt" (nslices = 8..16 moduli of the integer modular scheme)."""
+    try:
+        return SLICE_TYPES[slice_type]
+    except KeyError:
+        raise ValueError(f"slice_type must be one of {sorted(SLICE_TYPES)}, got {slice_type!r}") from None
+
+
+def crt_bits(nmoduli: int, k: int) -> int:
+    """Bits kept per operand element by the "crt" split GEMM with `nmoduli` moduli at
+    contraction length k (the relative rounding of every A and B element is 2^-bits)."""
+    b = lib.ein_split_crt_bits(int(nmoduli), int(k))
+    if b < 0:
+        raise ValueError(f"crt_bits: {nmoduli} moduli at k = {k} is not a valid CRT split")
+    return b
+
+
 def _gemm_split_body(transA, transB, alpha, A, B, beta, C, nslices, slice_type):
     stream = None  # the current stream, set by the caller
     for t in (A, B, C):
@@ -399,7 +421,7 @@ def _gemm_split_body(transA, transB, alpha, A, B, beta, C, nslices, slice_type):
     m = A.shape[1] if ta else A.shape[0]
     k = A.shape[0] if ta else A.shape[1]
     n = B.shape[0] if tb else B.shape[1]
-    st = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16, "int8": _lib.SLICE_INT8}[slice_type]
+    st = _slice_code(slice_type)
     status = lib.ein_gemm_split_f64(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0), _ptr(A),
                                     A.stride(0), float(beta), _ptr(C), C.stride(0), int(nslices), st, stream_handle(stream))
     check(status, "gemm_split")
@@ -455,7 +477,7 @@ def _gemm_host_c_body(transA, transB, alpha, A, B, beta, C, panel_bytes, split, 
     cptr = ctypes.c_void_p(C.data_ptr())
     # row-major C = op(A) op(B) is column-major C^T = op(B)^T op(A)^T: swap the operands
     if A.dtype == torch.float64:
-        st = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16, "int8": _lib.SLICE_INT8}[slice_type]
+        st = _slice_code(slice_type)
         status = lib.ein_gemm_host_c_f64(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0),
                                          _ptr(A), A.stride(0), float(beta), cptr, C.stride(0), int(panel_bytes),
                                          int(split), st, stream_handle(stream))

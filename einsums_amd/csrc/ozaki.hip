@@ -98,30 +98,6 @@ __global__ __launch_bounds__(256) void row_exp_k_fast(SliceSrc S, int32_t *__res
     if (lane == 0) e[r] = row_exponent(mx, guard);
 }
 
-// Scratch doubles row_exponents needs for `rows` rows.
-inline size_t row_exp_scratch(int64_t rows) { return size_t(rows) + 65536; }
-
-// Per-row exponents of a strided operand (e[r] with max_p |X(r,p)| < 2^e[r]).
-int row_exponents(SliceSrc const &S, int32_t *e, bool guard, double *scratch, hipStream_t stream) {
-    bool const rows_fast = S.s_row == 1 || (S.s_k != 1 && std::llabs(S.s_row) < std::llabs(S.s_k));
-    if (!rows_fast) {
-        row_exp_k_fast<<<uint32_t(ceil_div(S.rows, 4)), 256, 0, stream>>>(S, e, guard);
-        EIN_LAUNCH_CHECK();
-        return EIN_SUCCESS;
-    }
-    int64_t const rb = ceil_div(S.rows, 64);
-    int64_t nck      = std::max<int64_t>(1, std::min<int64_t>(ceil_div(int64_t(num_cus()) * 4, rb), 64));
-    nck              = std::min<int64_t>(nck, std::max<int64_t>(1, ceil_div(S.k, 256)));
-    while (nck > 1 && size_t(nck) * size_t(S.rows) > row_exp_scratch(S.rows)) --nck;
-    int64_t const kch = ceil_div(std::max<int64_t>(S.k, 1), nck);
-    if (rb > 0x7fffffff) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: too many rows");
-    row_max_partial<<<dim3(uint32_t(rb), uint32_t(nck)), 256, 0, stream>>>(S, kch, scratch);
-    EIN_LAUNCH_CHECK();
-    row_exp_finish<<<uint32_t(ceil_div(S.rows, 256)), 256, 0, stream>>>(S, scratch, int(nck), e, guard);
-    EIN_LAUNCH_CHECK();
-    return EIN_SUCCESS;
-}
-
 // ----------------------------------------------------------------- slicing
 // 64 x 64 tiles of X staged through LDS so that the read follows X's unit-stride axis
 // and the slice writes run along p (slice layout: out[t][r][kp], p fastest).
@@ -732,9 +708,38 @@ int gemm_split_typed(GemmProblem<double> P, int ns, int bm, hipStream_t stream) 
 
 } // namespace
 
+// Scratch doubles row_exponents needs for `rows` rows.
+size_t row_exp_scratch(int64_t rows) { return size_t(rows) + 65536; }
+
+// Per-row exponents of a strided operand (e[r] with max_p |X(r,p)| < 2^e[r]).
+int row_exponents(SliceSrc const &S, int32_t *e, bool guard, double *scratch, hipStream_t stream) {
+    bool const rows_fast = S.s_row == 1 || (S.s_k != 1 && std::llabs(S.s_row) < std::llabs(S.s_k));
+    if (!rows_fast) {
+        row_exp_k_fast<<<uint32_t(ceil_div(S.rows, 4)), 256, 0, stream>>>(S, e, guard);
+        EIN_LAUNCH_CHECK();
+        return EIN_SUCCESS;
+    }
+    int64_t const rb = ceil_div(S.rows, 64);
+    int64_t nck      = std::max<int64_t>(1, std::min<int64_t>(ceil_div(int64_t(num_cus()) * 4, rb), 64));
+    nck              = std::min<int64_t>(nck, std::max<int64_t>(1, ceil_div(S.k, 256)));
+    while (nck > 1 && size_t(nck) * size_t(S.rows) > row_exp_scratch(S.rows)) --nck;
+    int64_t const kch = ceil_div(std::max<int64_t>(S.k, 1), nck);
+    if (rb > 0x7fffffff) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: too many rows");
+    row_max_partial<<<dim3(uint32_t(rb), uint32_t(nck)), 256, 0, stream>>>(S, kch, scratch);
+    EIN_LAUNCH_CHECK();
+    row_exp_finish<<<uint32_t(ceil_div(S.rows, 256)), 256, 0, stream>>>(S, scratch, int(nck), e, guard);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
 int gemm_split(GemmProblem<double> P, int nslices, int slice_type, hipStream_t stream) {
     if (P.m <= 0 || P.n <= 0) return EIN_SUCCESS;
-    if (nslices < 3 || nslices > 6) return fail(EIN_ERROR_VALUE, "split gemm: nslices must be 3..6");
+    if (slice_type == EIN_SLICE_CRT) {
+        if (nslices < CRT_MIN_MODULI || nslices > 16)
+            return fail(EIN_ERROR_VALUE, "split gemm: EIN_SLICE_CRT takes 8..16 moduli");
+    } else if (nslices < 3 || nslices > 6) {
+        return fail(EIN_ERROR_VALUE, "split gemm: nslices must be 3..6");
+    }
     if (P.batch != 1) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: batched problems are not supported");
     // C's unit-stride direction must be i (lane-contiguous stores): transpose the problem otherwise
     bool const c_row_fast = (P.c_sn == 1 && P.c_sm != 1) || (P.c_sm != 1 && P.c_sn != 1 && P.c_sn < P.c_sm);
@@ -758,10 +763,11 @@ int gemm_split(GemmProblem<double> P, int nslices, int slice_type, hipStream_t s
     }
     int64_t const t128 = ceil_div(P.m, 128) * ceil_div(P.n, 64);
     int const bm = t128 >= int64_t(num_cus()) * 2 ? 128 : 64;
+    if (slice_type == EIN_SLICE_CRT) return gemm_split_crt(P, nslices, stream);
     if (slice_type == EIN_SLICE_INT8) return gemm_split_i8(P, nslices, stream);
     if (slice_type == EIN_SLICE_BF16) return gemm_split_typed<__bf16>(P, nslices, bm, stream);
     if (slice_type == EIN_SLICE_FP16) return gemm_split_typed<_Float16>(P, nslices, bm, stream);
-    return fail(EIN_ERROR_VALUE, "split gemm: slice_type must be EIN_SLICE_BF16, EIN_SLICE_FP16 or EIN_SLICE_INT8");
+    return fail(EIN_ERROR_VALUE, "split gemm: slice_type must be EIN_SLICE_BF16, EIN_SLICE_FP16, EIN_SLICE_INT8 or EIN_SLICE_CRT");
 }
 
 } // namespace ein

@@ -32,4 +32,16 @@ struct OzProblem {
 
 int gemm_split(GemmProblem<double> P, int nslices, int slice_type, hipStream_t stream);
 
+// Per-row exponents of a strided operand: e[r] with max_p |X(r,p)| < 2^e[r] (0 for a zero
+// row); with `guard` raised by one where the scaled maximum exceeds 127/128. `scratch` holds
+// row_exp_scratch(rows) doubles.
+int row_exponents(SliceSrc const &S, int32_t *e, bool guard, double *scratch, hipStream_t stream);
+size_t row_exp_scratch(int64_t rows);
+
+// CRT split GEMM (crt.hip): nmod moduli, CRT_MIN_MODULI..16.
+constexpr int CRT_MIN_MODULI = 8;
+int gemm_split_crt(GemmProblem<double> P, int nmod, hipStream_t stream);
+// Bits kept per operand element for nmod moduli and contraction length k (-1 if invalid).
+int crt_bits(int nmod, int64_t k);
+
 } // namespace ein

@@ -173,9 +173,19 @@ int ein_gemm_tuning_load(const char *path);
 #define EIN_SLICE_BF16 0
 #define EIN_SLICE_FP16 1
 #define EIN_SLICE_INT8 2
+/*   EIN_SLICE_CRT: nslices = number of moduli, 8..16 (Chinese remainder theorem, the
+ *     integer modular scheme "Ozaki scheme II"): op(A) and op(B) are rounded to kb-bit
+ *     integers per row / column, one exact int8 GEMM runs per modulus, and the product is
+ *     reconstructed exactly from the residues; kb = ein_split_crt_bits(nslices, k) is the
+ *     largest value with 4 k 2^(2 kb) <= prod(moduli), relative representation error 2^-kb
+ *     (13 moduli: kb = 44 at k = 2304, 42 at k = 65536). */
+#define EIN_SLICE_CRT 3
 int ein_gemm_split_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
                        const double *B, int64_t ldb, double beta, double *C, int64_t ldc, int32_t nslices,
                        int32_t slice_type, ein_stream_t stream);
+/* Bits kept per operand element by EIN_SLICE_CRT with nmoduli moduli at contraction length k
+ * (-1 for an invalid count or k < 0). No device work. */
+int ein_split_crt_bits(int32_t nmoduli, int64_t k);
 /* C = alpha*op(A)*op(B) + beta*C with A, B on the device and C (column-major, ldc) in host
  * memory, for outputs larger than HBM (the out-of-core case of the v^4 CCD intermediates,
  * which the reference keeps as a DiskTensor, libs/Einsums/Tensor/include/Einsums/

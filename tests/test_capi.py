@@ -151,3 +151,23 @@ def test_fock_argument_errors_without_gpu():
     assert f(129, fake, fake, 129, 2.0, -1.0, 0.0, fake, 129, None) == 1005
     assert b"exceeds 128" in _lib.lib.ein_last_error_string()
     assert f(0, None, None, 1, 2.0, -1.0, 0.0, None, 1, None) == 0  # empty: nothing to do
+
+
+def test_split_crt_bits_host_only():
+    """ein_split_crt_bits: the largest kb with 4 k 2^(2 kb) <= product of the first L moduli
+    (256, 255, 253, ...), at most 53; -1 outside 8..16 moduli or for k < 0. Checked against
+    exact integer arithmetic here; no device call."""
+    from math import gcd, prod
+
+    from einsums_amd import _lib
+
+    mods = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191]
+    assert all(gcd(a, b) == 1 for i, a in enumerate(mods) for b in mods[i + 1:])  # pairwise coprime
+    f = _lib.lib.ein_split_crt_bits
+    for L in range(8, 17):
+        M = prod(mods[:L])
+        for k in (1, 7, 64, 1000, 2304, 12288, 65536, 70000, 1 << 20):
+            kb = max(b for b in range(0, 54) if 4 * k * 2 ** (2 * b) <= M)
+            assert f(L, k) == kb, (L, k)
+    assert f(13, 2304) == 44 and f(13, 65536) == 42
+    assert f(7, 100) == -1 and f(17, 100) == -1 and f(13, -1) == -1

@@ -7,7 +7,8 @@ Cases:
   * square DGEMM n in {4096, 8192, 16384}, entries U(-1,1) and the stressed set
     U(-1,1)*2^e with e uniform in [-20, 20].
 Each runs as plain FP64, pure FP32 (operands rounded to FP32) and split precision with
-3..6 BF16, FP16 and int8 slices (DGEMM: the headline modes BF16x5, FP16x5, int8x6).
+3..6 BF16, FP16 and int8 slices and 12..13 CRT moduli (DGEMM: the headline modes BF16x5,
+FP16x5, int8x6, CRTx13).
 
 Oracle coverage (stated per case):
   * Wmnij (2304 x 2304 output) and DGEMM n <= 8192: the full oracle result;
@@ -18,7 +19,8 @@ Oracle coverage (stated per case):
 The error is relative Frobenius of the contraction part (W - W_initial) against the
 oracle's contraction on the compared rows.
 
-Tolerances (beta = bits per slice: 8 for BF16/FP16, 7 for int8; u = 2^(-beta*s) for s slices):
+Tolerances (beta = bits per slice: 8 for BF16/FP16, 7 for int8; u = 2^(-beta*s) for s slices;
+u = 2^-kb for the CRT scheme, kb = ein_split_crt_bits(moduli, K)):
   FP64 <= 1e-13; FP32 <= 1e-5;
   split: <= max(4 * u * (rho_A + rho_B), 1e-13), where rho_X is the mean over the rows of
   op(X) (a row runs along the contracted index) of max|row| / rms(row). Each row is scaled
@@ -26,8 +28,8 @@ Tolerances (beta = bits per slice: 8 for BF16/FP16, 7 for int8; u = 2^(-beta*s) 
   about u * max|row|; summed with random signs over the contraction this is a relative error
   of about u * (rho_A + rho_B) in C. rho is about 1.7 for U(-1,1) data, about 4-5 for the
   normal CCD data over 2304-65536 contracted elements and larger for the stressed set.
-  The headline modes (BF16x5, FP16x5, int8x6) must also meet the 1e-11 target on the CCD
-  and U(-1,1) data.
+  The headline modes (BF16x5, FP16x5, int8x6, CRTx13) must also meet the 1e-11 target on
+  the CCD and U(-1,1) data.
 """
 import numpy as np
 import pytest
@@ -40,11 +42,11 @@ from oracle import sampled
 pytestmark = pytest.mark.gpu
 
 O, V = 48, 256
-SPLIT = [(st, ns) for st in ("bf16", "fp16", "int8") for ns in (3, 4, 5, 6)]
+SPLIT = [(st, ns) for st in ("bf16", "fp16", "int8") for ns in (3, 4, 5, 6)] + [("crt", 12), ("crt", 13)]
 MODES = [("fp64", 0), ("fp32", 0)] + SPLIT
 
 
-HEADLINE = {("bf16", 5), ("fp16", 5), ("int8", 6)}
+HEADLINE = {("bf16", 5), ("fp16", 5), ("int8", 6), ("crt", 13)}
 
 
 def rho(X, contracted) -> float:
@@ -55,11 +57,15 @@ def rho(X, contracted) -> float:
     return float(ratios[torch.isfinite(ratios)].mean())
 
 
-def tolerance(mode: str, ns: int, rho_sum: float) -> float:
+def tolerance(mode: str, ns: int, rho_sum: float, k: int) -> float:
     if mode == "fp64":
         return 1e-13
     if mode == "fp32":
         return 1e-5
+    if mode == "crt":
+        from einsums_amd import core
+
+        return max(4.0 * 2.0 ** -core.crt_bits(ns, k) * rho_sum, 1e-13)
     beta = 7 if mode == "int8" else 8
     return max(4.0 * 2.0 ** (-beta * ns) * rho_sum, 1e-13)
 
@@ -132,7 +138,8 @@ def test_ccd_o48_v256(core, ccdl, name, mode):
         b = base.reshape(nrows, -1)[r].double()
     got = (W.reshape(nrows, -1)[r].double() - b).cpu().numpy()
     err = sampled.rel_err(got, ref)
-    tol = tolerance(st, ns, ccdl.rho_sum(name))
+    kdim = {"Wmnij": V * V, "Wabef": O * O, "Wmbej": O * V}[name]
+    tol = tolerance(st, ns, ccdl.rho_sum(name), kdim)
     print(f"{name} o={O} v={V} {mode_id(mode)}: rel err {err:.3e} over {len(rows)} rows (tol {tol:.1e})")
     del W, base
     assert err <= tol
@@ -141,7 +148,7 @@ def test_ccd_o48_v256(core, ccdl, name, mode):
 
 
 # ------------------------------------------------------------------- DGEMM sweep
-DG_MODES = [("fp64", 0), ("fp32", 0), ("bf16", 5), ("fp16", 5), ("int8", 6)]
+DG_MODES = [("fp64", 0), ("fp32", 0), ("bf16", 5), ("fp16", 5), ("int8", 6), ("crt", 13)]
 DG_CASES = [(n, dist, m) for n in (4096, 8192, 16384) for dist in ("uniform", "stressed") for m in DG_MODES]
 
 
@@ -194,7 +201,7 @@ def test_dgemm_sweep(core, oracle, n, dist, mode):
             core.gemm("N", "N", 1.0, A, B, 0.0, C)
     got = C[torch.from_numpy(rows).cuda()].double().cpu().numpy()
     err = sampled.rel_err(got, ref)
-    tol = tolerance(st, ns, rho(A, [1]) + rho(B, [0]))
+    tol = tolerance(st, ns, rho(A, [1]) + rho(B, [0]), n)
     print(f"DGEMM n={n} {dist} {mode_id(mode)}: rel err {err:.3e} over {len(rows)} rows (tol {tol:.1e})")
     del C
     assert err <= tol

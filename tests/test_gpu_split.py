@@ -5,6 +5,8 @@ Tolerances (relative Frobenius error against the FP64 oracle):
              <= 1e-11 for U(-1,1) data and the CCD contractions,
              <= 1e-10 for the stressed set U(-1,1)*2^e, e in [-20, 20];
   BF16 nslices=3 (24 bits): <= 1e-6; int8 nslices=3 (21 bits): <= 1e-5;
+  CRT with L moduli (kb = ein_split_crt_bits(L, k) bits per element): <= 8 * 2^-kb for U(-1,1)
+  data (about 1e-12 with 13 moduli);
   integer-valued known answers: exact.
 """
 import ctypes
@@ -32,7 +34,7 @@ def seeded(*parts):
     return np.random.default_rng(zlib.crc32("|".join(map(str, parts)).encode()))
 
 
-@pytest.mark.parametrize("slice_type", [0, 1, 2])
+@pytest.mark.parametrize("slice_type", [0, 1, 2, 3])
 @pytest.mark.parametrize("trans", ["NN", "TN", "NT", "TT"])
 def test_split_gemm_3x3_known_answers(slice_type, trans):
     from einsums_amd import _lib
@@ -41,11 +43,12 @@ def test_split_gemm_3x3_known_answers(slice_type, trans):
     A, B = dev(np.array(g["A"], dtype=np.float64)), dev(np.array(g["B"], dtype=np.float64))
     C = dev(np.full(9, np.nan))
     _lib.check(_lib.lib.ein_gemm_split_f64(trans[1].encode(), trans[0].encode(), 3, 3, 3, 1.0, vp(B), 3, vp(A), 3, 0.0,
-                                           vp(C), 3, 6 if slice_type == 2 else 5, slice_type, None))
+                                           vp(C), 3, {2: 6, 3: 13}.get(slice_type, 5), slice_type, None))
     assert host(C).tolist() == g["expected"][trans]
 
 
-SHAPES = [(1, 1, 1), (5, 7, 3), (64, 64, 32), (100, 130, 257), (257, 129, 1000), (64, 64, 65536), (1024, 1024, 2304)]
+SHAPES = [(1, 1, 1), (5, 7, 3), (64, 64, 32), (100, 130, 257), (257, 129, 1000), (64, 64, 65536), (1024, 1024, 2304),
+          (33, 300, 70000)]
 
 
 @pytest.mark.parametrize("m,n,k", SHAPES)
@@ -56,6 +59,22 @@ SHAPES = [(1, 1, 1), (5, 7, 3), (64, 64, 32), (100, 130, 257), (257, 129, 1000),
                          ids=["bf16x5", "bf16x3", "bf16x4", "bf16x6", "fp16x4", "fp16x6", "int8x6", "int8x3", "int8x4",
                               "int8x5"])
 def test_split_gemm_vs_oracle(oracle, m, n, k, ta, tb, slice_type, nslices, tol):
+    split_vs_oracle(oracle, m, n, k, ta, tb, slice_type, nslices, tol)
+
+
+@pytest.mark.parametrize("m,n,k", SHAPES + [(300, 257, 513), (513, 1000, 64)])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("nmod", [8, 10, 13, 16])
+def test_crt_gemm_vs_oracle(oracle, m, n, k, ta, tb, nmod):
+    """The CRT split: ragged edges of the 256 x 256 residue tiles, k = 1 .. 70000 (k > 65536
+    runs as two residue launches that add), every transpose, 8..16 moduli."""
+    from einsums_amd import core
+
+    tol = max(8.0 * 2.0 ** -core.crt_bits(nmod, k), 2e-14)  # FP64 oracle rounding below that
+    split_vs_oracle(oracle, m, n, k, ta, tb, 3, nmod, tol)
+
+
+def split_vs_oracle(oracle, m, n, k, ta, tb, slice_type, nslices, tol):
     from einsums_amd import _lib
 
     rng = seeded(m, n, k, ta, tb)
@@ -78,7 +97,7 @@ def test_split_gemm_vs_oracle(oracle, m, n, k, ta, tb, slice_type, nslices, tol)
     assert err <= tol
 
 
-@pytest.mark.parametrize("slice_type", ["bf16", "fp16", "int8"])
+@pytest.mark.parametrize("slice_type", ["bf16", "fp16", "int8", "crt"])
 @pytest.mark.parametrize("n", [1024, 2048])
 def test_split_dgemm_uniform_and_stressed(core, oracle, slice_type, n):
     rng = seeded("stress", n)
@@ -93,14 +112,14 @@ def test_split_dgemm_uniform_and_stressed(core, oracle, slice_type, n):
         oracle.gemm_colmajor("N", "N", n, n, n, 1.0, np.ascontiguousarray(B).ravel(), n, np.ascontiguousarray(A).ravel(),
                              n, 0.0, ref, n)
         dC = torch.zeros((n, n), dtype=torch.float64, device="cuda")
-        ns = 6 if slice_type == "int8" else 5
+        ns = {"int8": 6, "crt": 13}.get(slice_type, 5)
         core.gemm_split("N", "N", 1.0, dev(A), dev(B), 0.0, dC, nslices=ns, slice_type=slice_type)
         err = rel_fro(host(dC).ravel(), ref)
         print(f"{slice_type} n={n} stressed={stressed}: rel err {err:.3e}")
         assert err <= (1e-10 if stressed else 1e-11)
 
 
-@pytest.mark.parametrize("slice_type,nslices", [("bf16", 5), ("int8", 6)])
+@pytest.mark.parametrize("slice_type,nslices", [("bf16", 5), ("int8", 6), ("crt", 13)])
 @pytest.mark.parametrize("name", ["Wmnij", "Wabef"])
 def test_split_ccd_contractions(core, oracle, name, slice_type, nslices):
     I = workloads.ccd_inputs(24, 96, seed=7)
