@@ -170,18 +170,26 @@ template <typename T, int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_streamk_fixup(GemmProblem<T> P) {
     StreamK const &sk = P.sk;
     uint64_t const t  = uint64_t(sk.dp_tiles) + blockIdx.y;
-    uint64_t g0, g1;
-    sk_owners(sk, t, g0, g1);
-    if (g0 == g1) return;
-    int const e0     = int(blockIdx.x) * 256 + int(threadIdx.x);
-    int const stride = int(gridDim.x) * 256;
+    int const e0      = int(blockIdx.x) * 256 + int(threadIdx.x);
+    int const stride  = int(gridDim.x) * 256;
     T sum[FIXUP_EPT];
 #pragma unroll
     for (int x = 0; x < FIXUP_EPT; ++x) sum[x] = T(0);
-    for (uint64_t g = g0; g <= g1; ++g) {
-        T const *__restrict__ pc = P.partial + sk_slot(sk, g, t) * (BM * BN);
+    if (sk.ksplit) { // split-K: the pieces of tile t in chunk order
+        for (uint64_t c = 0; c < sk.ksplit; ++c) {
+            T const *__restrict__ pc = P.partial + (c * sk.tiles + t) * (BM * BN);
 #pragma unroll
-        for (int x = 0; x < FIXUP_EPT; ++x) sum[x] += pc[e0 + x * stride];
+            for (int x = 0; x < FIXUP_EPT; ++x) sum[x] += pc[e0 + x * stride];
+        }
+    } else {
+        uint64_t g0, g1;
+        sk_owners(sk, t, g0, g1);
+        if (g0 == g1) return;
+        for (uint64_t g = g0; g <= g1; ++g) {
+            T const *__restrict__ pc = P.partial + sk_slot(sk, g, t) * (BM * BN);
+#pragma unroll
+            for (int x = 0; x < FIXUP_EPT; ++x) sum[x] += pc[e0 + x * stride];
+        }
     }
     uint32_t const tiles_mn = P.tiles_m * P.tiles_n;
     int64_t const batch     = int64_t(t / tiles_mn);
@@ -265,8 +273,8 @@ int launch_dense(GemmProblem<T> &P, int blocks_per_cu, hipStream_t stream, size_
     // resident workgroups: the stream-K schedule balances the last round over them
     int const fit  = resident_blocks(kern, GEMM_THREADS);
     int const bpc  = blocks_per_cu > 0 && blocks_per_cu < fit ? blocks_per_cu : fit;
-    uint64_t slots = P.force_dp ? tiles : uint64_t(num_cus()) * uint64_t(bpc);
-    P.sk           = plan_stream_k(tiles, slabs, slots);
+    uint64_t slots = P.force_dp == 1 ? tiles : uint64_t(num_cus()) * uint64_t(bpc);
+    P.sk           = P.force_dp == 2 ? plan_split_k(tiles, slabs, slots) : plan_stream_k(tiles, slabs, slots);
     P.partial      = nullptr;
     size_t const piece_bytes = size_t(sk_piece_slots(P.sk)) * BM * BN * sizeof(T);
     if (ws_bytes != nullptr) {
@@ -345,7 +353,7 @@ std::vector<GemmConfig> tune_candidates() {
     std::vector<GemmConfig> c;
     for (int bm : {128, 64})
         for (int bn : {128, 64})
-            for (int dp : {0, 1})
+            for (int dp : {0, 1, 2})
                 for (int bpc : {0, 1}) {
                     if (dp == 1 && bpc == 1) continue; // data-parallel ignores the planned occupancy
                     GemmConfig g;
@@ -768,7 +776,7 @@ int ein_gemm_tuning_load(const char *path) {
         long long m, n, k, b;
         if (std::sscanf(line, "%d %lld %lld %lld %lld %d %d : %d %d %d %d", &eb, &m, &n, &k, &b, &ak, &bk, &bm, &bn, &dp,
                         &bpc) != 11 ||
-            (eb != 4 && eb != 8) || (bm != 64 && bm != 128) || (bn != 64 && bn != 128) || dp < 0 || dp > 1 || bpc < 0 ||
+            (eb != 4 && eb != 8) || (bm != 64 && bm != 128) || (bn != 64 && bn != 128) || dp < 0 || dp > 2 || bpc < 0 ||
             m <= 0 || n <= 0 || k <= 0 || b <= 0) {
             std::fclose(f);
             return ein::fail(EIN_ERROR_VALUE, "gemm tuning: malformed line " + std::to_string(lineno) + " in " + path);

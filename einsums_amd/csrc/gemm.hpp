@@ -25,7 +25,8 @@ struct GemmProblem {
 struct GemmConfig {
     int bm            = 128;
     int bn            = 128;
-    int data_parallel = 0; // 1: one tile per workgroup even where stream-K would balance better
+    int data_parallel = 0; // 1: one tile per workgroup even where stream-K would balance better;
+                           // 2: split-K over XCD-local contraction chunks (plan_split_k)
     int blocks_per_cu = 0; // workgroups per CU the schedule plans for; 0 = as many as fit
 };
 

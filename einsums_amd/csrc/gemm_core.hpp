@@ -131,6 +131,8 @@ struct StreamK {
     uint32_t dp_tiles; // tiles owned whole by one workgroup each
     uint32_t grid;     // workgroups launched: dp_tiles + stream-K workgroups
     uint64_t units;    // units per stream-K workgroup; 0 = no stream-K part
+    uint32_t ksplit;   // > 1: split-K schedule (plan_split_k), 0 otherwise
+    uint32_t kchunk;   // split-K: slabs per contraction chunk
 };
 
 // Decides the schedule for `tiles` tiles of `slabs` slabs with `slots` workgroups
@@ -160,10 +162,45 @@ inline StreamK plan_stream_k(uint64_t tiles, uint64_t slabs, uint64_t slots) {
     return sk;
 }
 
+// Split-K schedule for launches with less than half a round of tiles and a long
+// contraction (the 576 x 576 x 9216 Wmnij GEMM: 81 tiles of 64 x 64): the contraction is cut
+// into P chunks of equal length and workgroup w (XCD-remapped) owns chunk w / tiles of tile
+// w % tiles. Consecutive workgroups, which share an XCD and its L2, thus work on all tiles
+// of one chunk at the same contraction offsets, so each A and B panel slab is fetched from
+// HBM / MALL about once per XCD; a stream-K range instead spans every chunk of a few tiles.
+// Every workgroup writes its partial tile to piece slot w; the fix-up sums the P pieces of a
+// tile in chunk order (deterministic). Falls back to plan_stream_k where it does not apply.
+inline StreamK plan_split_k(uint64_t tiles, uint64_t slabs, uint64_t slots) {
+    if (tiles == 0 || slabs == 0 || 2 * tiles > slots) return plan_stream_k(tiles, slabs, slots);
+    uint64_t p = slots / tiles;
+    if (p > slabs / 8) p = slabs / 8; // chunks of at least 8 slabs
+    if (p < 2) return plan_stream_k(tiles, slabs, slots);
+    uint64_t const chunk = (slabs + p - 1) / p;
+    p                    = (slabs + chunk - 1) / chunk; // every chunk non-empty
+    StreamK sk{};
+    sk.tiles    = uint32_t(tiles);
+    sk.slabs    = uint32_t(slabs);
+    sk.dp_tiles = 0;
+    sk.grid     = uint32_t(tiles * p);
+    sk.units    = chunk; // non-zero: pieces and a fix-up pass
+    sk.ksplit   = uint32_t(p);
+    sk.kchunk   = uint32_t(chunk);
+    return sk;
+}
+
 // Unit range [u0, u1) of launched workgroup `block`, and its stream-K index w (0 for a
-// data-parallel workgroup).
+// data-parallel workgroup; the logical index under split-K).
 __host__ __device__ __forceinline__ void sk_range_of(StreamK const &sk, uint32_t block, uint64_t &u0, uint64_t &u1,
                                                      uint64_t &w) {
+    if (sk.ksplit) {
+        w                   = xcd_remap(block, sk.grid);
+        uint64_t const c    = w / sk.tiles;
+        uint64_t const t    = w - c * sk.tiles;
+        uint64_t const end  = (c + 1) * sk.kchunk;
+        u0                  = t * sk.slabs + c * sk.kchunk;
+        u1                  = t * sk.slabs + (end < sk.slabs ? end : sk.slabs);
+        return;
+    }
     if (block < sk.dp_tiles) {
         uint64_t const t = xcd_remap(block, sk.dp_tiles);
         u0               = t * sk.slabs;
@@ -186,6 +223,7 @@ __device__ __forceinline__ void sk_range(StreamK const &sk, uint64_t &u0, uint64
 // the range starts inside t (t is w's first tile), 2w+1 otherwise. No division, so the
 // fix-up can evaluate it per piece.
 __host__ __device__ __forceinline__ uint64_t sk_slot(StreamK const &sk, uint64_t w, uint64_t t) {
+    if (sk.ksplit) return w; // split-K: one piece per workgroup, slot c * tiles + t
     uint64_t const start = uint64_t(sk.dp_tiles) * sk.slabs + w * sk.units;
     return 2 * w + (start >= t * sk.slabs ? 0 : 1);
 }
@@ -207,6 +245,8 @@ inline dim3 sk_fixup_grid(StreamK const &sk, int tile_elems) {
 }
 
 // Number of piece slots a launch needs.
-inline uint64_t sk_piece_slots(StreamK const &sk) { return sk.units ? 2ull * (sk.grid - sk.dp_tiles) : 0; }
+inline uint64_t sk_piece_slots(StreamK const &sk) {
+    return sk.ksplit ? uint64_t(sk.grid) : sk.units ? 2ull * (sk.grid - sk.dp_tiles) : 0;
+}
 
 } // namespace ein

@@ -147,12 +147,14 @@ int ein_axpby_f32(int64_t n, float alpha, const float *x, int64_t incx, float be
  * GEMM-mapped einsum looks its problem up in a process-wide table keyed by the normalised
  * problem (element size, m, n, k, batch, whether A and B are contiguous along k); an entry
  * overrides the built-in tile heuristic. With tuning mode on, a GEMM whose key is absent
- * first times every candidate (tile 128/64 x 128/64, hybrid stream-K or data-parallel,
- * planned workgroups per CU) on its own operands, writing a scratch C, and stores the
+ * first times every candidate (tile 128/64 x 128/64; hybrid stream-K, data-parallel or
+ * split-K over XCD-local contraction chunks; planned workgroups per CU) on its own
+ * operands, writing a scratch C, and stores the
  * fastest (not while its stream is captured). No counterpart in the reference (its GEMMs
  * are vendor BLAS calls). The table is saved and loaded as text.
  * ein_gemm_tuning_mode returns the previous mode; ein_gemm_tuning_lookup writes
- * {bm, bn, data_parallel, blocks_per_cu} to config[0..3]. */
+ * {bm, bn, schedule, blocks_per_cu} to config[0..3] (schedule 0 stream-K, 1 data-parallel,
+ * 2 split-K). */
 int ein_gemm_tuning_mode(int32_t on);
 int ein_gemm_tuning_clear(void);
 int64_t ein_gemm_tuning_count(void);

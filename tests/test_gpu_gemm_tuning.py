@@ -46,3 +46,39 @@ def test_tuning_mode_on_gemm_and_einsum(core, oracle, tmp_path):
     W3, C3 = run()
     assert np.array_equal(W1, W3) and np.array_equal(C1, C3)
     core.clear_gemm_tuning()
+
+
+@pytest.mark.parametrize("m,n,k", [(576, 576, 9216), (100, 300, 5000), (144, 144, 2304)])
+@pytest.mark.parametrize("tile", [(64, 64), (64, 128), (128, 128)])
+@pytest.mark.parametrize("bpc", [0, 1])
+def test_split_k_schedule_from_table(core, oracle, tmp_path, m, n, k, tile, bpc):
+    """Schedule 2 (split-K over XCD-local contraction chunks) forced through a loaded table
+    entry: matches the oracle and repeats bit for bit; the fix-up adds the chunks in order."""
+    rng = np.random.default_rng(m + n + k)
+    A = rng.uniform(-1, 1, (m, k))
+    B = rng.uniform(-1, 1, (k, n))
+    C0 = rng.uniform(-1, 1, (m, n))
+    dA, dB = dev(A), dev(B)
+    core.clear_gemm_tuning()
+    with core.gemm_tuning():  # creates the problem's key
+        C = dev(C0)
+        core.gemm("N", "N", 0.5, dA, dB, -1.0, C)
+    f = tmp_path / "t.txt"
+    core.save_gemm_tuning(f)
+    lines = [ln for ln in f.read_text().splitlines() if ln and not ln.startswith("#")]
+    assert len(lines) == 1
+    key = lines[0].split(":")[0]
+    f.write_text(f"{key}: {tile[0]} {tile[1]} 2 {bpc}\n")
+    core.clear_gemm_tuning()
+    core.load_gemm_tuning(f)
+    ref = C0.copy().ravel()
+    # row-major C = A B is column-major C^T = B^T A^T
+    oracle.gemm_colmajor("N", "N", n, m, k, 0.5, B.ravel(), n, A.ravel(), k, -1.0, ref, n)
+    outs = []
+    for _ in range(2):
+        C = dev(C0)
+        core.gemm("N", "N", 0.5, dA, dB, -1.0, C)
+        outs.append(host(C))
+    core.clear_gemm_tuning()
+    assert rel_fro(outs[0].ravel(), ref) < 1e-13
+    assert np.array_equal(outs[0], outs[1])
