@@ -26,6 +26,7 @@ EIN_ERROR_VALUE = 1007
 # detail::AlgorithmChoice numbering (TensorAlgebra/Detail/Utilities.hpp:26-34)
 GENERIC, DOT, DIRECT, GER, GEMV, GEMM, INDETERMINATE = range(7)
 SLICE_BF16, SLICE_FP16, SLICE_INT8, SLICE_CRT = 0, 1, 2, 3
+SPLIT_REFINE = 0x100
 ROUTE_AUTO, ROUTE_GATHER, ROUTE_TTGT = 0, 1, 2
 ALGORITHM_NAMES = {GENERIC: "GENERIC", DOT: "DOT", DIRECT: "DIRECT", GER: "GER", GEMV: "GEMV", GEMM: "GEMM",
                    INDETERMINATE: "INDETERMINATE"}

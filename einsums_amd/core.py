@@ -396,11 +396,13 @@ SLICE_TYPES = {"bf16": _lib.SLICE_BF16, "fp16": _lib.SLICE_FP16, "int8": _lib.SL
 
 def _slice_code(slice_type: str) -> int:
     """ABI code of a split-precision slice type: "bf16", "fp16" (nslices 3..6 slices),
-    "int8" (3..6 slices) or "crt" (nslices = 8..16 moduli of the integer modular scheme)."""
-    try:
-        return SLICE_TYPES[slice_type]
-    except KeyError:
-        raise ValueError(f"slice_type must be one of {sorted(SLICE_TYPES)}, got {slice_type!r}") from None
+    "int8" (3..6 slices) or "crt" (nslices = 8..16 moduli of the integer modular scheme);
+    a "+refine" suffix (e.g. "crt+refine") adds one iterative-refinement correction."""
+    base, plus, flag = slice_type.partition("+")
+    if base not in SLICE_TYPES or (plus and flag != "refine"):
+        raise ValueError(f"slice_type must be one of {sorted(SLICE_TYPES)}, optionally with '+refine', "
+                         f"got {slice_type!r}")
+    return SLICE_TYPES[base] | (_lib.SPLIT_REFINE if plus else 0)
 
 
 def crt_bits(nmoduli: int, k: int) -> int:

@@ -420,10 +420,10 @@ int ein_einsum_execute_f64(int32_t algorithm, double c_prefactor, const ein_tens
 int ein_einsum_split_f64(int32_t nslices, int32_t slice_type, double c_prefactor, const ein_tensor_desc *C, double *c,
                          double ab_prefactor, const ein_tensor_desc *A, const double *a, const ein_tensor_desc *B,
                          const double *b, int32_t *algorithm_used, ein_stream_t stream) {
-    if (slice_type != EIN_SLICE_BF16 && slice_type != EIN_SLICE_FP16 && slice_type != EIN_SLICE_INT8 &&
-        slice_type != EIN_SLICE_CRT)
+    int const type = slice_type & ~EIN_SPLIT_REFINE;
+    if (type != EIN_SLICE_BF16 && type != EIN_SLICE_FP16 && type != EIN_SLICE_INT8 && type != EIN_SLICE_CRT)
         return ein::fail(-2, "einsum_split: bad slice type");
-    if (slice_type == EIN_SLICE_CRT ? (nslices < 8 || nslices > 16) : (nslices < 3 || nslices > 6))
+    if (type == EIN_SLICE_CRT ? (nslices < 8 || nslices > 16) : (nslices < 3 || nslices > 6))
         return ein::fail(-1, "einsum_split: nslices must be 3..6 (8..16 moduli for EIN_SLICE_CRT)");
     ein::t_split_slices = nslices;
     ein::t_split_type   = slice_type;

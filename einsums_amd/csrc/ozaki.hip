@@ -732,14 +732,97 @@ int row_exponents(SliceSrc const &S, int32_t *e, bool guard, double *scratch, hi
     return EIN_SUCCESS;
 }
 
+namespace {
+
+// Residual of the split representation of a strided operand, written along k:
+// out[r*k + p] = X(r,p) - rep(X(r,p)). rep is the representation the schemes multiply:
+// x = X 2^-e_r, then `steps` times y = x 2^bits, d = rint(y), x = y - d; the remainder is
+// x 2^(e_r - bits*steps) (slices: bits 7 or 8, steps = slices; CRT: bits = kb, one step).
+template <bool ROWS_FAST>
+__global__ __launch_bounds__(256) void split_residual_kernel(SliceSrc S, int32_t const *__restrict__ e, int bits,
+                                                             int steps, double *__restrict__ out) {
+    __shared__ double tile[64][65];
+    int64_t const r0 = int64_t(blockIdx.y) * 64;
+    int64_t const p0 = int64_t(blockIdx.x) * 64;
+    int const t      = threadIdx.x;
+    if constexpr (ROWS_FAST) {
+        int const lr = t & 63;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            int lp       = (t >> 6) + 4 * q;
+            int64_t r    = r0 + lr, p = p0 + lp;
+            tile[lr][lp] = (r < S.rows && p < S.k) ? S.X[r * S.s_row + p * S.s_k] : 0.0;
+        }
+    } else {
+        int const lp = t & 63;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            int lr       = (t >> 6) + 4 * q;
+            int64_t r    = r0 + lr, p = p0 + lp;
+            tile[lr][lp] = (r < S.rows && p < S.k) ? S.X[r * S.s_row + p * S.s_k] : 0.0;
+        }
+    }
+    __syncthreads();
+    int const lp    = t & 63;
+    int64_t const p = p0 + lp;
+    if (p >= S.k) return;
+#pragma unroll 1
+    for (int q = 0; q < 16; ++q) {
+        int lr    = (t >> 6) + 4 * q;
+        int64_t r = r0 + lr;
+        if (r >= S.rows) continue;
+        double x = ldexp(tile[lr][lp], -e[r]);
+        for (int s = 0; s < steps; ++s) {
+            double const y = ldexp(x, bits);
+            x              = y - rint(y);
+        }
+        out[r * S.k + p] = ldexp(x, e[r] - bits * steps);
+    }
+}
+
+int split_residual(SliceSrc const &S, bool guard, int bits, int steps, int32_t *e, double *out, double *scratch,
+                   hipStream_t stream) {
+    bool const rows_fast = S.s_row == 1 || (S.s_k != 1 && std::llabs(S.s_row) < std::llabs(S.s_k));
+    EIN_TRY(row_exponents(S, e, guard, scratch, stream));
+    dim3 grid{uint32_t(ceil_div(S.k, 64)), uint32_t(ceil_div(S.rows, 64))};
+    if (grid.y > 65535) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: more than 4M rows");
+    if (rows_fast)
+        split_residual_kernel<true><<<grid, 256, 0, stream>>>(S, e, bits, steps, out);
+    else
+        split_residual_kernel<false><<<grid, 256, 0, stream>>>(S, e, bits, steps, out);
+    EIN_LAUNCH_CHECK();
+    return EIN_SUCCESS;
+}
+
+// One split GEMM of a normalised problem (C's unit stride along i, k > 0, alpha != 0).
+int gemm_split_once(GemmProblem<double> const &P, int nslices, int type, hipStream_t stream) {
+    int64_t const t128 = ceil_div(P.m, 128) * ceil_div(P.n, 64);
+    int const bm       = t128 >= int64_t(num_cus()) * 2 ? 128 : 64;
+    if (type == EIN_SLICE_CRT) return gemm_split_crt(P, nslices, stream);
+    if (type == EIN_SLICE_INT8) return gemm_split_i8(P, nslices, stream);
+    if (type == EIN_SLICE_BF16) return gemm_split_typed<__bf16>(P, nslices, bm, stream);
+    return gemm_split_typed<_Float16>(P, nslices, bm, stream);
+}
+
+} // namespace
+
 int gemm_split(GemmProblem<double> P, int nslices, int slice_type, hipStream_t stream) {
-    if (P.m <= 0 || P.n <= 0) return EIN_SUCCESS;
-    if (slice_type == EIN_SLICE_CRT) {
+    bool const refine = (slice_type & EIN_SPLIT_REFINE) != 0;
+    int const type    = slice_type & ~EIN_SPLIT_REFINE;
+    if (type != EIN_SLICE_BF16 && type != EIN_SLICE_FP16 && type != EIN_SLICE_INT8 && type != EIN_SLICE_CRT)
+        return fail(EIN_ERROR_VALUE,
+                    "split gemm: slice_type must be EIN_SLICE_BF16, EIN_SLICE_FP16, EIN_SLICE_INT8 or EIN_SLICE_CRT");
+    if (type == EIN_SLICE_CRT) {
         if (nslices < CRT_MIN_MODULI || nslices > 16)
             return fail(EIN_ERROR_VALUE, "split gemm: EIN_SLICE_CRT takes 8..16 moduli");
     } else if (nslices < 3 || nslices > 6) {
         return fail(EIN_ERROR_VALUE, "split gemm: nslices must be 3..6");
     }
+    if (refine && type != EIN_SLICE_CRT)
+        return fail(EIN_ERROR_VALUE, "split gemm: EIN_SPLIT_REFINE applies to EIN_SLICE_CRT only (the slice schemes "
+                                     "drop the cross products of slices t + u >= nslices, which residuals of A and B "
+                                     "do not correct)");
+    if (P.m <= 0 || P.n <= 0) return EIN_SUCCESS;
     if (P.batch != 1) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: batched problems are not supported");
     // C's unit-stride direction must be i (lane-contiguous stores): transpose the problem otherwise
     bool const c_row_fast = (P.c_sn == 1 && P.c_sm != 1) || (P.c_sm != 1 && P.c_sn != 1 && P.c_sn < P.c_sm);
@@ -761,13 +844,47 @@ int gemm_split(GemmProblem<double> P, int nslices, int slice_type, hipStream_t s
         GemmConfig cfg;
         return gemm_strided<double>(P, &cfg, stream); // scales C by beta
     }
-    int64_t const t128 = ceil_div(P.m, 128) * ceil_div(P.n, 64);
-    int const bm = t128 >= int64_t(num_cus()) * 2 ? 128 : 64;
-    if (slice_type == EIN_SLICE_CRT) return gemm_split_crt(P, nslices, stream);
-    if (slice_type == EIN_SLICE_INT8) return gemm_split_i8(P, nslices, stream);
-    if (slice_type == EIN_SLICE_BF16) return gemm_split_typed<__bf16>(P, nslices, bm, stream);
-    if (slice_type == EIN_SLICE_FP16) return gemm_split_typed<_Float16>(P, nslices, bm, stream);
-    return fail(EIN_ERROR_VALUE, "split gemm: slice_type must be EIN_SLICE_BF16, EIN_SLICE_FP16, EIN_SLICE_INT8 or EIN_SLICE_CRT");
+    EIN_TRY(gemm_split_once(P, nslices, type, stream));
+    if (!refine) return EIN_SUCCESS;
+
+    // One refinement correction: with dA = A - rep(A) and dB = B - rep(B) formed in FP64,
+    // A B = rep(A) rep(B) + dA B + A dB - dA dB. The CRT GEMM above gives the first term
+    // exactly (one rounding); the next two run as CRT GEMMs with beta = 1 and 8 moduli (their
+    // magnitude is 2^-kb of C's, so their own 2^-24..2^-30 error is below FP64 rounding);
+    // dA dB (2^-2kb) is dropped. The residual kernel also covers the slice representations
+    // (bits 7 or 8 per slice), which the schemes above do not use: their error is dominated
+    // by the dropped slice cross products.
+    int const bits = crt_bits(nslices, P.k), steps = 1;
+    bool const guard = false;
+    int const corr_slices = CRT_MIN_MODULI;
+    size_t const da = size_t(P.m) * size_t(P.k), db = size_t(P.n) * size_t(P.k);
+    auto up         = [](size_t x) { return (x * sizeof(double) + 255) / 256 * 256; };
+    size_t const ea_off = up(da) + up(db);
+    size_t const eb_off = ea_off + (size_t(P.m) * 4 + 255) / 256 * 256;
+    size_t const x_off  = eb_off + (size_t(P.n) * 4 + 255) / 256 * 256;
+    size_t const total  = x_off + row_exp_scratch(std::max(P.m, P.n)) * sizeof(double);
+    void *ws            = nullptr;
+    EIN_TRY(workspace_slot(stream, 2, total, &ws)); // slot 0 is the split GEMM's, slot 1 the packed route's
+    char *base = static_cast<char *>(ws);
+    double *dA = reinterpret_cast<double *>(base);
+    double *dB = reinterpret_cast<double *>(base + up(da));
+    int32_t *ea = reinterpret_cast<int32_t *>(base + ea_off);
+    int32_t *eb = reinterpret_cast<int32_t *>(base + eb_off);
+    double *scratch = reinterpret_cast<double *>(base + x_off);
+    EIN_TRY(split_residual(SliceSrc{P.m, P.k, P.k, P.A, P.a_sm, P.a_sk}, guard, bits, steps, ea, dA, scratch, stream));
+    EIN_TRY(split_residual(SliceSrc{P.n, P.k, P.k, P.B, P.b_sn, P.b_sk}, guard, bits, steps, eb, dB, scratch, stream));
+    GemmProblem<double> Q = P;
+    Q.beta = 1.0;
+    Q.A    = dA; // dA(i,p) = dA[i*k + p]
+    Q.a_sm = P.k;
+    Q.a_sk = 1;
+    EIN_TRY(gemm_split_once(Q, corr_slices, type, stream));
+    Q      = P;
+    Q.beta = 1.0;
+    Q.B    = dB; // dB(p,j) = dB[j*k + p]
+    Q.b_sn = P.k;
+    Q.b_sk = 1;
+    return gemm_split_once(Q, corr_slices, type, stream);
 }
 
 } // namespace ein

@@ -182,6 +182,13 @@ int ein_gemm_tuning_load(const char *path);
  *     largest value with 4 k 2^(2 kb) <= prod(moduli), relative representation error 2^-kb
  *     (13 moduli: kb = 44 at k = 2304, 42 at k = 65536). */
 #define EIN_SLICE_CRT 3
+/* OR'd into EIN_SLICE_CRT: one iterative-refinement correction. The residuals
+ * dA = A - rep(A) and dB = B - rep(B) of the kb-bit integer representation are formed in
+ * FP64 and C += alpha*(dA*B + A*dB) runs as two more CRT GEMMs with 8 moduli, which takes the
+ * result to FP64 rounding (about 1e-15) for about 1.5x the time of the plain CRT GEMM.
+ * Rejected (EIN_ERROR_VALUE) with the slice types, whose error is dominated by the dropped
+ * slice cross products rather than by the representation of A and B. */
+#define EIN_SPLIT_REFINE 0x100
 int ein_gemm_split_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
                        const double *B, int64_t ldb, double beta, double *C, int64_t ldc, int32_t nslices,
                        int32_t slice_type, ein_stream_t stream);

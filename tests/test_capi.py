@@ -171,3 +171,14 @@ def test_split_crt_bits_host_only():
             assert f(L, k) == kb, (L, k)
     assert f(13, 2304) == 44 and f(13, 65536) == 42
     assert f(7, 100) == -1 and f(17, 100) == -1 and f(13, -1) == -1
+
+
+def test_slice_type_codes():
+    """Split-precision slice names map to the ABI codes; "+refine" sets EIN_SPLIT_REFINE."""
+    from einsums_amd import _lib, core
+
+    assert core._slice_code("bf16") == _lib.SLICE_BF16 and core._slice_code("crt") == _lib.SLICE_CRT
+    assert core._slice_code("int8+refine") == _lib.SLICE_INT8 | _lib.SPLIT_REFINE == 0x102
+    for bad in ("bf8", "crt+", "crt+refine+x", "int8+fast"):
+        with pytest.raises(ValueError):
+            core._slice_code(bad)

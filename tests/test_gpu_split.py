@@ -137,3 +137,63 @@ def test_split_ccd_contractions(core, oracle, name, slice_type, nslices):
     err = rel_fro(host(dW) - W0, ref - W0)
     print(f"{name} {slice_type} x{nslices} rel err of the contraction {err:.3e}")
     assert err <= 1e-11
+
+
+REFINE_CASES = [("crt", 13), ("crt", 10), ("crt", 8)]
+
+
+@pytest.mark.parametrize("m,n,k", [(257, 129, 1000), (64, 64, 65536), (1024, 1024, 2304), (33, 300, 70000)])
+@pytest.mark.parametrize("st,ns", REFINE_CASES, ids=[f"{s}x{n}" for s, n in REFINE_CASES])
+@pytest.mark.parametrize("stressed", [False, True])
+def test_split_refine_vs_oracle(core, oracle, m, n, k, st, ns, stressed):
+    """One iterative-refinement correction ("crt+refine"): the FP64 residuals of the kb-bit
+    representation are multiplied by CRT GEMMs with 8 moduli (kb8 bits) and added. Error
+    against the oracle after refinement <= max(8 * 2^-(kb + kb8), 4e-14), FP64 level unless
+    both are short (8 moduli at k = 65536: 22 + 22 bits); the unrefined error is about 2^-kb.
+    Uniform and stressed (2^+-20) data, 8..13 moduli."""
+    rng = seeded("refine", m, n, k, st, ns, stressed)
+    A = rng.uniform(-1, 1, (m, k))
+    B = rng.uniform(-1, 1, (k, n))
+    if stressed:
+        A = A * np.exp2(rng.integers(-20, 21, (m, k)))
+        B = B * np.exp2(rng.integers(-20, 21, (k, n)))
+    C0 = rng.uniform(-1, 1, (m, n))
+    ref = C0.copy().ravel()
+    # row-major C = A B is column-major C^T = B^T A^T
+    oracle.gemm_colmajor("N", "N", n, m, k, 0.75, np.ascontiguousarray(B).ravel(), n, np.ascontiguousarray(A).ravel(), k,
+                         -0.5, ref, n)
+    dA, dB = dev(A), dev(B)
+    errs = {}
+    for mode in (st, st + "+refine"):
+        dC = dev(C0)
+        core.gemm_split("N", "N", 0.75, dA, dB, -0.5, dC, nslices=ns, slice_type=mode)
+        errs[mode] = rel_fro(host(dC).ravel(), ref)
+    tol = max(8.0 * 2.0 ** -(core.crt_bits(ns, k) + core.crt_bits(8, k)), 4e-14)
+    print(f"{st}x{ns} {m}x{n}x{k} stressed={stressed}: {errs[st]:.3e} -> refined {errs[st + '+refine']:.3e} (tol {tol:.1e})")
+    assert errs[st + "+refine"] <= tol
+    assert errs[st + "+refine"] <= max(errs[st], 4e-14)
+
+
+@pytest.mark.parametrize("st", ["bf16", "fp16", "int8"])
+def test_split_refine_rejected_for_slices(core, st):
+    C = torch.zeros((8, 8), dtype=torch.float64, device="cuda")
+    A = torch.ones((8, 8), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="EIN_SLICE_CRT only"):
+        core.gemm_split("N", "N", 1.0, A, A, 0.0, C, nslices=5, slice_type=st + "+refine")
+
+
+def test_split_refine_einsum(core, oracle):
+    """The refinement through the einsum entry point (GEMM-mapped Wabef at o=12, v=48)."""
+    I = workloads.ccd_inputs(12, 48, seed=3)
+    spec = {c[0]: c for c in workloads.CCD_CONTRACTIONS}["Wabef"]
+    _, Ci, An, Ai, Bn, Bi, cp, abp, init = spec
+    arrs = {"t_oovv": I.t_oovv, "g_oovv": I.g_oovv, "g_vvvv": I.g_vvvv}
+    ref = arrs[init].copy()
+    oracle.einsum(cp, Ci, ref, abp, Ai, arrs[An], Bi, arrs[Bn])
+    dW = dev(arrs[init])
+    alg = core.einsum(Ci, dW, Ai, dev(arrs[An]), Bi, dev(arrs[Bn]), C_prefactor=cp, AB_prefactor=abp, split=13,
+                      slice_type="crt+refine")
+    assert core.algorithm_name(alg) == "GEMM"
+    err = rel_fro(host(dW) - arrs[init], ref - arrs[init])
+    print(f"Wabef o=12 v=48 crt x13 + refine: {err:.3e}")
+    assert err <= 1e-14
