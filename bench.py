@@ -254,13 +254,16 @@ def hf_block(core, nbf=48, nocc=12, reps=20):
     return out
 
 
-SPLIT_MODES = (("crt", 13), ("crt+refine", 13), ("int8", 6), ("bf16", 5), ("fp16", 5))
+SPLIT_MODES = (("crt", "auto"), ("crt", 13), ("crt+refine", 13), ("int8", 6), ("bf16", 5), ("fp16", 5))
 
 
 def split_key(st, ns):
-    """Name of a split mode in the JSON: "crtx13", "crtx13+refine", "int8x6"; the plain modes
-    ("fp64", "fp32") keep their names."""
+    """Name of a split mode in the JSON: "crtx13", "crtx13+refine", "int8x6", "crt-auto" (the
+    fewest moduli keeping 40 bits at each contraction's length); the plain modes ("fp64",
+    "fp32") keep their names."""
     base, plus, flag = st.partition("+")
+    if ns == "auto":
+        return f"{base}-auto{plus}{flag}"
     return st if not ns else f"{base}x{ns}{plus}{flag}"
 
 

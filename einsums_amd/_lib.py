@@ -152,6 +152,7 @@ def _load():
     sig["ein_gemm_split_f64"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64, _dp,
                                           _i64, ctypes.c_double, _dp, _i64, ctypes.c_int32, ctypes.c_int32, _vp])
     sig["ein_split_crt_bits"] = (_c_int, [ctypes.c_int32, _i64])
+    sig["ein_split_crt_moduli"] = (_c_int, [_i64])
     sig["ein_gemm_host_c_f64"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64,
                                            _dp, _i64, ctypes.c_double, _vp, _i64, ctypes.c_size_t, ctypes.c_int32,
                                            ctypes.c_int32, _vp])

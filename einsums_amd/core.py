@@ -188,7 +188,7 @@ class EinsumGenericPlan:
             if dtype != torch.float64:
                 raise TypeError("split precision applies to float64 operands")
             st = _slice_code(slice_type)
-            status = lib.ein_einsum_split_f64(int(split), st, float(C_prefactor), ctypes.byref(cd), _ptr(dC),
+            status = lib.ein_einsum_split_f64(_split_count(split), st, float(C_prefactor), ctypes.byref(cd), _ptr(dC),
                                               float(AB_prefactor), ctypes.byref(ad), _ptr(dA), ctypes.byref(bd), _ptr(dB),
                                               ctypes.byref(used), stream_handle(stream))
         else:
@@ -405,6 +405,17 @@ def _slice_code(slice_type: str) -> int:
     return SLICE_TYPES[base] | (_lib.SPLIT_REFINE if plus else 0)
 
 
+def _split_count(split) -> int:
+    """nslices for the ABI: an int, or "auto" (0: the CRT scheme's automatic moduli count)."""
+    return 0 if split == "auto" else int(split)
+
+
+def crt_moduli(k: int) -> int:
+    """Moduli the "crt" split uses with split="auto" at contraction length k: the fewest that
+    keep at least 40 bits per element."""
+    return lib.ein_split_crt_moduli(int(k))
+
+
 def crt_bits(nmoduli: int, k: int) -> int:
     """Bits kept per operand element by the "crt" split GEMM with `nmoduli` moduli at
     contraction length k (the relative rounding of every A and B element is 2^-bits)."""
@@ -425,7 +436,8 @@ def _gemm_split_body(transA, transB, alpha, A, B, beta, C, nslices, slice_type):
     n = B.shape[0] if tb else B.shape[1]
     st = _slice_code(slice_type)
     status = lib.ein_gemm_split_f64(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0), _ptr(A),
-                                    A.stride(0), float(beta), _ptr(C), C.stride(0), int(nslices), st, stream_handle(stream))
+                                    A.stride(0), float(beta), _ptr(C), C.stride(0), _split_count(nslices), st,
+                                    stream_handle(stream))
     check(status, "gemm_split")
 
 
@@ -482,7 +494,7 @@ def _gemm_host_c_body(transA, transB, alpha, A, B, beta, C, panel_bytes, split, 
         st = _slice_code(slice_type)
         status = lib.ein_gemm_host_c_f64(transB.encode(), transA.encode(), n, m, k, float(alpha), _ptr(B), B.stride(0),
                                          _ptr(A), A.stride(0), float(beta), cptr, C.stride(0), int(panel_bytes),
-                                         int(split), st, stream_handle(stream))
+                                         crt_moduli(k) if split == "auto" else int(split), st, stream_handle(stream))
     else:
         if split:
             raise TypeError("gemm_host_c: split precision applies to float64")

@@ -523,10 +523,21 @@ int crt_bits(int nmod, int64_t k) {
     return crt_kbits(nmod, std::max<int64_t>(k, 1));
 }
 
+int crt_auto_moduli(int64_t k) {
+    for (int L = CRT_MIN_MODULI; L < CRT_MAX_MODS; ++L)
+        if (crt_kbits(L, std::max<int64_t>(k, 1)) >= CRT_AUTO_BITS) return L;
+    return CRT_MAX_MODS;
+}
+
 } // namespace ein
 
 extern "C" {
 
-int ein_split_crt_bits(int32_t nmoduli, int64_t k) { return ein::crt_bits(nmoduli, k); }
+int ein_split_crt_bits(int32_t nmoduli, int64_t k) {
+    if (nmoduli == 0 && k >= 0) nmoduli = ein::crt_auto_moduli(k);
+    return ein::crt_bits(nmoduli, k);
+}
+
+int ein_split_crt_moduli(int64_t k) { return k < 0 ? -1 : ein::crt_auto_moduli(k); }
 
 } // extern "C"

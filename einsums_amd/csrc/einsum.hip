@@ -21,7 +21,8 @@ namespace ein {
 
 namespace {
 
-// Split-precision setting for FP64 GEMM-mapped einsums on this thread (0 = plain FP64).
+// Split-precision setting for FP64 GEMM-mapped einsums on this thread (0 = plain FP64;
+// -1 = the automatic moduli count of EIN_SLICE_CRT, passed on as nslices 0).
 thread_local int t_split_slices = 0;
 thread_local int t_split_type   = EIN_SLICE_BF16;
 
@@ -30,7 +31,8 @@ thread_local int t_split_type   = EIN_SLICE_BF16;
 template <typename T>
 int gemm_dispatch(GemmProblem<T> const &P, hipStream_t stream) {
     if constexpr (std::is_same_v<T, double>) {
-        if (t_split_slices > 0 && P.batch == 1) return gemm_split(P, t_split_slices, t_split_type, stream);
+        if (t_split_slices != 0 && P.batch == 1)
+            return gemm_split(P, t_split_slices < 0 ? 0 : t_split_slices, t_split_type, stream);
     }
     return gemm_strided<T>(P, nullptr, stream);
 }
@@ -423,9 +425,9 @@ int ein_einsum_split_f64(int32_t nslices, int32_t slice_type, double c_prefactor
     int const type = slice_type & ~EIN_SPLIT_REFINE;
     if (type != EIN_SLICE_BF16 && type != EIN_SLICE_FP16 && type != EIN_SLICE_INT8 && type != EIN_SLICE_CRT)
         return ein::fail(-2, "einsum_split: bad slice type");
-    if (type == EIN_SLICE_CRT ? (nslices < 8 || nslices > 16) : (nslices < 3 || nslices > 6))
-        return ein::fail(-1, "einsum_split: nslices must be 3..6 (8..16 moduli for EIN_SLICE_CRT)");
-    ein::t_split_slices = nslices;
+    if (type == EIN_SLICE_CRT ? (nslices != 0 && (nslices < 8 || nslices > 16)) : (nslices < 3 || nslices > 6))
+        return ein::fail(-1, "einsum_split: nslices must be 3..6 (8..16 moduli, or 0 for automatic, with EIN_SLICE_CRT)");
+    ein::t_split_slices = nslices == 0 ? -1 : nslices;
     ein::t_split_type   = slice_type;
     int st = ein_einsum_execute_f64(EIN_ALG_INDETERMINATE, c_prefactor, C, c, ab_prefactor, A, a, B, b, algorithm_used,
                                     stream);

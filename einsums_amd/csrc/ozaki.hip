@@ -813,8 +813,9 @@ int gemm_split(GemmProblem<double> P, int nslices, int slice_type, hipStream_t s
         return fail(EIN_ERROR_VALUE,
                     "split gemm: slice_type must be EIN_SLICE_BF16, EIN_SLICE_FP16, EIN_SLICE_INT8 or EIN_SLICE_CRT");
     if (type == EIN_SLICE_CRT) {
-        if (nslices < CRT_MIN_MODULI || nslices > 16)
-            return fail(EIN_ERROR_VALUE, "split gemm: EIN_SLICE_CRT takes 8..16 moduli");
+        if (nslices != 0 && (nslices < CRT_MIN_MODULI || nslices > 16))
+            return fail(EIN_ERROR_VALUE, "split gemm: EIN_SLICE_CRT takes 8..16 moduli (0: automatic)");
+        if (nslices == 0) nslices = crt_auto_moduli(P.k);
     } else if (nslices < 3 || nslices > 6) {
         return fail(EIN_ERROR_VALUE, "split gemm: nslices must be 3..6");
     }

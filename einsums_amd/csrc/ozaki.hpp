@@ -43,5 +43,8 @@ constexpr int CRT_MIN_MODULI = 8;
 int gemm_split_crt(GemmProblem<double> P, int nmod, hipStream_t stream);
 // Bits kept per operand element for nmod moduli and contraction length k (-1 if invalid).
 int crt_bits(int nmod, int64_t k);
+// The fewest moduli (8..16) that keep at least CRT_AUTO_BITS bits at contraction length k.
+constexpr int CRT_AUTO_BITS = 40;
+int crt_auto_moduli(int64_t k);
 
 } // namespace ein

@@ -193,8 +193,12 @@ int ein_gemm_split_f64(char transa, char transb, int64_t m, int64_t n, int64_t k
                        const double *B, int64_t ldb, double beta, double *C, int64_t ldc, int32_t nslices,
                        int32_t slice_type, ein_stream_t stream);
 /* Bits kept per operand element by EIN_SLICE_CRT with nmoduli moduli at contraction length k
- * (-1 for an invalid count or k < 0). No device work. */
+ * (-1 for an invalid count or k < 0). nslices = 0 with EIN_SLICE_CRT picks the fewest moduli
+ * that keep at least 40 bits (relative representation error <= 2^-40 per element), which
+ * ein_split_crt_moduli(k) reports (12 at k = 2304, 13 at k = 12288 and 65536); nmoduli = 0
+ * here gives that choice's bits. No device work. */
 int ein_split_crt_bits(int32_t nmoduli, int64_t k);
+int ein_split_crt_moduli(int64_t k);
 /* C = alpha*op(A)*op(B) + beta*C with A, B on the device and C (column-major, ldc) in host
  * memory, for outputs larger than HBM (the out-of-core case of the v^4 CCD intermediates,
  * which the reference keeps as a DiskTensor, libs/Einsums/Tensor/include/Einsums/

@@ -182,3 +182,18 @@ def test_slice_type_codes():
     for bad in ("bf8", "crt+", "crt+refine+x", "int8+fast"):
         with pytest.raises(ValueError):
             core._slice_code(bad)
+
+
+def test_split_crt_auto_moduli_host_only():
+    """nslices = 0 with EIN_SLICE_CRT: the fewest moduli (8..16) keeping >= 40 bits; checked
+    against ein_split_crt_bits (itself checked against exact integers above)."""
+    from einsums_amd import _lib
+
+    for k in (1, 64, 1000, 2304, 12288, 65536, 70000, 1 << 22):
+        L = _lib.lib.ein_split_crt_moduli(k)
+        bits = [_lib.lib.ein_split_crt_bits(n, k) for n in range(8, 17)]
+        want = next((n for n, b in zip(range(8, 17), bits) if b >= 40), 16)
+        assert L == want, (k, L, want)
+        assert _lib.lib.ein_split_crt_bits(0, k) == _lib.lib.ein_split_crt_bits(L, k)
+    assert _lib.lib.ein_split_crt_moduli(2304) == 12 and _lib.lib.ein_split_crt_moduli(65536) == 13
+    assert _lib.lib.ein_split_crt_moduli(-1) == -1

@@ -197,3 +197,22 @@ def test_split_refine_einsum(core, oracle):
     err = rel_fro(host(dW) - arrs[init], ref - arrs[init])
     print(f"Wabef o=12 v=48 crt x13 + refine: {err:.3e}")
     assert err <= 1e-14
+
+
+@pytest.mark.parametrize("m,n,k", [(300, 257, 2304), (64, 80, 12288), (100, 64, 65536)])
+def test_crt_auto_moduli(core, oracle, m, n, k):
+    """split "auto" (nslices 0) runs the CRT scheme with ein_split_crt_moduli(k) moduli: same
+    bits as that explicit count, error within its tolerance."""
+    rng = seeded("auto", m, n, k)
+    A = rng.uniform(-1, 1, (m, k))
+    B = rng.uniform(-1, 1, (k, n))
+    L = core.crt_moduli(k)
+    outs = []
+    for ns in (0, L):
+        C = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+        core.gemm_split("N", "N", 1.0, dev(A), dev(B), 0.0, C, nslices=ns, slice_type="crt")
+        outs.append(host(C))
+    assert np.array_equal(outs[0], outs[1])
+    ref = np.zeros(m * n)
+    oracle.gemm_colmajor("N", "N", n, m, k, 1.0, B.ravel(), n, A.ravel(), k, 0.0, ref, n)
+    assert rel_fro(outs[0].ravel(), ref) <= max(8.0 * 2.0 ** -core.crt_bits(L, k), 2e-14)
