@@ -91,8 +91,7 @@ def test_4096_uneven_gemms_autotuned(oracle, dtype):
         dA, dB = dev(A), dev(B)
         dC = torch.full((m, n), float("nan"), dtype=tdt, device="cuda")
         terms.append(("ij", dC, "ik", dA, "kj", dB, 0.0, 1.0))
-        if k % 97 == 0:
-            checks.append((A, B, dC))
+        checks.append((A, B, dC))  # every output is checked
     plan = blocks.GroupedEinsum(terms, autotune=True)
     info = plan.info()
     assert info["waves"] == 1 and info["tile_rows"] in (64, 128)
