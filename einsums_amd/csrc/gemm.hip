@@ -349,8 +349,21 @@ TuneKey tune_key(GemmProblem<T> const &P) {
 
 // Candidate configurations: tile shape x schedule (hybrid stream-K or data-parallel) x
 // workgroups per CU the schedule plans for (all that fit, or one).
-std::vector<GemmConfig> tune_candidates() {
+// krow_both: both operands contiguous along k (row-major LDS staging), the case the 96 x 96
+// tile is built for (576 = 6 x 96: the Wmnij GEMM at o=24 has no ragged tiles).
+std::vector<GemmConfig> tune_candidates(bool krow_both) {
     std::vector<GemmConfig> c;
+    if (krow_both)
+        for (int dp : {0, 1, 2})
+            for (int bpc : {0, 1}) {
+                if (dp == 1 && bpc == 1) continue;
+                GemmConfig g;
+                g.bm            = 96;
+                g.bn            = 96;
+                g.data_parallel = dp;
+                g.blocks_per_cu = bpc;
+                c.push_back(g);
+            }
     for (int bm : {128, 64})
         for (int bn : {128, 64})
             for (int dp : {0, 1, 2})
@@ -386,6 +399,14 @@ int gemm_strided_impl(GemmProblem<T> P, GemmConfig const *force, hipStream_t str
 // the tuning run does not favour the early or the late candidates), each pass running it
 // for about 10 ms (3 to 200 launches) after a warm-up; its time is the smaller of the two.
 // P is normalised.
+// Whether the 96 x 96 tile applies: FP64 (its FP32 16-byte row-major staging is not
+// realisable at 96 rows) with both operands contiguous along k.
+template <typename T>
+bool krow_both(GemmProblem<T> const &P) {
+    return sizeof(T) == 8 && choose_staging<T>(96, P.a_sm, P.a_sk, P.a_sb, P.batch, P.m, P.k, P.A).krow &&
+           choose_staging<T>(96, P.b_sn, P.b_sk, P.b_sb, P.batch, P.n, P.k, P.B).krow;
+}
+
 template <typename T>
 int tune_gemm(GemmProblem<T> const &P, hipStream_t stream, GemmConfig *best) {
     size_t const c_elems = size_t(P.m) * size_t(P.n) * size_t(P.batch);
@@ -397,7 +418,7 @@ int tune_gemm(GemmProblem<T> const &P, hipStream_t stream, GemmConfig *best) {
     Q.c_sn           = P.m;
     Q.c_sb           = P.m * P.n;
     Q.beta           = P.beta;
-    std::vector<GemmConfig> const cands = tune_candidates();
+    std::vector<GemmConfig> const cands = tune_candidates(krow_both(P));
     std::vector<float> best_of(cands.size(), 0.f);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int st = EIN_SUCCESS;
@@ -476,7 +497,7 @@ int gemm_strided_impl(GemmProblem<T> P, GemmConfig const *force, hipStream_t str
             if (ws_bytes != nullptr) {
                 // an untuned key in tuning mode may run any candidate: report the largest need
                 size_t most = 0;
-                for (GemmConfig const &c : tune_candidates()) {
+                for (GemmConfig const &c : tune_candidates(krow_both(P))) {
                     size_t b = 0;
                     EIN_TRY(gemm_strided_impl<T>(P, &c, stream, &b));
                     most = std::max(most, b);
@@ -503,6 +524,11 @@ int gemm_strided_impl(GemmProblem<T> P, GemmConfig const *force, hipStream_t str
     }
     P.force_dp     = cfg.data_parallel;
     constexpr int BK = 16;
+    if (cfg.bm == 96 && cfg.bn == 96) {
+        if (!krow_both(P))
+            return fail(EIN_ERROR_UNSUPPORTED, "gemm: 96 x 96 tiles need FP64 operands contiguous along k");
+        return launch_dense<T, 96, 96, BK>(P, cfg.blocks_per_cu, stream, ws_bytes);
+    }
     if (cfg.bm == 128 && cfg.bn == 128) return launch_dense<T, 128, 128, BK>(P, cfg.blocks_per_cu, stream, ws_bytes);
     if (cfg.bm == 128 && cfg.bn == 64) return launch_dense<T, 128, 64, BK>(P, cfg.blocks_per_cu, stream, ws_bytes);
     if (cfg.bm == 64 && cfg.bn == 128) return launch_dense<T, 64, 128, BK>(P, cfg.blocks_per_cu, stream, ws_bytes);
@@ -776,7 +802,8 @@ int ein_gemm_tuning_load(const char *path) {
         long long m, n, k, b;
         if (std::sscanf(line, "%d %lld %lld %lld %lld %d %d : %d %d %d %d", &eb, &m, &n, &k, &b, &ak, &bk, &bm, &bn, &dp,
                         &bpc) != 11 ||
-            (eb != 4 && eb != 8) || (bm != 64 && bm != 128) || (bn != 64 && bn != 128) || dp < 0 || dp > 2 || bpc < 0 ||
+            (eb != 4 && eb != 8) || (bm != 64 && bm != 96 && bm != 128) || (bn != 64 && bn != 96 && bn != 128) || (bm == 96) != (bn == 96) ||
+            dp < 0 || dp > 2 || bpc < 0 ||
             m <= 0 || n <= 0 || k <= 0 || b <= 0) {
             std::fclose(f);
             return ein::fail(EIN_ERROR_VALUE, "gemm tuning: malformed line " + std::to_string(lineno) + " in " + path);

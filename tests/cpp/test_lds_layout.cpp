@@ -86,6 +86,10 @@ int main() {
     check_layout<double, 64, 16, false, true>("f64 64 kmajor x4");
     check_layout<double, 64, 16, true, false>("f64 64 krow dword");
     check_layout<double, 64, 16, true, true>("f64 64 krow x4");
+    check_layout<double, 96, 16, true, false>("f64 96 krow dword");
+    check_layout<double, 96, 16, true, true>("f64 96 krow x4");
+    check_layout<float, 96, 16, true, false>("f32 96 krow dword");
+    check_layout<float, 96, 16, true, true>("f32 96 krow x4");
     check_layout<float, 128, 16, false, false>("f32 128 kmajor dword");
     check_layout<float, 128, 16, false, true>("f32 128 kmajor x4");
     check_layout<float, 128, 16, true, false>("f32 128 krow dword");

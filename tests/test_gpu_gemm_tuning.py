@@ -82,3 +82,36 @@ def test_split_k_schedule_from_table(core, oracle, tmp_path, m, n, k, tile, bpc)
     core.clear_gemm_tuning()
     assert rel_fro(outs[0].ravel(), ref) < 1e-13
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("sched", [0, 1, 2])
+@pytest.mark.parametrize("o,v", [(12, 48), (24, 96), (7, 20)])
+def test_96_tile_wmnij_from_table(core, oracle, tmp_path, sched, o, v):
+    """The 96 x 96 tile (both operands contiguous along k) forced through a loaded table entry
+    on the Wmnij pattern mnij <- ijef, mnef: matches the oracle, repeats bit for bit."""
+    rng = np.random.default_rng(o * 100 + v)
+    t = rng.uniform(-1, 1, (o, o, v, v))
+    g = rng.uniform(-1, 1, (o, o, v, v))
+    W0 = rng.uniform(-1, 1, (o, o, o, o))
+    dt, dg = dev(t), dev(g)
+    core.clear_gemm_tuning()
+    with core.gemm_tuning():
+        W = dev(W0)
+        core.einsum("mnij", W, "ijef", dt, "mnef", dg, C_prefactor=1.0, AB_prefactor=0.25)
+    f = tmp_path / "t.txt"
+    core.save_gemm_tuning(f)
+    lines = [ln for ln in f.read_text().splitlines() if ln and not ln.startswith("#")]
+    assert len(lines) == 1
+    f.write_text(lines[0].split(":")[0] + f": 96 96 {sched} 0\n")
+    core.clear_gemm_tuning()
+    core.load_gemm_tuning(f)
+    outs = []
+    for _ in range(2):
+        W = dev(W0)
+        core.einsum("mnij", W, "ijef", dt, "mnef", dg, C_prefactor=1.0, AB_prefactor=0.25)
+        outs.append(host(W))
+    core.clear_gemm_tuning()
+    ref = W0.copy()
+    oracle.einsum(1.0, "mnij", ref, 0.25, "ijef", t, "mnef", g)
+    assert rel_fro(outs[0], ref) < 1e-13
+    assert np.array_equal(outs[0], outs[1])
