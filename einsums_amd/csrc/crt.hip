@@ -64,7 +64,8 @@ struct CrtBasis {
 
 // ------------------------------------------------------------------ residues
 // 64 x 64 tiles of X staged through LDS so that the read follows X's unit-stride axis;
-// residues written along p: out[l][r][kp].
+// residues written along p: out[l][r][kp]. Each thread takes four consecutive p of four rows
+// and writes one 32-bit word (four residues) per row and modulus.
 template <bool ROWS_FAST>
 __global__ __launch_bounds__(256) void crt_residue_kernel(SliceSrc S, int32_t const *__restrict__ e, CrtMods K,
                                                           int8_t *__restrict__ out) {
@@ -90,23 +91,33 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(SliceSrc S, int32_t co
         }
     }
     __syncthreads();
-    int const lp    = t & 63;
-    int64_t const p = p0 + lp;
+    int const p4    = (t & 15) * 4;
+    int64_t const p = p0 + p4; // kp is a multiple of 64: p .. p+3 < kp whenever p < kp
     if (p >= S.kp) return;
 #pragma unroll 1
-    for (int q = 0; q < 16; ++q) {
-        int lr    = (t >> 6) + 4 * q;
-        int64_t r = r0 + lr;
+    for (int q = 0; q < 4; ++q) {
+        int const lr    = (t >> 4) + 16 * q;
+        int64_t const r = r0 + lr;
         if (r >= S.rows) continue;
-        double const x = rint(ldexp(tile[lr][lp], K.kbits - e[r])); // |x| <= 2^kb, exact
+        int const sh = K.kbits - e[r];
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = rint(ldexp(tile[lr][p4 + u], sh)); // |x| <= 2^kb, exact
+        int8_t *__restrict__ dst = out + r * S.kp + p;
 #pragma unroll 1
         for (int l = 0; l < K.nmod; ++l) {
-            double const m = double(K.mod[l]);
-            double v       = fma(-rint(x * K.rinv[l]), m, x); // x - m*q: an exact small integer
+            double const m  = double(K.mod[l]);
             double const hi = K.mod[l] == 256 ? 127.0 : 0.5 * (m - 1.0);
-            if (v > hi) v -= m;
-            if (v < -hi - (K.mod[l] == 256 ? 1.0 : 0.0)) v += m;
-            out[(int64_t(l) * S.rows + r) * S.kp + p] = static_cast<int8_t>(v);
+            double const lo = K.mod[l] == 256 ? -128.0 : -hi;
+            uint32_t word   = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                double v = fma(-rint(x[u] * K.rinv[l]), m, x[u]); // x - m*q: an exact small integer
+                v        = v > hi ? v - m : v;
+                v        = v < lo ? v + m : v;
+                word |= uint32_t(uint8_t(int8_t(v))) << (8 * u);
+            }
+            *reinterpret_cast<uint32_t *>(dst + int64_t(l) * S.rows * S.kp) = word;
         }
     }
 }
