@@ -27,7 +27,10 @@
 // int32 accumulator per output: the GEMM here can use 256 x 256 tiles and a four-deep
 // LDS ring, where six slices leave room for 128 x 64 tiles and two stages.
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "ozaki.hpp"
@@ -434,6 +437,43 @@ int crt_residues(SliceSrc const &S, CrtMods const &K, int32_t *e, int8_t *out, d
     return EIN_SUCCESS;
 }
 
+// The side stream of the current device (created once, default priority) and four events
+// of this host thread on it.
+struct CrtSide {
+    hipStream_t aux = nullptr;
+    hipEvent_t gemm_done[2]  = {nullptr, nullptr};
+    hipEvent_t recon_done[2] = {nullptr, nullptr};
+};
+
+int crt_side(CrtSide &out) {
+    int dev = 0;
+    EIN_HIP_CHECK(hipGetDevice(&dev));
+    {
+        static std::mutex mu;
+        static std::map<int, hipStream_t> aux;
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = aux.find(dev);
+        if (it == aux.end()) {
+            hipStream_t s = nullptr;
+            EIN_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            it = aux.emplace(dev, s).first;
+        }
+        out.aux = it->second;
+    }
+    thread_local std::map<int, std::array<hipEvent_t, 4>> events;
+    auto it = events.find(dev);
+    if (it == events.end()) {
+        std::array<hipEvent_t, 4> e{};
+        for (auto &x : e) EIN_HIP_CHECK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+        it = events.emplace(dev, e).first;
+    }
+    out.gemm_done[0]  = it->second[0];
+    out.gemm_done[1]  = it->second[1];
+    out.recon_done[0] = it->second[2];
+    out.recon_done[1] = it->second[3];
+    return EIN_SUCCESS;
+}
+
 } // namespace
 
 // Residue planes are written for column panels of at most this many bytes.
@@ -452,9 +492,18 @@ int gemm_split_crt(GemmProblem<double> P, int nmod, hipStream_t stream) {
 
     uint32_t const tiles_m = uint32_t(ceil_div(P.m, CG_TILE));
     int64_t const ldr      = int64_t(tiles_m) * CG_TILE;
-    // column panel: whole 256-column tiles, residue planes within CRT_PANEL_BYTES
-    int64_t pn = int64_t(CRT_PANEL_BYTES / (size_t(nmod) * size_t(ldr))) / CG_TILE * CG_TILE;
-    pn         = std::max<int64_t>(CG_TILE, std::min<int64_t>(pn, ceil_div(P.n, CG_TILE) * CG_TILE));
+    // column panels of whole 256-column tiles whose residue planes take at most
+    // CRT_PANEL_BYTES; with more than one panel, two plane buffers of half that so the
+    // reconstruction of panel p (HBM-bound) runs on a side stream while the residue GEMM of
+    // panel p+1 (MFMA-bound) runs on the caller's stream
+    auto panel_cols = [&](size_t budget) {
+        int64_t c = int64_t(budget / (size_t(nmod) * size_t(ldr))) / CG_TILE * CG_TILE;
+        return std::max<int64_t>(CG_TILE, std::min<int64_t>(c, ceil_div(P.n, CG_TILE) * CG_TILE));
+    };
+    int64_t pn      = panel_cols(CRT_PANEL_BYTES);
+    bool const fork = pn < P.n && !stream_capturing(stream);
+    if (fork) pn = panel_cols(CRT_PANEL_BYTES / 2);
+    int const nbuf = fork ? 2 : 1;
 
     size_t const a_bytes = size_t(nmod) * size_t(P.m) * size_t(kp);
     size_t const b_bytes = size_t(nmod) * size_t(P.n) * size_t(kp);
@@ -462,7 +511,7 @@ int gemm_split_crt(GemmProblem<double> P, int nmod, hipStream_t stream) {
     auto up              = [](size_t x) { return (x + 255) / 256 * 256; };
     size_t const b_off   = up(a_bytes);
     size_t const r_off   = b_off + up(b_bytes);
-    size_t const ea_off  = r_off + up(r_bytes);
+    size_t const ea_off  = r_off + size_t(nbuf) * up(r_bytes);
     size_t const eb_off  = ea_off + up(size_t(P.m) * 4);
     size_t const x_off   = eb_off + up(size_t(P.n) * 4);
     size_t const total   = x_off + row_exp_scratch(std::max(P.m, P.n)) * sizeof(double);
@@ -481,8 +530,15 @@ int gemm_split_crt(GemmProblem<double> P, int nmod, hipStream_t stream) {
     EIN_TRY(crt_residues(SA, K, ea, As, scratch, stream));
     EIN_TRY(crt_residues(SB, K, eb, Bs, scratch, stream));
 
-    for (int64_t j0 = 0; j0 < P.n; j0 += pn) {
+    CrtSide side;
+    if (fork) EIN_TRY(crt_side(side));
+    int last = 0;
+    for (int64_t j0 = 0, panel = 0; j0 < P.n; j0 += pn, ++panel) {
         int64_t const nj = std::min(pn, P.n - j0);
+        int const b      = fork ? int(panel & 1) : 0;
+        uint8_t *Rb      = R + size_t(b) * up(r_bytes);
+        last             = b;
+        if (fork && panel >= 2) EIN_HIP_CHECK(hipStreamWaitEvent(stream, side.recon_done[b], 0)); // buffer b free
         CrtGemm G{};
         G.m       = P.m;
         G.n       = nj;
@@ -491,7 +547,7 @@ int gemm_split_crt(GemmProblem<double> P, int nmod, hipStream_t stream) {
         G.b_plane = P.n * kp;
         G.A       = As;
         G.B       = Bs + j0 * kp;
-        G.R       = R;
+        G.R       = Rb;
         G.ldr     = ldr;
         G.r_plane = nj * ldr;
         G.tiles_m = tiles_m;
@@ -507,10 +563,16 @@ int gemm_split_crt(GemmProblem<double> P, int nmod, hipStream_t stream) {
             EIN_LAUNCH_CHECK();
         }
 
+        hipStream_t rs = stream;
+        if (fork) {
+            EIN_HIP_CHECK(hipEventRecord(side.gemm_done[b], stream));
+            EIN_HIP_CHECK(hipStreamWaitEvent(side.aux, side.gemm_done[b], 0));
+            rs = side.aux;
+        }
         CrtRec Q{};
         Q.m       = P.m;
         Q.n       = nj;
-        Q.R       = R;
+        Q.R       = Rb;
         Q.ldr     = ldr;
         Q.r_plane = nj * ldr;
         Q.ea      = ea;
@@ -523,9 +585,12 @@ int gemm_split_crt(GemmProblem<double> P, int nmod, hipStream_t stream) {
         Q.W       = W;
         int64_t const threads = nj * (ldr / 4);
         if (ceil_div(threads, 256) > 0x7fffffff) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: panel too large");
-        crt_reconstruct_kernel<<<uint32_t(ceil_div(threads, 256)), 256, 0, stream>>>(Q);
+        crt_reconstruct_kernel<<<uint32_t(ceil_div(threads, 256)), 256, 0, rs>>>(Q);
         EIN_LAUNCH_CHECK();
+        if (fork) EIN_HIP_CHECK(hipEventRecord(side.recon_done[b], side.aux));
     }
+    // join: the side stream runs the reconstructions in order, so the last one covers all
+    if (fork) EIN_HIP_CHECK(hipStreamWaitEvent(stream, side.recon_done[last], 0));
     return EIN_SUCCESS;
 }
 

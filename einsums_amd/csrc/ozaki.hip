@@ -346,20 +346,29 @@ __global__ __launch_bounds__(256) void slice_i8_kernel(SliceSrc S, int32_t const
         }
     }
     __syncthreads();
-    int const lp    = t & 63;
-    int64_t const p = p0 + lp;
+    // four consecutive p of four rows per thread: one 32-bit store per row and slice
+    int const p4    = (t & 15) * 4;
+    int64_t const p = p0 + p4; // kp is a multiple of 64: p .. p+3 < kp whenever p < kp
     if (p >= S.kp) return;
 #pragma unroll 1
-    for (int q = 0; q < 16; ++q) {
-        int lr    = (t >> 6) + 4 * q;
-        int64_t r = r0 + lr;
+    for (int q = 0; q < 4; ++q) {
+        int const lr    = (t >> 4) + 16 * q;
+        int64_t const r = r0 + lr;
         if (r >= S.rows) continue;
-        double x = ldexp(tile[lr][lp], -e[r]);
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = ldexp(tile[lr][p4 + u], -e[r]);
+        int8_t *__restrict__ dst = out + r * S.kp + p;
         for (int s = 0; s < ns; ++s) {
-            double y = ldexp(x, I8_BETA);
-            double d = rint(y);
-            x        = y - d;
-            out[(int64_t(s) * S.rows + r) * S.kp + p] = static_cast<int8_t>(d);
+            uint32_t word = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                double const y = ldexp(x[u], I8_BETA);
+                double const d = rint(y);
+                x[u]           = y - d;
+                word |= uint32_t(uint8_t(static_cast<int8_t>(d))) << (8 * u);
+            }
+            *reinterpret_cast<uint32_t *>(dst + int64_t(s) * S.rows * S.kp) = word;
         }
     }
 }
