@@ -56,6 +56,9 @@ def parse():
                         "the tuning table to PATH and exit")
     p.add_argument("--tuning-table", type=str, default=str(ROOT / "einsums_amd" / "gemm_tuning_mi355x.txt"),
                    help="GEMM tuning table loaded before the timed steps (skipped when the file is absent)")
+    p.add_argument("--out-of-core", type=str, default="", metavar="DIR",
+                   help="only run the Wabef update with C in HBM, in pinned host memory and in an HDF5 disk "
+                        "tensor under DIR (o=24 v=96 and o=32 v=128; one JSON line each) and exit")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES bytes (counter calibration) and exit")
     return p.parse_args()
@@ -550,6 +553,78 @@ def wmbej_case(timed, o=48, v=256):
     return row
 
 
+def out_of_core_bench(dirpath: str, reps: int = 3) -> None:
+    """Wabef += 0.25 t_mnab g_mnef (FP64, beta = 1) with C resident in HBM, in pinned host
+    memory (core.gemm_host_c, panels over PCIe) and in an HDF5 disk tensor of shape
+    (v, v, v, v) under `dirpath` (disk.gemm_disk_c, panels disk -> host -> device and back).
+    The time of each is the median of `reps` calls after one warm-up; `c_gbs` is C's read
+    plus write bytes over that time. The streamed results are compared with the HBM one."""
+    import numpy as np
+    import torch
+
+    import einsums_amd.core as core
+    from einsums_amd import disk
+
+    for o, v in ((24, 96), (32, 128)):
+        rng = np.random.default_rng(3)
+        t = torch.from_numpy(rng.uniform(-1, 1, (o * o, v * v))).cuda()
+        g = torch.from_numpy(rng.uniform(-1, 1, (o * o, v * v))).cuda()
+        W0 = torch.from_numpy(rng.uniform(-1, 1, (v * v, v * v)))
+        c_bytes = 2 * W0.numel() * 8
+        flop = 2.0 * (v * v) ** 2 * o * o
+
+        def timed(fn):
+            fn()
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            return float(np.median(ts))
+
+        Wd = W0.cuda()
+        plan = core.compile_plan("abef", "mnab", "mnef")
+        t4, g4 = t.view(o, o, v, v), g.view(o, o, v, v)
+        W4 = Wd.view(v, v, v, v)
+        t_hbm = timed(lambda: plan.execute(1.0, W4, 0.25, t4, g4))
+        Wd.copy_(W0.cuda())
+        plan.execute(1.0, W4, 0.25, t4, g4)
+        ref = Wd.cpu().numpy()
+        Wh = torch.empty_like(W0).pin_memory()
+
+        def host_call():
+            core.gemm_host_c("T", "N", 0.25, t, g, 1.0, Wh)
+
+        t_host = timed(host_call)
+        Wh.copy_(W0)
+        host_call()
+        err_host = float(np.linalg.norm(Wh.numpy() - ref) / np.linalg.norm(ref))
+        path = Path(dirpath) / f"wabef_o{o}_v{v}.h5"
+        if path.exists():
+            path.unlink()
+        Wk = disk.DiskTensor(path, "Wabef", (v, v, v, v))
+
+        def disk_call():
+            disk.gemm_disk_c("T", "N", 0.25, t, g, 1.0, Wk)
+
+        Wk.put(W0.numpy().reshape(v, v, v, v))
+        t_disk = timed(disk_call)
+        Wk.put(W0.numpy().reshape(v, v, v, v))
+        disk_call()
+        err_disk = float(np.linalg.norm(Wk.get().reshape(v * v, v * v) - ref) / np.linalg.norm(ref))
+        path.unlink()
+        row = {"case": f"Wabef o={o} v={v} FP64 beta=1", "c_mb": round(W0.numel() * 8 / 1e6, 1),
+               "hbm_ms": round(t_hbm * 1e3, 2), "hbm_tflops": round(flop / t_hbm / 1e12, 2),
+               "host_ms": round(t_host * 1e3, 2), "host_c_gbs": round(c_bytes / t_host / 1e9, 2),
+               "host_rel_err_vs_hbm": float(f"{err_host:.2e}"),
+               "disk_ms": round(t_disk * 1e3, 2), "disk_c_gbs": round(c_bytes / t_disk / 1e9, 2),
+               "disk_rel_err_vs_hbm": float(f"{err_disk:.2e}"), "disk_dir": str(dirpath)}
+        print(json.dumps({"out_of_core": row}), flush=True)
+        del t, g, W0, Wd, Wh
+
+
 def grouped_bench(reps=5):
     """Batched uneven blocks: (1) per-irrep CCD W intermediates of 8 irreps (o_h in [2,24],
     v_h in [8,96]) as one grouped plan vs one einsum call per block (the reference's block
@@ -680,6 +755,9 @@ def main():
         return
     if args.bandwidth:
         bandwidth_bench()
+        return
+    if args.out_of_core:
+        out_of_core_bench(args.out_of_core)
         return
     if args.grouped:
         print(json.dumps({"grouped": grouped_bench()}))

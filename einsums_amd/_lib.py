@@ -158,6 +158,17 @@ def _load():
                                            ctypes.c_int32, _vp])
     sig["ein_gemm_host_c_f32"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_float, _dp, _i64,
                                            _dp, _i64, ctypes.c_float, _vp, _i64, ctypes.c_size_t, _vp])
+    _i64p = ctypes.POINTER(ctypes.c_int64)
+    for sfx in ("f64", "f32"):
+        sig[f"ein_disk_create_{sfx}"] = (_c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32, _i64p])
+        sig[f"ein_disk_write_{sfx}"] = (_c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32, _i64p, _vp])
+        sig[f"ein_disk_read_{sfx}"] = (_c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32, _i64p, _vp])
+    sig["ein_gemm_disk_c_f64"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64,
+                                           _dp, _i64, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
+                                           ctypes.c_size_t, ctypes.c_int32, ctypes.c_int32, _vp])
+    sig["ein_gemm_disk_c_f32"] = (_c_int, [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_float, _dp, _i64,
+                                           _dp, _i64, ctypes.c_float, ctypes.c_char_p, ctypes.c_char_p,
+                                           ctypes.c_size_t, _vp])
     sig["ein_einsum_split_f64"] = (_c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _desc_p, _dp, ctypes.c_double,
                                             _desc_p, _dp, _desc_p, _dp, ctypes.POINTER(ctypes.c_int32), _vp])
     sig["ein_grouped_plan_create"] = (_c_int, [ctypes.c_int32, ctypes.POINTER(ContractionItem), ctypes.c_int32,

@@ -1,0 +1,481 @@
+// Out-of-core tensors in HDF5 files, and the GEMM whose output C lives in one.
+//
+// The reference keeps tensors that do not fit in memory as DiskTensor: one chunked HDF5
+// dataset per tensor, its dims in row-major order, created on construction when absent and
+// opened otherwise (libs/Einsums/Tensor/include/Einsums/Tensor/DiskTensor.hpp:200-306). This
+// file reads and writes such datasets and runs C = alpha*op(A)*op(B) + beta*C with C in one:
+// the v^4 CCD intermediate beyond HBM, streamed through the GPU in panels. The column-major
+// m x n C of the GEMM is the dataset's row-major [n][m] view, so dataset row j is C's column j.
+//
+// libhdf5 is loaded at run time (dlopen), so the library has no link-time dependency on it:
+// $EINSUMS_AMD_HDF5, then libhdf5.so on the loader path, then the conda location. HDF5 is
+// not thread-safe in its default build; every call here holds one process-wide lock.
+#include <dlfcn.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "ein_common.hpp"
+
+namespace ein {
+namespace {
+
+// HDF5 1.10+ C types and the constants used here (H5Ipublic.h, H5public.h, H5Fpublic.h,
+// H5Ppublic.h, H5Spublic.h, H5Tpublic.h).
+using hid_t   = int64_t;
+using herr_t  = int;
+using htri_t  = int;
+using hsize_t = unsigned long long;
+constexpr hid_t kDefault       = 0;       // H5P_DEFAULT, H5S_ALL, H5E_DEFAULT
+constexpr unsigned kAccRdonly  = 0x0000u; // H5F_ACC_RDONLY
+constexpr unsigned kAccRdwr    = 0x0001u; // H5F_ACC_RDWR
+constexpr unsigned kAccExcl    = 0x0004u; // H5F_ACC_EXCL
+constexpr int kSelectSet       = 0;       // H5S_SELECT_SET
+constexpr int kClassFloat      = 1;       // H5T_FLOAT
+constexpr size_t kChunkBytes   = size_t(16) << 20;
+
+struct H5 {
+    herr_t (*open)();
+    herr_t (*eset_auto)(hid_t, void *, void *);
+    hid_t (*fopen)(char const *, unsigned, hid_t);
+    hid_t (*fcreate)(char const *, unsigned, hid_t, hid_t);
+    herr_t (*fclose)(hid_t);
+    htri_t (*lexists)(hid_t, char const *, hid_t);
+    hid_t (*dopen)(hid_t, char const *, hid_t);
+    hid_t (*dcreate)(hid_t, char const *, hid_t, hid_t, hid_t, hid_t, hid_t);
+    herr_t (*dclose)(hid_t);
+    hid_t (*dget_space)(hid_t);
+    hid_t (*dget_type)(hid_t);
+    herr_t (*dread)(hid_t, hid_t, hid_t, hid_t, hid_t, void *);
+    herr_t (*dwrite)(hid_t, hid_t, hid_t, hid_t, hid_t, void const *);
+    hid_t (*screate)(int, hsize_t const *, hsize_t const *);
+    int (*sndims)(hid_t);
+    int (*sdims)(hid_t, hsize_t *, hsize_t *);
+    herr_t (*shyperslab)(hid_t, int, hsize_t const *, hsize_t const *, hsize_t const *, hsize_t const *);
+    herr_t (*sclose)(hid_t);
+    int (*tclass)(hid_t);
+    size_t (*tsize)(hid_t);
+    herr_t (*tclose)(hid_t);
+    hid_t (*pcreate)(hid_t);
+    herr_t (*pchunk)(hid_t, int, hsize_t const *);
+    herr_t (*pclose)(hid_t);
+    hid_t const *native_double;
+    hid_t const *native_float;
+    hid_t const *dcpl_class;
+};
+
+std::mutex g_h5_mutex; // every HDF5 call runs under it
+
+// The loaded library, or nullptr with the reason in *why.
+H5 const *h5_lib(std::string *why) {
+    static H5 lib;
+    static int state = 0; // 0 not tried, 1 loaded, -1 failed
+    static std::string reason;
+    if (state == 0) {
+        std::vector<std::string> names;
+        if (char const *env = std::getenv("EINSUMS_AMD_HDF5")) names.push_back(env);
+        names.insert(names.end(), {"libhdf5.so", "libhdf5_serial.so", "/opt/conda/lib/libhdf5.so"});
+        void *h = nullptr;
+        for (auto const &n : names)
+            if ((h = dlopen(n.c_str(), RTLD_NOW | RTLD_LOCAL)) != nullptr) break;
+        state = -1;
+        if (h == nullptr) {
+            reason = "HDF5 library not found (set EINSUMS_AMD_HDF5 to the path of libhdf5.so)";
+        } else {
+            bool ok    = true;
+            auto bind  = [&](auto &fn, char const *sym) {
+                fn = reinterpret_cast<std::remove_reference_t<decltype(fn)>>(dlsym(h, sym));
+                if (fn == nullptr) {
+                    ok     = false;
+                    reason = std::string("HDF5 library lacks ") + sym;
+                }
+            };
+            bind(lib.open, "H5open");
+            bind(lib.eset_auto, "H5Eset_auto2");
+            bind(lib.fopen, "H5Fopen");
+            bind(lib.fcreate, "H5Fcreate");
+            bind(lib.fclose, "H5Fclose");
+            bind(lib.lexists, "H5Lexists");
+            bind(lib.dopen, "H5Dopen2");
+            bind(lib.dcreate, "H5Dcreate2");
+            bind(lib.dclose, "H5Dclose");
+            bind(lib.dget_space, "H5Dget_space");
+            bind(lib.dget_type, "H5Dget_type");
+            bind(lib.dread, "H5Dread");
+            bind(lib.dwrite, "H5Dwrite");
+            bind(lib.screate, "H5Screate_simple");
+            bind(lib.sndims, "H5Sget_simple_extent_ndims");
+            bind(lib.sdims, "H5Sget_simple_extent_dims");
+            bind(lib.shyperslab, "H5Sselect_hyperslab");
+            bind(lib.sclose, "H5Sclose");
+            bind(lib.tclass, "H5Tget_class");
+            bind(lib.tsize, "H5Tget_size");
+            bind(lib.tclose, "H5Tclose");
+            bind(lib.pcreate, "H5Pcreate");
+            bind(lib.pchunk, "H5Pset_chunk");
+            bind(lib.pclose, "H5Pclose");
+            bind(lib.native_double, "H5T_NATIVE_DOUBLE_g");
+            bind(lib.native_float, "H5T_NATIVE_FLOAT_g");
+            bind(lib.dcpl_class, "H5P_CLS_DATASET_CREATE_ID_g");
+            if (ok && lib.open() < 0) {
+                ok     = false;
+                reason = "H5open failed";
+            }
+            if (ok) {
+                lib.eset_auto(kDefault, nullptr, nullptr); // errors come back as return codes
+                state = 1;
+            }
+        }
+    }
+    if (state != 1) {
+        if (why) *why = reason;
+        return nullptr;
+    }
+    return &lib;
+}
+
+// An HDF5 identifier closed by its close function on scope exit.
+struct Hid {
+    hid_t id = -1;
+    herr_t (*close)(hid_t) = nullptr;
+    Hid() = default;
+    Hid(hid_t i, herr_t (*c)(hid_t)) : id(i), close(c) {}
+    Hid(Hid const &)            = delete;
+    Hid &operator=(Hid const &) = delete;
+    Hid &operator=(Hid &&o) noexcept {
+        if (this != &o) {
+            reset();
+            id      = o.id;
+            close   = o.close;
+            o.id    = -1;
+        }
+        return *this;
+    }
+    ~Hid() { reset(); }
+    void reset() {
+        if (id >= 0 && close) close(id);
+        id = -1;
+    }
+    bool ok() const { return id >= 0; }
+};
+
+template <typename T>
+hid_t native_type(H5 const &h) {
+    return sizeof(T) == 8 ? *h.native_double : *h.native_float;
+}
+
+bool file_exists(char const *path) {
+    struct stat st;
+    return ::stat(path, &st) == 0;
+}
+
+int check_names(char const *path, char const *name) {
+    if (path == nullptr || *path == '\0') return fail(EIN_ERROR_VALUE, "disk tensor: empty file path");
+    if (name == nullptr || *name == '\0') return fail(EIN_ERROR_VALUE, "disk tensor: empty dataset name");
+    return EIN_SUCCESS;
+}
+
+int check_dims(int rank, int64_t const *dims) {
+    if (rank < 1 || rank > 32) return fail(EIN_ERROR_RANK, "disk tensor: rank must be 1..32");
+    if (dims == nullptr) return fail(EIN_ERROR_VALUE, "disk tensor: dims is null");
+    for (int i = 0; i < rank; ++i)
+        if (dims[i] < 0) return fail(EIN_ERROR_DIMENSION, "disk tensor: negative extent");
+    return EIN_SUCCESS;
+}
+
+// Checks that dataset `d` holds FP elements of sizeof(T) with the given dims (rank < 0:
+// any; its dims are then returned in *out).
+template <typename T>
+int check_dataset(H5 const &h, hid_t d, int rank, int64_t const *dims, std::vector<int64_t> *out, std::string const &what) {
+    Hid type(h.dget_type(d), h.tclose);
+    if (!type.ok() || h.tclass(type.id) != kClassFloat || h.tsize(type.id) != sizeof(T))
+        return fail(EIN_ERROR_VALUE, what + ": dataset elements are not " + (sizeof(T) == 8 ? "FP64" : "FP32"));
+    Hid space(h.dget_space(d), h.sclose);
+    int const nd = space.ok() ? h.sndims(space.id) : -1;
+    if (nd < 0) return fail(EIN_ERROR_VALUE, what + ": cannot read the dataset's extents");
+    std::vector<hsize_t> have(size_t(std::max(nd, 1)));
+    if (nd > 0 && h.sdims(space.id, have.data(), nullptr) < 0) return fail(EIN_ERROR_VALUE, what + ": cannot read the dataset's extents");
+    if (rank >= 0) {
+        if (nd != rank) return fail(EIN_ERROR_RANK, what + ": dataset rank differs");
+        for (int i = 0; i < rank; ++i)
+            if (int64_t(have[size_t(i)]) != dims[i]) return fail(EIN_ERROR_DIMENSION, what + ": dataset extents differ");
+    }
+    if (out) out->assign(have.begin(), have.begin() + nd);
+    return EIN_SUCCESS;
+}
+
+// Opens `path` (read-write, or read-only), creating it when `create` and it is absent.
+int open_file(H5 const &h, char const *path, bool write, bool create, Hid &f) {
+    if (file_exists(path))
+        f = Hid(h.fopen(path, write ? kAccRdwr : kAccRdonly, kDefault), h.fclose);
+    else if (create)
+        f = Hid(h.fcreate(path, kAccExcl, kDefault, kDefault), h.fclose);
+    else
+        return fail(EIN_ERROR_VALUE, std::string("disk tensor: no file ") + path);
+    if (!f.ok()) return fail(EIN_ERROR_VALUE, std::string("disk tensor: cannot open ") + path + " as HDF5");
+    return EIN_SUCCESS;
+}
+
+// Opens dataset `name`, creating it (chunked, zero-filled on first read) when absent. The
+// chunk is one leading slice, cut down to about 16 MiB by taking single indices of the next
+// dims, so the GEMM's panels (whole leading slices) read and write whole chunks.
+template <typename T>
+int open_or_create(H5 const &h, hid_t file, char const *name, int rank, int64_t const *dims, Hid &d) {
+    htri_t const ex = h.lexists(file, name, kDefault);
+    if (ex > 0) {
+        d = Hid(h.dopen(file, name, kDefault), h.dclose);
+        if (!d.ok()) return fail(EIN_ERROR_VALUE, std::string("disk tensor: cannot open dataset ") + name);
+        return check_dataset<T>(h, d.id, rank, dims, nullptr, std::string("disk tensor ") + name);
+    }
+    std::vector<hsize_t> dd(static_cast<size_t>(rank)), chunk(static_cast<size_t>(rank));
+    for (int i = 0; i < rank; ++i) dd[size_t(i)] = hsize_t(dims[i]);
+    bool empty = false;
+    for (int i = 0; i < rank; ++i) {
+        chunk[size_t(i)] = std::max<hsize_t>(1, dd[size_t(i)]);
+        empty            = empty || dims[i] == 0;
+    }
+    chunk[0]     = 1;
+    size_t bytes = sizeof(T);
+    for (int i = 1; i < rank; ++i) bytes *= size_t(chunk[size_t(i)]);
+    for (int i = 1; i < rank - 1 && bytes > kChunkBytes; ++i) {
+        bytes /= size_t(chunk[size_t(i)]);
+        chunk[size_t(i)] = 1;
+    }
+    Hid space(h.screate(rank, dd.data(), nullptr), h.sclose);
+    Hid dcpl(h.pcreate(*h.dcpl_class), h.pclose);
+    if (!space.ok() || !dcpl.ok() || (!empty && h.pchunk(dcpl.id, rank, chunk.data()) < 0))
+        return fail(EIN_ERROR_VALUE, std::string("disk tensor: cannot describe dataset ") + name);
+    d = Hid(h.dcreate(file, name, native_type<T>(h), space.id, kDefault, dcpl.id, kDefault), h.dclose);
+    if (!d.ok()) return fail(EIN_ERROR_VALUE, std::string("disk tensor: cannot create dataset ") + name);
+    return EIN_SUCCESS;
+}
+
+int64_t count_of(int rank, int64_t const *dims) {
+    int64_t n = 1;
+    for (int i = 0; i < rank; ++i) n *= dims[i];
+    return n;
+}
+
+template <typename T>
+int disk_create(char const *path, char const *name, int rank, int64_t const *dims) {
+    EIN_TRY(check_names(path, name));
+    EIN_TRY(check_dims(rank, dims));
+    std::lock_guard<std::mutex> lock(g_h5_mutex);
+    std::string why;
+    H5 const *h = h5_lib(&why);
+    if (h == nullptr) return fail(EIN_ERROR_UNSUPPORTED, why);
+    Hid f, d;
+    EIN_TRY(open_file(*h, path, true, true, f));
+    return open_or_create<T>(*h, f.id, name, rank, dims, d);
+}
+
+template <typename T>
+int disk_write(char const *path, char const *name, int rank, int64_t const *dims, T const *data) {
+    EIN_TRY(check_names(path, name));
+    EIN_TRY(check_dims(rank, dims));
+    if (data == nullptr && count_of(rank, dims) > 0) return fail(EIN_ERROR_VALUE, "disk tensor: data is null");
+    std::lock_guard<std::mutex> lock(g_h5_mutex);
+    std::string why;
+    H5 const *h = h5_lib(&why);
+    if (h == nullptr) return fail(EIN_ERROR_UNSUPPORTED, why);
+    Hid f, d;
+    EIN_TRY(open_file(*h, path, true, true, f));
+    EIN_TRY(open_or_create<T>(*h, f.id, name, rank, dims, d));
+    if (count_of(rank, dims) == 0) return EIN_SUCCESS;
+    if (h->dwrite(d.id, native_type<T>(*h), kDefault, kDefault, kDefault, data) < 0)
+        return fail(EIN_ERROR_VALUE, std::string("disk tensor: write of ") + name + " failed");
+    return EIN_SUCCESS;
+}
+
+template <typename T>
+int disk_read(char const *path, char const *name, int rank, int64_t const *dims, T *data) {
+    EIN_TRY(check_names(path, name));
+    EIN_TRY(check_dims(rank, dims));
+    if (data == nullptr && count_of(rank, dims) > 0) return fail(EIN_ERROR_VALUE, "disk tensor: data is null");
+    std::lock_guard<std::mutex> lock(g_h5_mutex);
+    std::string why;
+    H5 const *h = h5_lib(&why);
+    if (h == nullptr) return fail(EIN_ERROR_UNSUPPORTED, why);
+    Hid f;
+    EIN_TRY(open_file(*h, path, false, false, f));
+    if (h->lexists(f.id, name, kDefault) <= 0) return fail(EIN_ERROR_VALUE, std::string("disk tensor: no dataset ") + name);
+    Hid d(h->dopen(f.id, name, kDefault), h->dclose);
+    if (!d.ok()) return fail(EIN_ERROR_VALUE, std::string("disk tensor: cannot open dataset ") + name);
+    EIN_TRY(check_dataset<T>(*h, d.id, rank, dims, nullptr, std::string("disk tensor ") + name));
+    if (count_of(rank, dims) == 0) return EIN_SUCCESS;
+    if (h->dread(d.id, native_type<T>(*h), kDefault, kDefault, kDefault, data) < 0)
+        return fail(EIN_ERROR_VALUE, std::string("disk tensor: read of ") + name + " failed");
+    return EIN_SUCCESS;
+}
+
+bool notrans(char t) { return t == 'n' || t == 'N'; }
+bool valid_trans(char t) { return t == 'n' || t == 'N' || t == 't' || t == 'T' || t == 'c' || t == 'C'; }
+
+// C = alpha*op(A)*op(B) + beta*C with A, B on the device and C the dataset `name` of `path`,
+// whose dims d[0..r) split as d[0..p) -> n (C's columns) and d[p..r) -> m (C's rows). Panels
+// are whole leading slices (n/d[0] columns each), so each is one hyperslab of whole chunks:
+// the panel's read into pinned host memory, upload, GEMM (the caller's stream), download and
+// write-back run in that order, and the host's disk I/O of panels p+1 (read) and p-1 (write)
+// overlaps the device work of panel p. beta == 0 skips the reads. Returns when C is on disk.
+template <typename T>
+int gemm_disk_c(char ta, char tb, int64_t m, int64_t n, int64_t k, T alpha, T const *A, int64_t lda, T const *B,
+                int64_t ldb, T beta, char const *path, char const *name, size_t panel_bytes, int nslices, int slice_type,
+                hipStream_t stream) {
+    if (!valid_trans(ta)) return fail(-1, "gemm_disk_c: transa must be N, T or C");
+    if (!valid_trans(tb)) return fail(-2, "gemm_disk_c: transb must be N, T or C");
+    if (m < 0) return fail(-3, "gemm_disk_c: m < 0");
+    if (n < 0) return fail(-4, "gemm_disk_c: n < 0");
+    if (k < 0) return fail(-5, "gemm_disk_c: k < 0");
+    if (lda < std::max<int64_t>(1, notrans(ta) ? m : k)) return fail(-8, "gemm_disk_c: lda too small");
+    if (ldb < std::max<int64_t>(1, notrans(tb) ? k : n)) return fail(-10, "gemm_disk_c: ldb too small");
+    EIN_TRY(check_names(path, name));
+    if (nslices > 0 && sizeof(T) != 8) return fail(EIN_ERROR_UNSUPPORTED, "gemm_disk_c: split precision is FP64 only");
+
+    std::lock_guard<std::mutex> lock(g_h5_mutex);
+    std::string why;
+    H5 const *h = h5_lib(&why);
+    if (h == nullptr) return fail(EIN_ERROR_UNSUPPORTED, why);
+    Hid f;
+    EIN_TRY(open_file(*h, path, true, false, f));
+    if (h->lexists(f.id, name, kDefault) <= 0) return fail(EIN_ERROR_VALUE, std::string("gemm_disk_c: no dataset ") + name);
+    Hid d(h->dopen(f.id, name, kDefault), h->dclose);
+    if (!d.ok()) return fail(EIN_ERROR_VALUE, std::string("gemm_disk_c: cannot open dataset ") + name);
+    std::vector<int64_t> dims;
+    EIN_TRY(check_dataset<T>(*h, d.id, -1, nullptr, &dims, "gemm_disk_c"));
+    int const rank = int(dims.size());
+    int p          = -1;
+    int64_t lead   = 1;
+    for (int i = 0; i < rank && p < 0; ++i) {
+        lead *= dims[size_t(i)];
+        if (lead == n && count_of(rank - i - 1, dims.data() + i + 1) == m) p = i + 1;
+    }
+    if (rank == 0 || p < 0)
+        return fail(EIN_ERROR_DIMENSION, "gemm_disk_c: the dataset's leading dims must multiply to n and the rest to m");
+    if (m == 0 || n == 0) return EIN_SUCCESS;
+    int64_t const slice = n / dims[0]; // columns per leading index
+    size_t const col_bytes = size_t(m) * sizeof(T);
+    if (panel_bytes == 0) panel_bytes = size_t(256) << 20;
+    int64_t const per = std::max<int64_t>(1, std::min<int64_t>(dims[0], int64_t(panel_bytes / (col_bytes * size_t(slice)))));
+    int64_t const npan     = (dims[0] + per - 1) / per;
+    size_t const pbytes    = col_bytes * size_t(slice) * size_t(per);
+    hid_t const ntype      = native_type<T>(*h);
+    Hid fspace(h->dget_space(d.id), h->sclose);
+    if (!fspace.ok()) return fail(EIN_ERROR_VALUE, "gemm_disk_c: cannot read the dataset's extents");
+
+    void *dev[2] = {nullptr, nullptr}, *host[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    int status         = EIN_SUCCESS;
+    // selects the panel's hyperslab in fspace and returns its memory space, of the
+    // hyperslab's own shape: with a flat memory space HDF5 gathers element by element
+    // (measured 0.16 against 4.6 GB/s for the Wabef panels at v = 96)
+    auto select = [&](int64_t a0, int64_t na, Hid &mspace) -> int {
+        std::vector<hsize_t> start(static_cast<size_t>(rank), 0), count(static_cast<size_t>(rank));
+        for (int i = 0; i < rank; ++i) count[size_t(i)] = hsize_t(dims[size_t(i)]);
+        start[0] = hsize_t(a0);
+        count[0] = hsize_t(na);
+        if (h->shyperslab(fspace.id, kSelectSet, start.data(), nullptr, count.data(), nullptr) < 0)
+            return fail(EIN_ERROR_VALUE, "gemm_disk_c: hyperslab selection failed");
+        mspace = Hid(h->screate(rank, count.data(), nullptr), h->sclose);
+        if (!mspace.ok()) return fail(EIN_ERROR_VALUE, "gemm_disk_c: memory space creation failed");
+        return EIN_SUCCESS;
+    };
+    auto write_back = [&](int64_t q) -> int {
+        int const b      = int(q & 1);
+        int64_t const a0 = q * per, na = std::min(per, dims[0] - a0);
+        EIN_HIP_CHECK(hipEventSynchronize(done[b]));
+        Hid ms;
+        EIN_TRY(select(a0, na, ms));
+        if (h->dwrite(d.id, ntype, ms.id, fspace.id, kDefault, host[b]) < 0)
+            return fail(EIN_ERROR_VALUE, std::string("gemm_disk_c: write of ") + name + " failed");
+        return EIN_SUCCESS;
+    };
+    auto run = [&]() -> int {
+        for (int b = 0; b < 2 && b < npan; ++b) {
+            EIN_TRY(ein_malloc(&dev[b], pbytes));
+            EIN_HIP_CHECK(hipHostMalloc(&host[b], pbytes, hipHostMallocDefault));
+            EIN_HIP_CHECK(hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
+        }
+        for (int64_t q = 0; q < npan; ++q) {
+            int const b       = int(q & 1);
+            int64_t const a0  = q * per, na = std::min(per, dims[0] - a0);
+            int64_t const j0  = a0 * slice, nc = na * slice;
+            size_t const bytes = col_bytes * size_t(nc);
+            T *Cd             = static_cast<T *>(dev[b]);
+            if (beta != T(0)) {
+                Hid ms;
+                EIN_TRY(select(a0, na, ms));
+                if (h->dread(d.id, ntype, ms.id, fspace.id, kDefault, host[b]) < 0)
+                    return fail(EIN_ERROR_VALUE, std::string("gemm_disk_c: read of ") + name + " failed");
+                EIN_HIP_CHECK(hipMemcpyAsync(Cd, host[b], bytes, hipMemcpyHostToDevice, stream));
+            }
+            T const *Bp = notrans(tb) ? B + j0 * ldb : B + j0;
+            if constexpr (sizeof(T) == 8) {
+                if (nslices > 0)
+                    EIN_TRY(ein_gemm_split_f64(ta, tb, m, nc, k, alpha, A, lda, Bp, ldb, beta, Cd, m, nslices, slice_type,
+                                               stream));
+                else
+                    EIN_TRY(ein_gemm_f64(ta, tb, m, nc, k, alpha, A, lda, Bp, ldb, beta, Cd, m, stream));
+            } else {
+                EIN_TRY(ein_gemm_f32(ta, tb, m, nc, k, alpha, A, lda, Bp, ldb, beta, Cd, m, stream));
+            }
+            EIN_HIP_CHECK(hipMemcpyAsync(host[b], Cd, bytes, hipMemcpyDeviceToHost, stream));
+            EIN_HIP_CHECK(hipEventRecord(done[b], stream));
+            if (q >= 1) EIN_TRY(write_back(q - 1)); // overlaps panel q on the device
+        }
+        return write_back(npan - 1);
+    };
+    status = run();
+    hipError_t const e = hipStreamSynchronize(stream); // nothing queued still reads the buffers
+    for (int b = 0; b < 2; ++b) {
+        if (done[b]) (void)hipEventDestroy(done[b]);
+        if (host[b]) (void)hipHostFree(host[b]);
+        if (dev[b]) ein_free(dev[b]);
+    }
+    if (status != EIN_SUCCESS) return status;
+    if (e != hipSuccess) return fail(EIN_ERROR_HIP, std::string("gemm_disk_c: ") + hipGetErrorString(e));
+    return EIN_SUCCESS;
+}
+
+} // namespace
+} // namespace ein
+
+extern "C" {
+
+int ein_disk_create_f64(const char *path, const char *dataset, int32_t rank, const int64_t *dims) {
+    return ein::disk_create<double>(path, dataset, rank, dims);
+}
+int ein_disk_create_f32(const char *path, const char *dataset, int32_t rank, const int64_t *dims) {
+    return ein::disk_create<float>(path, dataset, rank, dims);
+}
+int ein_disk_write_f64(const char *path, const char *dataset, int32_t rank, const int64_t *dims, const double *data) {
+    return ein::disk_write<double>(path, dataset, rank, dims, data);
+}
+int ein_disk_write_f32(const char *path, const char *dataset, int32_t rank, const int64_t *dims, const float *data) {
+    return ein::disk_write<float>(path, dataset, rank, dims, data);
+}
+int ein_disk_read_f64(const char *path, const char *dataset, int32_t rank, const int64_t *dims, double *data) {
+    return ein::disk_read<double>(path, dataset, rank, dims, data);
+}
+int ein_disk_read_f32(const char *path, const char *dataset, int32_t rank, const int64_t *dims, float *data) {
+    return ein::disk_read<float>(path, dataset, rank, dims, data);
+}
+int ein_gemm_disk_c_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+                        int64_t lda, const double *B, int64_t ldb, double beta, const char *path, const char *dataset,
+                        size_t panel_bytes, int32_t nslices, int32_t slice_type, ein_stream_t stream) {
+    return ein::gemm_disk_c<double>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, path, dataset, panel_bytes,
+                                    nslices, slice_type, ein::as_stream(stream));
+}
+int ein_gemm_disk_c_f32(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                        int64_t lda, const float *B, int64_t ldb, float beta, const char *path, const char *dataset,
+                        size_t panel_bytes, ein_stream_t stream) {
+    return ein::gemm_disk_c<float>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, path, dataset, panel_bytes, 0,
+                                   0, ein::as_stream(stream));
+}
+
+} // extern "C"

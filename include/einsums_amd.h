@@ -212,6 +212,33 @@ int ein_gemm_host_c_f64(char transa, char transb, int64_t m, int64_t n, int64_t 
 int ein_gemm_host_c_f32(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
                         int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc, size_t panel_bytes,
                         ein_stream_t stream);
+/* Tensors on disk: one chunked HDF5 dataset per tensor, its dims in row-major order, as the
+ * reference's DiskTensor keeps them (libs/Einsums/Tensor/include/Einsums/Tensor/
+ * DiskTensor.hpp:200-306: created on construction when absent, opened otherwise; get()/put()
+ * move the data). libhdf5 is loaded at first use ($EINSUMS_AMD_HDF5, libhdf5.so, the conda
+ * path); without it these return EIN_ERROR_UNSUPPORTED. ein_disk_create_* creates the file
+ * and the dataset when absent (chunks of one leading slice, cut to about 16 MiB) and checks
+ * rank, extents and element type when present; ein_disk_write_* does the same, then writes
+ * all of data; ein_disk_read_* reads an existing dataset. All HDF5 calls are serialised. */
+int ein_disk_create_f64(const char *path, const char *dataset, int32_t rank, const int64_t *dims);
+int ein_disk_create_f32(const char *path, const char *dataset, int32_t rank, const int64_t *dims);
+int ein_disk_write_f64(const char *path, const char *dataset, int32_t rank, const int64_t *dims, const double *data);
+int ein_disk_write_f32(const char *path, const char *dataset, int32_t rank, const int64_t *dims, const float *data);
+int ein_disk_read_f64(const char *path, const char *dataset, int32_t rank, const int64_t *dims, double *data);
+int ein_disk_read_f32(const char *path, const char *dataset, int32_t rank, const int64_t *dims, float *data);
+/* ein_gemm_host_c_* with C in a disk tensor: C (column-major m x n) is the row-major view
+ * [n][m] of dataset `dataset` in `path`, whose leading dims multiply to n and the remaining
+ * ones to m (the Wabef intermediate as a (v, v, v, v) DiskTensor: n = m = v^2). Panels of
+ * whole leading slices (about panel_bytes, 0 = 256 MiB) go disk -> pinned host -> device ->
+ * GEMM -> host -> disk; the disk reads and writes of neighbouring panels overlap the device
+ * work. beta == 0 skips the reads. nslices > 0 runs each panel's GEMM in split precision
+ * (slice_type EIN_SLICE_*). Returns when C is on disk; calls are serialised. */
+int ein_gemm_disk_c_f64(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+                        int64_t lda, const double *B, int64_t ldb, double beta, const char *path, const char *dataset,
+                        size_t panel_bytes, int32_t nslices, int32_t slice_type, ein_stream_t stream);
+int ein_gemm_disk_c_f32(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                        int64_t lda, const float *B, int64_t ldb, float beta, const char *path, const char *dataset,
+                        size_t panel_bytes, ein_stream_t stream);
 /* ein_einsum_f64 with the GEMM-mapped case run by the split-precision GEMM; other
  * patterns run exactly as in ein_einsum_f64. */
 int ein_einsum_split_f64(int32_t nslices, int32_t slice_type, double c_prefactor, const ein_tensor_desc *C, double *c,
