@@ -2,6 +2,7 @@
 #pragma once
 
 #include "DeviceTensor.hpp"
+#include "DiskTensor.hpp"
 #include "Index.hpp"
 #include "Tensor.hpp"
 #include "TensorAlgebra.hpp"

@@ -9,11 +9,11 @@ import pytest
 BIN = Path(__file__).resolve().parent / "cpp" / "bin"
 
 
-def _run(name, timeout):
+def _run(name, timeout, *args):
     exe = BIN / name
     if not exe.exists():
         subprocess.run(["make", "-s", "-C", str(exe.parent.parent)], check=True)
-    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=timeout)
+    res = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=timeout)
     print(res.stdout)
     print(res.stderr)
     return res
@@ -28,6 +28,20 @@ def test_streamk_schedule_program():
     """Host check of the stream-K unit ownership, piece slots and fix-up coverage."""
     res = _run("test_streamk_schedule", 120)
     assert res.returncode == 0 and "0 failures" in res.stdout
+
+
+def test_disk_tensor_program_cpu():
+    """DiskTensor create / put / get round trip and refusals (HDF5, no GPU)."""
+    res = _run("test_disk_tensor", 60, "cpu")
+    assert res.returncode == 0 and "PASS test_disk_tensor cpu" in res.stdout, res.stdout + res.stderr
+
+
+@pytest.mark.gpu
+def test_disk_tensor_program_gpu():
+    """linear_algebra::gemm into a (v, v, v, v) DiskTensor in 5 panels, against a
+    long-double triple loop."""
+    res = _run("test_disk_tensor", 120, "gpu")
+    assert res.returncode == 0 and "PASS test_disk_tensor gpu" in res.stdout, res.stdout + res.stderr
 
 
 @pytest.mark.gpu
