@@ -532,66 +532,73 @@ int gemm_split_crt(GemmProblem<double> P, int nmod, hipStream_t stream) {
 
     CrtSide side;
     if (fork) EIN_TRY(crt_side(side));
-    int last = 0;
-    for (int64_t j0 = 0, panel = 0; j0 < P.n; j0 += pn, ++panel) {
-        int64_t const nj = std::min(pn, P.n - j0);
-        int const b      = fork ? int(panel & 1) : 0;
-        uint8_t *Rb      = R + size_t(b) * up(r_bytes);
-        last             = b;
-        if (fork && panel >= 2) EIN_HIP_CHECK(hipStreamWaitEvent(stream, side.recon_done[b], 0)); // buffer b free
-        CrtGemm G{};
-        G.m       = P.m;
-        G.n       = nj;
-        G.kp      = kp;
-        G.a_plane = P.m * kp;
-        G.b_plane = P.n * kp;
-        G.A       = As;
-        G.B       = Bs + j0 * kp;
-        G.R       = Rb;
-        G.ldr     = ldr;
-        G.r_plane = nj * ldr;
-        G.tiles_m = tiles_m;
-        G.tiles_n = uint32_t(ceil_div(nj, CG_TILE));
-        G.K       = K;
-        uint64_t const grid = uint64_t(G.tiles_m) * G.tiles_n * uint64_t(nmod);
-        if (grid > 0x7fffffffu) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: tile grid too large");
-        for (int64_t k0 = 0; k0 < kp; k0 += int64_t(CRT_FOLD_STAGES) * CG_KS) {
-            G.k0         = k0;
-            G.kc         = std::min<int64_t>(int64_t(CRT_FOLD_STAGES) * CG_KS, kp - k0);
-            G.accumulate = k0 > 0;
-            crt_gemm_kernel<<<uint32_t(grid), 64 * CG_WAVES, 0, stream>>>(G);
+    int last    = -1; // plane buffer of the last reconstruction queued on the side stream
+    auto panels = [&]() -> int {
+        for (int64_t j0 = 0, panel = 0; j0 < P.n; j0 += pn, ++panel) {
+            int64_t const nj = std::min(pn, P.n - j0);
+            int const b      = fork ? int(panel & 1) : 0;
+            uint8_t *Rb      = R + size_t(b) * up(r_bytes);
+            if (fork && panel >= 2) EIN_HIP_CHECK(hipStreamWaitEvent(stream, side.recon_done[b], 0)); // buffer b free
+            CrtGemm G{};
+            G.m       = P.m;
+            G.n       = nj;
+            G.kp      = kp;
+            G.a_plane = P.m * kp;
+            G.b_plane = P.n * kp;
+            G.A       = As;
+            G.B       = Bs + j0 * kp;
+            G.R       = Rb;
+            G.ldr     = ldr;
+            G.r_plane = nj * ldr;
+            G.tiles_m = tiles_m;
+            G.tiles_n = uint32_t(ceil_div(nj, CG_TILE));
+            G.K       = K;
+            uint64_t const grid = uint64_t(G.tiles_m) * G.tiles_n * uint64_t(nmod);
+            if (grid > 0x7fffffffu) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: tile grid too large");
+            for (int64_t k0 = 0; k0 < kp; k0 += int64_t(CRT_FOLD_STAGES) * CG_KS) {
+                G.k0         = k0;
+                G.kc         = std::min<int64_t>(int64_t(CRT_FOLD_STAGES) * CG_KS, kp - k0);
+                G.accumulate = k0 > 0;
+                crt_gemm_kernel<<<uint32_t(grid), 64 * CG_WAVES, 0, stream>>>(G);
+                EIN_LAUNCH_CHECK();
+            }
+
+            hipStream_t rs = stream;
+            if (fork) {
+                EIN_HIP_CHECK(hipEventRecord(side.gemm_done[b], stream));
+                EIN_HIP_CHECK(hipStreamWaitEvent(side.aux, side.gemm_done[b], 0));
+                rs = side.aux;
+            }
+            CrtRec Q{};
+            Q.m       = P.m;
+            Q.n       = nj;
+            Q.R       = Rb;
+            Q.ldr     = ldr;
+            Q.r_plane = nj * ldr;
+            Q.ea      = ea;
+            Q.eb      = eb + j0;
+            Q.C       = P.C + j0 * P.c_sn;
+            Q.c_sm    = P.c_sm;
+            Q.c_sn    = P.c_sn;
+            Q.alpha   = P.alpha;
+            Q.beta    = P.beta;
+            Q.W       = W;
+            int64_t const threads = nj * (ldr / 4);
+            if (ceil_div(threads, 256) > 0x7fffffff) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: panel too large");
+            crt_reconstruct_kernel<<<uint32_t(ceil_div(threads, 256)), 256, 0, rs>>>(Q);
+            if (fork) {
+                last = b;
+                EIN_HIP_CHECK(hipEventRecord(side.recon_done[b], side.aux));
+            }
             EIN_LAUNCH_CHECK();
         }
-
-        hipStream_t rs = stream;
-        if (fork) {
-            EIN_HIP_CHECK(hipEventRecord(side.gemm_done[b], stream));
-            EIN_HIP_CHECK(hipStreamWaitEvent(side.aux, side.gemm_done[b], 0));
-            rs = side.aux;
-        }
-        CrtRec Q{};
-        Q.m       = P.m;
-        Q.n       = nj;
-        Q.R       = Rb;
-        Q.ldr     = ldr;
-        Q.r_plane = nj * ldr;
-        Q.ea      = ea;
-        Q.eb      = eb + j0;
-        Q.C       = P.C + j0 * P.c_sn;
-        Q.c_sm    = P.c_sm;
-        Q.c_sn    = P.c_sn;
-        Q.alpha   = P.alpha;
-        Q.beta    = P.beta;
-        Q.W       = W;
-        int64_t const threads = nj * (ldr / 4);
-        if (ceil_div(threads, 256) > 0x7fffffff) return fail(EIN_ERROR_UNSUPPORTED, "split gemm: panel too large");
-        crt_reconstruct_kernel<<<uint32_t(ceil_div(threads, 256)), 256, 0, rs>>>(Q);
-        EIN_LAUNCH_CHECK();
-        if (fork) EIN_HIP_CHECK(hipEventRecord(side.recon_done[b], side.aux));
-    }
-    // join: the side stream runs the reconstructions in order, so the last one covers all
-    if (fork) EIN_HIP_CHECK(hipStreamWaitEvent(stream, side.recon_done[last], 0));
-    return EIN_SUCCESS;
+        return EIN_SUCCESS;
+    };
+    int const status = panels();
+    // join, also after a failure, so no queued reconstruction outlives the call's workspace:
+    // the side stream runs the reconstructions in order, so the last one covers all
+    if (last >= 0) EIN_HIP_CHECK(hipStreamWaitEvent(stream, side.recon_done[last], 0));
+    return status;
 }
 
 int crt_bits(int nmod, int64_t k) {
