@@ -16,7 +16,8 @@
 // takes the rows c = w, w+W, ... (W = 4 or 16 waves); its lanes run over d. Per lane it keeps
 // K partials per row; J(a,b) is complete inside the workgroup and is written directly, K is
 // written as one partial row per chunk. Kernel 2 adds the chunks' K partials in chunk order
-// and forms F (deterministic: fixed summation trees, no atomics). n <= 128.
+// and forms F (deterministic: fixed summation trees, no atomics). n <= 128. For even n <= 64
+// with 16-byte aligned rows, kernel 1 reads two elements per lane (fock_jk16_kernel).
 #include <algorithm>
 #include <string>
 
@@ -108,6 +109,81 @@ __global__ __launch_bounds__(64 * WAVES) void fock_jk_kernel(int n, double const
     }
 }
 
+// The same pass with 16-byte loads, for even n <= 64 with 16-byte aligned rows of g and D
+// (even ldd): lane l reads d = 2(l%32), 2(l%32)+1 of row c = 2p + l/32 (lanes with
+// 2(l%32) >= n idle), so one load covers two adjacent rows of a slab; wave w takes the row
+// pairs p = w, w+4, ... (MP >= ceil(n/8) of them). K's row sums reduce within each 32-lane
+// half. Measured at n = 48 (HF): 18.0 against 21.1 us cold for the two launches.
+template <int MP, int BC>
+__global__ __launch_bounds__(256) void fock_jk16_kernel(int n, double const *__restrict__ g, double const *__restrict__ D,
+                                                        int64_t ldd, double *__restrict__ J, double *__restrict__ Kp) {
+    constexpr int WAVES = 4;
+    __shared__ double red[WAVES][BC];
+    int const a      = int(blockIdx.x);
+    int const ch     = int(blockIdx.y);
+    int const wave   = int(threadIdx.x) >> 6;
+    int const lane   = int(threadIdx.x) & 63;
+    int const h      = lane >> 5;
+    int const d0     = 2 * (lane & 31);
+    bool const d_in  = d0 < n;
+    int const b0     = ch * BC;
+    int const np     = (n / 2 - wave + WAVES - 1) / WAVES; // row pairs p = wave + WAVES*i, i < np
+    int64_t const nn = int64_t(n) * n;
+
+    double2 x[BC][MP];
+#pragma unroll
+    for (int q = 0; q < BC; ++q) {
+        bool const b_in = b0 + q < n;
+        double const *__restrict__ slab = g + (int64_t(a) * n + (b_in ? b0 + q : b0)) * nn;
+#pragma unroll
+        for (int i = 0; i < MP; ++i) {
+            int const c = 2 * (wave + WAVES * i) + h;
+            x[q][i]     = (b_in && i < np && d_in) ? *reinterpret_cast<double2 const *>(slab + int64_t(c) * n + d0)
+                                                   : double2{0.0, 0.0};
+        }
+    }
+    double2 dc[MP], db[BC];
+#pragma unroll
+    for (int i = 0; i < MP; ++i) {
+        int const c = 2 * (wave + WAVES * i) + h;
+        dc[i] = (i < np && d_in) ? *reinterpret_cast<double2 const *>(D + int64_t(c) * ldd + d0) : double2{0.0, 0.0};
+    }
+#pragma unroll
+    for (int q = 0; q < BC; ++q)
+        db[q] = (b0 + q < n && d_in) ? *reinterpret_cast<double2 const *>(D + int64_t(b0 + q) * ldd + d0)
+                                     : double2{0.0, 0.0};
+
+    double kacc[MP];
+#pragma unroll
+    for (int i = 0; i < MP; ++i) kacc[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < BC; ++q) {
+        double jsum = 0.0;
+#pragma unroll
+        for (int i = 0; i < MP; ++i) {
+            jsum += x[q][i].x * dc[i].x + x[q][i].y * dc[i].y;
+            kacc[i] += x[q][i].x * db[q].x + x[q][i].y * db[q].y;
+        }
+        jsum = wave_reduce_sum(jsum);
+        if (lane == 0) red[wave][q] = jsum;
+    }
+    double *__restrict__ kp = Kp + (int64_t(a) * gridDim.y + ch) * n;
+#pragma unroll
+    for (int i = 0; i < MP; ++i) {
+        double s = kacc[i];
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o, 64); // within the 32-lane half
+        if ((lane & 31) == 0 && i < np) kp[2 * (wave + WAVES * i) + h] = s;
+    }
+    __syncthreads();
+    if (int(threadIdx.x) < BC && b0 + int(threadIdx.x) < n) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) s += red[w][threadIdx.x];
+        J[int64_t(a) * n + b0 + int(threadIdx.x)] = s;
+    }
+}
+
 // F(a,c) = beta*F(a,c) + alpha_j*J(a,c) + alpha_k*sum_chunk Kp(a,chunk,c), chunks in order.
 __global__ __launch_bounds__(256) void fock_finish_kernel(int n, int chunks, double const *__restrict__ J,
                                                           double const *__restrict__ Kp, double alpha_j, double alpha_k,
@@ -152,7 +228,17 @@ int fock_jk_f64(int64_t n, double const *g, double const *D, int64_t ldd, double
     double *Kp = J + size_t(n) * size_t(n);
     dim3 grid{uint32_t(n), uint32_t(chunks)};
     int const ni = int(n);
-    if (n <= 32)
+    // two elements per lane for even n <= 64 (a one-row-per-load form for 64 < n <= 128
+    // measured slower than the kernel below it: 215 against 201 us cold at n = 96)
+    bool const wide = n % 2 == 0 && n <= 64 && ldd % 2 == 0 && reinterpret_cast<uintptr_t>(g) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(D) % 16 == 0;
+    if (wide && n <= 32)
+        fock_jk16_kernel<4, 4><<<grid, 256, 0, stream>>>(ni, g, D, ldd, J, Kp);
+    else if (wide && n <= 48)
+        fock_jk16_kernel<6, 4><<<grid, 256, 0, stream>>>(ni, g, D, ldd, J, Kp);
+    else if (wide)
+        fock_jk16_kernel<8, 4><<<grid, 256, 0, stream>>>(ni, g, D, ldd, J, Kp);
+    else if (n <= 32)
         fock_jk_kernel<1, 4, 8, 4><<<grid, 256, 0, stream>>>(ni, g, D, ldd, J, Kp);
     else if (n <= 48)
         fock_jk_kernel<1, 4, 12, 4><<<grid, 256, 0, stream>>>(ni, g, D, ldd, J, Kp);

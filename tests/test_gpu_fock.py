@@ -3,7 +3,9 @@ oracle's two einsum calls of the reference's Fock build (QChemTests.cpp:299-304)
   F = beta*F + alpha_j * einsum(ij <- ijkl g, kl D) + alpha_k * einsum(ij <- ikjl g, kl D).
 Cases: the HF workload at nbf = 48 (8-fold symmetric g, idempotent D), unsymmetric random g
 and D at extents that exercise every lane/row split of the kernel (n = 1, 7, 17, 63, 64, 65,
-96, 128), a padded leading dimension, beta != 0, and bit-for-bit repeatability.
+96, 128), a padded leading dimension, beta != 0, bit-for-bit repeatability, and the
+two-element-load kernel (even n <= 64, even leading dimension, aligned D) beside the
+one-element ones it falls back to (odd n, n > 64, odd leading dimension, unaligned D).
 Tolerance: FP64 relative Frobenius 1e-13."""
 import numpy as np
 import pytest
@@ -70,6 +72,22 @@ def test_fock_padded_and_repeatable(core, oracle):
     assert torch.equal(first, Fbig)  # same bits
     assert torch.all(Fbig[:, n:] == 0)  # padding untouched
     ref = oracle_fock(oracle, g, host(D).copy(), 2.0, -1.0, 0.0, np.zeros((n, n)))
+    assert rel_fro(host(F), ref) < 1e-13
+
+
+@pytest.mark.parametrize("n,pad,shift", [(2, 0, 0), (32, 8, 0), (40, 8, 0), (48, 2, 1), (64, 0, 0), (66, 2, 0),
+                                         (96, 4, 0), (96, 0, 1), (128, 2, 0)])
+def test_fock_two_element_loads(core, oracle, n, pad, shift):
+    # even n <= 64 with an even leading dimension and 16-byte aligned D reads two elements
+    # per lane; shift = 1 moves D by one element and n > 64 runs the one-element kernel
+    rng = np.random.default_rng(70 + n)
+    g = rng.uniform(-1, 1, (n, n, n, n))
+    Dbuf = dev(rng.uniform(-1, 1, (n * (n + pad) + shift,)))
+    D = Dbuf[shift:].view(n, n + pad)[:, :n]
+    F0 = rng.uniform(-1, 1, (n, n))
+    F = dev(F0)
+    core.fock_jk(dev(g), D, F, alpha_j=1.5, alpha_k=-0.5, beta=-1.0)
+    ref = oracle_fock(oracle, g, host(D).copy(), 1.5, -0.5, -1.0, F0)
     assert rel_fro(host(F), ref) < 1e-13
 
 
