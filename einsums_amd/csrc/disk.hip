@@ -38,7 +38,8 @@ constexpr unsigned kAccRdwr    = 0x0001u; // H5F_ACC_RDWR
 constexpr unsigned kAccExcl    = 0x0004u; // H5F_ACC_EXCL
 constexpr int kSelectSet       = 0;       // H5S_SELECT_SET
 constexpr int kClassFloat      = 1;       // H5T_FLOAT
-constexpr size_t kChunkBytes   = size_t(16) << 20;
+constexpr size_t kChunkBytes    = size_t(16) << 20;
+constexpr size_t kMinChunkBytes = size_t(1) << 20;
 
 struct H5 {
     herr_t (*open)();
@@ -224,7 +225,8 @@ int open_file(H5 const &h, char const *path, bool write, bool create, Hid &f) {
 
 // Opens dataset `name`, creating it (chunked, zero-filled on first read) when absent. The
 // chunk is one leading slice, cut down to about 16 MiB by taking single indices of the next
-// dims, so the GEMM's panels (whole leading slices) read and write whole chunks.
+// dims, so the GEMM's panels (whole leading slices) read and write whole chunks; slices under
+// 1 MiB are grouped, as many leading indices as make about 1 MiB, into one chunk.
 template <typename T>
 int open_or_create(H5 const &h, hid_t file, char const *name, int rank, int64_t const *dims, Hid &d) {
     htri_t const ex = h.lexists(file, name, kDefault);
@@ -247,6 +249,8 @@ int open_or_create(H5 const &h, hid_t file, char const *name, int rank, int64_t 
         bytes /= size_t(chunk[size_t(i)]);
         chunk[size_t(i)] = 1;
     }
+    if (bytes < kMinChunkBytes) // small slices (rank 1 included): several leading indices a chunk
+        chunk[0] = std::max<hsize_t>(1, std::min<hsize_t>(dd[0], hsize_t(kMinChunkBytes / bytes)));
     Hid space(h.screate(rank, dd.data(), nullptr), h.sclose);
     Hid dcpl(h.pcreate(*h.dcpl_class), h.pclose);
     if (!space.ok() || !dcpl.ok() || (!empty && h.pchunk(dcpl.id, rank, chunk.data()) < 0))

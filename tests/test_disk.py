@@ -32,6 +32,20 @@ def test_large_leading_slices_are_chunked(tmp_path):
     assert np.array_equal(t.get(), a)
 
 
+def test_small_slices_and_rank_one(tmp_path):
+    # slices of a few bytes are grouped into larger chunks; the data round-trips either way
+    p = tmp_path / "s.h5"
+    a = np.random.default_rng(3).standard_normal((200_000, 2))
+    t = disk.DiskTensor(p, "pairs", a.shape)
+    t.put(a)
+    assert np.array_equal(t.get(), a)
+    v = np.arange(300_001, dtype=np.float64)
+    r = disk.DiskTensor(p, "vec", v.shape)
+    r.put(v)
+    assert np.array_equal(r.get(), v)
+    assert p.stat().st_size < 2 * (a.nbytes + v.nbytes)  # no per-element chunk overhead
+
+
 def test_refusals(tmp_path):
     p = tmp_path / "r.h5"
     disk.DiskTensor(p, "x", (4, 6))
