@@ -92,32 +92,37 @@ __global__ __launch_bounds__(256) void permute_elem_kernel(PermAxes P, T alpha, 
     }
 }
 
-// 64x64 LDS tile transpose. X = C's unit-stride axis (A stride ax), Y = A's unit-stride
-// axis (C stride cy). Remaining axes (P, n entries) are decoded from the batch index. Every
-// load of a tile is issued before its LDS writes (a load and LDS write under one bounds test
-// per element serialised the loads: 4.3 against 5.2 TB/s on a 2.4 GB transpose). Full
-// tiles of operands whose strides and base addresses allow it (vec) move 16-byte vectors.
-template <typename T, bool WIDE>
+// TS x TS LDS tile transpose (TS = 64, or 32 where the extents are multiples of 32 but not
+// of 64, so no tile is partial: the Wmbej operand pack at v = 96 has a 96-long axis). X = C's
+// unit-stride axis (A stride ax), Y = A's unit-stride axis (C stride cy). Remaining axes (P,
+// n entries) are decoded from the batch index. Every load of a tile is issued before its LDS
+// writes (a load and LDS write under one bounds test per element serialised the loads: 4.3
+// against 5.2 TB/s on a 2.4 GB transpose). Full tiles of operands whose strides and base
+// addresses allow it (vec) move 16-byte vectors.
+template <typename T, bool WIDE, int TS>
 __global__ __launch_bounds__(256) void permute_tile_kernel(PermAxes P, int64_t ex, int64_t ax, int64_t ey, int64_t cy,
                                                            int64_t tiles_x, T alpha, T const *__restrict__ A, T beta,
                                                            T *__restrict__ C, int copy, int vec) {
     constexpr int VW  = 16 / int(sizeof(T)); // elements per 16-byte vector
-    constexpr int LPR = 64 / VW;             // lanes per tile row
+    constexpr int LPR = TS / VW;             // lanes per tile row
     constexpr int RPP = 256 / LPR;           // tile rows per pass of the workgroup
-    constexpr int NP  = 64 / RPP;            // passes per tile
+    constexpr int NP  = TS / RPP;            // passes per tile
+    constexpr int TPR = 256 / TS;            // thread rows of the element-wise path
+    constexpr int NR  = TS / TPR;            // elements per thread of the element-wise path
+    static_assert(NP >= 1 && NR >= 1, "tile too small for 256 threads");
     struct alignas(16) Vec {
         T v[VW];
     };
-    __shared__ T tile[64][65];
-    int64_t const ntiles = tiles_x * ((ey + 63) / 64);
+    __shared__ T tile[TS][TS + 1];
+    int64_t const ntiles = tiles_x * ((ey + TS - 1) / TS);
     int const t          = threadIdx.x;
     for (int64_t b = blockIdx.y; b < P.total; b += gridDim.y) {
         int64_t oa, oc;
         perm_decode<WIDE>(b, P, P.n, oa, oc);
         for (int64_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
-            int64_t const x0 = (tt % tiles_x) * 64;
-            int64_t const y0 = (tt / tiles_x) * 64;
-            bool const full  = vec != 0 && x0 + 64 <= ex && y0 + 64 <= ey;
+            int64_t const x0 = (tt % tiles_x) * TS;
+            int64_t const y0 = (tt / tiles_x) * TS;
+            bool const full  = vec != 0 && x0 + TS <= ex && y0 + TS <= ey;
             if (full) {
                 int const c  = t % LPR;
                 int const r0 = t / LPR;
@@ -130,16 +135,16 @@ __global__ __launch_bounds__(256) void permute_tile_kernel(PermAxes P, int64_t e
 #pragma unroll
                     for (int i = 0; i < VW; ++i) tile[r0 + p * RPP][VW * c + i] = v[p].v[i];
             } else {
-                int const ly = t & 63;
-                int const lx = t >> 6;
-                T v[16];
+                int const ly = t % TS;
+                int const lx = t / TS;
+                T v[NR];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    int const x = lx + 4 * r;
+                for (int r = 0; r < NR; ++r) {
+                    int const x = lx + TPR * r;
                     v[r]        = (x0 + x < ex && y0 + ly < ey) ? A[oa + (x0 + x) * ax + (y0 + ly)] : T(0);
                 }
 #pragma unroll
-                for (int r = 0; r < 16; ++r) tile[lx + 4 * r][ly] = v[r];
+                for (int r = 0; r < NR; ++r) tile[lx + TPR * r][ly] = v[r];
             }
             __syncthreads();
             if (full) {
@@ -160,11 +165,11 @@ __global__ __launch_bounds__(256) void permute_tile_kernel(PermAxes P, int64_t e
                     *cp = w;
                 }
             } else {
-                int const lx = t & 63;
-                int const ly = t >> 6;
+                int const lx = t % TS;
+                int const ly = t / TS;
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    int y = ly + 4 * r;
+                for (int r = 0; r < NR; ++r) {
+                    int y = ly + TPR * r;
                     if (x0 + lx < ex && y0 + y < ey) {
                         T *cp = C + oc + (y0 + y) * cy + (x0 + lx);
                         *cp   = perm_value<T>(tile[lx][y], cp, alpha, beta, copy != 0);
@@ -246,7 +251,10 @@ int permute_impl(int32_t rank, int64_t const *dims_A, int32_t const *perm, T alp
         }
         fill_divs(P);
         int64_t ex = m[ic].e, axs = m[ic].sa, ey = m[ia].e, cy = m[ia].sc;
-        int64_t tiles_x = ceil_div(ex, 64), tiles = tiles_x * ceil_div(ey, 64);
+        // 32-wide tiles where they leave no partial tile and 64-wide ones would
+        bool const t32  = ex % 32 == 0 && ey % 32 == 0 && (ex % 64 != 0 || ey % 64 != 0);
+        int64_t const ts = t32 ? 32 : 64;
+        int64_t tiles_x = ceil_div(ex, ts), tiles = tiles_x * ceil_div(ey, ts);
         int64_t gy      = std::min<int64_t>(P.total, 65535);
         // one tile per workgroup (tiles of one batch entry used to share a workgroup and ran
         // one after another)
@@ -258,12 +266,15 @@ int permute_impl(int32_t rank, int64_t const *dims_A, int32_t const *perm, T alp
         constexpr int VW = 16 / int(sizeof(T));
         bool vec = axs % VW == 0 && cy % VW == 0 && (reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(C)) % 16 == 0;
         for (int d = 0; d < P.n; ++d) vec = vec && P.sa[d] % VW == 0 && P.sc[d] % VW == 0;
-        if (wide)
-            permute_tile_kernel<T, true>
-                <<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy, vec ? 1 : 0);
+        int const vf = vec ? 1 : 0;
+        if (wide && t32)
+            permute_tile_kernel<T, true, 32><<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy, vf);
+        else if (wide)
+            permute_tile_kernel<T, true, 64><<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy, vf);
+        else if (t32)
+            permute_tile_kernel<T, false, 32><<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy, vf);
         else
-            permute_tile_kernel<T, false>
-                <<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy, vec ? 1 : 0);
+            permute_tile_kernel<T, false, 64><<<grid, 256, 0, stream>>>(P, ex, axs, ey, cy, tiles_x, alpha, A, beta, C, copy, vf);
         EIN_LAUNCH_CHECK();
         return EIN_SUCCESS;
     }

@@ -1,6 +1,7 @@
 """Permute on the GPU: bit-exact against numpy's transpose (alpha=1, beta=0) and the
-CPU oracle, on every kernel route (row copy with 16-byte vectors, LDS tile transpose,
-element-wise strided), plus alpha/beta blending and inverse round trips."""
+CPU oracle, on every kernel route (row copy with 16-byte vectors, LDS tile transpose with
+64- and 32-wide tiles, element-wise strided), plus alpha/beta blending and inverse round
+trips."""
 import itertools
 import zlib
 
@@ -28,6 +29,8 @@ CASES = [
     ((3, 130, 66), "ikj"),        # tile transpose with partial edge tiles on both axes
     ((2, 128, 192), "ikj"),       # full tiles only, 16-byte vectors for both widths
     ((2, 136, 200), "ikj"),       # full (vector) and edge (element) tiles in one launch
+    ((24, 24, 96, 96), "iljk"),   # Wmbej pack jnfb -> jbnf: a 96-long axis takes 32-wide tiles
+    ((2, 96, 160), "ikj"),        # 32-wide tiles on both axes (neither a multiple of 64)
     ((5, 7, 9), "kji"),
     ((64, 65), "ji"),
     ((1, 130, 1, 67), "lkji"),
@@ -71,11 +74,13 @@ def test_permute_all_rank4_permutations_roundtrip(core):
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-15), (np.float32, 1e-6)])
-def test_permute_tile_alpha_beta(core, oracle, dtype, tol):
-    """C = beta*C + alpha*perm(A) through the tile transpose, vector and edge tiles."""
-    rng = seeded("tile-ab", dtype.__name__)
-    A = rng.standard_normal((2, 136, 200)).astype(dtype)
-    C0 = rng.standard_normal((2, 200, 136)).astype(dtype)
+@pytest.mark.parametrize("shape", [(2, 136, 200), (3, 96, 160)])
+def test_permute_tile_alpha_beta(core, oracle, dtype, tol, shape):
+    """C = beta*C + alpha*perm(A) through the tile transpose: 64-wide vector and edge tiles,
+    and 32-wide tiles."""
+    rng = seeded("tile-ab", dtype.__name__, shape)
+    A = rng.standard_normal(shape).astype(dtype)
+    C0 = rng.standard_normal((shape[0], shape[2], shape[1])).astype(dtype)
     dC = dev(C0)
     core.permute(0.5, "ikj", dC, -2.0, "ijk", dev(A))
     ref = C0.astype(np.float64)
