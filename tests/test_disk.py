@@ -46,6 +46,16 @@ def test_small_slices_and_rank_one(tmp_path):
     assert p.stat().st_size < 2 * (a.nbytes + v.nbytes)  # no per-element chunk overhead
 
 
+def test_empty_and_single_element(tmp_path):
+    p = tmp_path / "e.h5"
+    t = disk.DiskTensor(p, "e", (0, 5))
+    t.put(np.zeros((0, 5)))
+    assert disk.DiskTensor(p, "e", (0, 5)).get().shape == (0, 5)
+    one = disk.DiskTensor(p, "one", (1,))
+    one.put(np.array([3.5]))
+    assert one.get().tolist() == [3.5]
+
+
 def test_refusals(tmp_path):
     p = tmp_path / "r.h5"
     disk.DiskTensor(p, "x", (4, 6))

@@ -71,6 +71,21 @@ def test_gemm_disk_split_and_fp32(disk, tmp_path):
     assert rel_fro(F.get(), ref32) < 1e-5
 
 
+def test_gemm_disk_k0_and_empty(disk, tmp_path):
+    # k = 0: C = beta * C through the panels; m = 0: nothing to do
+    C0 = np.random.default_rng(8).uniform(-1, 1, (30, 20))
+    C = disk.DiskTensor(tmp_path / "k0.h5", "C", (30, 20))
+    C.put(C0)
+    A = torch.zeros((30, 0), dtype=torch.float64, device="cuda")
+    B = torch.zeros((0, 20), dtype=torch.float64, device="cuda")
+    disk.gemm_disk_c("N", "N", 1.0, A, B, -2.0, C, panel_bytes=7 * 20 * 8)
+    assert np.array_equal(C.get(), -2.0 * C0)
+    E = disk.DiskTensor(tmp_path / "k0.h5", "E", (0, 20))
+    disk.gemm_disk_c("N", "N", 1.0, torch.zeros((0, 4), dtype=torch.float64, device="cuda"),
+                     torch.zeros((4, 20), dtype=torch.float64, device="cuda"), 0.0, E)
+    assert E.get().shape == (0, 20)
+
+
 def test_gemm_disk_refusals(disk, core, tmp_path):
     A = torch.rand(4, 5, dtype=torch.float64, device="cuda")
     B = torch.rand(5, 6, dtype=torch.float64, device="cuda")
