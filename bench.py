@@ -98,8 +98,8 @@ def rocprof_figure(path, flop):
 
 
 def host_cpu_info():
-    """(threads to use, CPU model). The GPU box allots 16 host CPUs to its one GPU whatever
-    `nproc` reports there, so the oracle uses at most 16 threads (this container: 8)."""
+    """(threads to use, CPU model). A one-GPU MI355X host share is 16 CPUs whatever `nproc`
+    reports, so the oracle uses at most 16 threads (fewer where fewer are available)."""
     cores = min(len(os.sched_getaffinity(0)), 16)
     model = "unknown"
     try:
