@@ -41,6 +41,9 @@ def parse():
     p.add_argument("--split", type=int, default=1,
                    help="1: add the split-precision vs FP64 block at o=48, v=256 (checked against the oracle in "
                         "the CPU leg), 0: skip it")
+    p.add_argument("--hbm", type=int, default=1,
+                   help="1: add the HBM-bound kernels at sizes above the 256 MB cache (nbf=96, o=48 v=256) "
+                        "to the line, 0: skip them")
     p.add_argument("--kernel-trace", type=str, default=str(ROOT / "profiles" / "ccd_step_o24_v96_kernel_trace.txt"),
                    help="rocprof kernel-trace summary of this bench holding the Wabef kernel's average duration")
     p.add_argument("--grouped", action="store_true",
@@ -411,14 +414,15 @@ def precision_sweep(sizes=(1024, 2048, 4096, 8192, 16384), reps=3, cases=("dgemm
     return out
 
 
-def bandwidth_bench(reps=20):
+def bandwidth_bench(reps=20, quiet=False, large_only=False):
     """HBM-bound kernels against the 8 TB/s peak: the HF J (GEMV) and K (generic) builds at
     nbf = 48 and 96, G = 2J - K both as the two einsum calls and as the fused J/K build, the CCD energy dot and the ijab->iajb permute at o=24, v=96 and
     o=48, v=256, and a plain copy. Each is timed warm (back-to-back calls, operands may sit
     in the 256 MB infinity cache) and cold (a 512 MB scratch read between calls, timed
     separately: a scratch write would leave dirty lines whose write-back then competes with
     the timed call); bytes are the algorithmic ones: every operand element read once, the
-    output written once (read too when beta != 0)."""
+    output written once (read too when beta != 0). quiet: return the rows without printing
+    them; large_only: only the sizes above the 256 MB cache (nbf = 96, o=48 v=256)."""
     import numpy as np
     import torch
 
@@ -458,12 +462,13 @@ def bandwidth_bench(reps=20):
                "warm_TBs": round(nbytes / warm / 1e9, 3)}
         if alg is not None:
             row["algorithm"] = alg
-        print(json.dumps(row), flush=True)
+        if not quiet:
+            print(json.dumps(row), flush=True)
         out.append(row)
 
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
     rng = np.random.default_rng(3)
-    for nbf in (48, 96):
+    for nbf in ((96,) if large_only else (48, 96)):
         g = dev(rng.standard_normal((nbf,) * 4))
         D = dev(rng.standard_normal((nbf, nbf)))
         J = torch.empty((nbf, nbf), dtype=torch.float64, device="cuda")
@@ -484,7 +489,7 @@ def bandwidth_bench(reps=20):
         report(f"HF G = 2J - K nbf={nbf}, two einsum calls", 2 * nb + 8 * nbf ** 2, two_calls, "GEMV + GENERIC")
         report(f"HF G = 2J - K nbf={nbf}, fused J/K build", nb, lambda: core.fock_jk(g, D, K), "FOCK_JK (one pass over g)")
         del g, D, J, K
-    for (o, v) in ((24, 96), (48, 256)):
+    for (o, v) in (((48, 256),) if large_only else ((24, 96), (48, 256))):
         t = torch.rand((o, o, v, v), dtype=torch.float64, device="cuda")
         gg = torch.rand((o, o, v, v), dtype=torch.float64, device="cuda")
         E = torch.zeros((), dtype=torch.float64, device="cuda")
@@ -864,6 +869,15 @@ def main():
         out["roofline"]["rocprof"] = prof
     hf = hf_block(core)
     out["hf"] = hf
+    if args.hbm:
+        # the HBM-bound kernels above the 256 MB cache, cold and warm (--bandwidth has all sizes)
+        rows = bandwidth_bench(reps=10, quiet=True, large_only=True)
+        torch.cuda.empty_cache()
+        out["hbm"] = {"workload": "HBM-bound kernels above the 256 MB cache: HF J/K/fused G at nbf=96, E dot, "
+                                  "permutes and a copy at o=48 v=256 (10 cold calls each, 512 MB scratch read between)",
+                      "peak_TBs": HBM_PEAK_GBS / 1e3,
+                      "cases": {r["case"]: {"cold_us": r["cold_us"], "cold_frac": r["cold_frac"],
+                                            "warm_frac": round(r["warm_TBs"] / 8.0, 3)} for r in rows}}
     large = None
     if args.split:
         large = split_large(core)
