@@ -36,6 +36,7 @@ __all__ = [
 MAP, COPY, DEVICE_TENSOR = 0, 1, 2
 
 _SUPPORTED = (torch.float64, torch.float32)
+_COMPLEX = (torch.complex128, torch.complex64)
 
 
 def gpu_enabled() -> bool:
@@ -66,7 +67,7 @@ class GPUView:
 
     def __init__(self, array, mode: int = COPY):
         if isinstance(array, torch.Tensor) and array.is_cuda:
-            if array.dtype not in _SUPPORTED:
+            if array.dtype not in _SUPPORTED + _COMPLEX:
                 raise TypeError(f"GPUView: unsupported dtype {array.dtype}")
             self._host, self._array, self.mode, self.tensor = None, array, DEVICE_TENSOR, array
             return
@@ -78,7 +79,7 @@ class GPUView:
             host = array
         else:
             host = torch.from_numpy(np.asarray(array))
-        if host.dtype not in _SUPPORTED:
+        if host.dtype not in _SUPPORTED + _COMPLEX:
             raise TypeError(f"GPUView: unsupported dtype {host.dtype}")
         self._host = host
         self._array = array
@@ -180,8 +181,16 @@ class EinsumGenericPlan:
         dA, _ = _as_device(A, write=False)
         dB, _ = _as_device(B, write=False)
         dtype = dC.dtype
+        if dtype in _COMPLEX and dA.dtype == dtype and dB.dtype == dtype:
+            self._execute_complex(C_prefactor, dC, AB_prefactor, dA, dB, split, slice_type)
+            if finC is not None:
+                torch.cuda.current_stream().synchronize()
+                finC()
+            return
         if dtype not in _SUPPORTED or dA.dtype != dtype or dB.dtype != dtype:
-            raise TypeError(f"execute: operands must share float64 or float32, got {dC.dtype}, {dA.dtype}, {dB.dtype}")
+            raise TypeError(f"execute: operands must share float64, float32, complex128 or complex64, got {dC.dtype}, "
+                            f"{dA.dtype}, {dB.dtype}")
+        C_prefactor, AB_prefactor = _real_prefactor(C_prefactor), _real_prefactor(AB_prefactor)
         cd, ad, bd = _desc(dC, self.C_indices), _desc(dA, self.A_indices), _desc(dB, self.B_indices)
         used = ctypes.c_int32(-1)
         if split:
@@ -200,6 +209,59 @@ class EinsumGenericPlan:
         if finC is not None:
             torch.cuda.current_stream().synchronize()
             finC()
+
+
+    def _execute_complex(self, C_prefactor, dC, AB_prefactor, dA, dB, split, slice_type) -> None:
+        """Complex operands as real einsums on the real and imaginary parts (strided real
+        views, the same kernels): with P = sum A B, Re P = Re A Re B - Im A Im B and
+        Im P = Re A Im B + Im A Re B, and C = c C + a P for complex c and a. A complex c
+        first forms c C from a copy of C (identity permutes), then the products accumulate."""
+        c, a = complex(C_prefactor), complex(AB_prefactor)
+        Cr, Ci = _parts(dC)
+        Ar, Ai = _parts(dA)
+        Br, Bi = _parts(dB)
+        if c.imag != 0.0:
+            _complex_scale(c, dC, Cr, Ci)
+            beta = 1.0
+        else:
+            beta = c.real
+        # (target part, [(prefactor, A part, B part)]): Re C += a_r Re P - a_i Im P, Im C += a_r Im P + a_i Re P
+        terms = [(Cr, [(a.real, Ar, Br), (-a.real, Ai, Bi), (-a.imag, Ar, Bi), (-a.imag, Ai, Br)]),
+                 (Ci, [(a.real, Ar, Bi), (a.real, Ai, Br), (a.imag, Ar, Br), (-a.imag, Ai, Bi)])]
+        for target, parts in terms:
+            first = True
+            for pre, x, y in parts:
+                if pre == 0.0 and not first:
+                    continue
+                self._execute_one(beta if first else 1.0, target, pre, x, y, split, slice_type)
+                first = False
+
+
+def _real_prefactor(x) -> float:
+    if isinstance(x, complex):
+        if x.imag != 0.0:
+            raise TypeError("a complex prefactor needs complex operands")
+        return x.real
+    return float(x)
+
+
+def _parts(t: torch.Tensor):
+    """Real and imaginary parts of a complex tensor as strided real views of its storage."""
+    r = torch.view_as_real(t)
+    return r[..., 0], r[..., 1]
+
+
+def _complex_scale(c: complex, dC: torch.Tensor, Cr: torch.Tensor, Ci: torch.Tensor) -> None:
+    """C = c C for complex c, through identity permutes of the parts (and a copy of C)."""
+    idx = "abcdefghijklmnop"[:dC.dim()]
+    T = torch.empty_like(dC)
+    Tr, Ti = _parts(T)
+    _permute_one(0.0, idx, Tr, 1.0, idx, Cr)
+    _permute_one(0.0, idx, Ti, 1.0, idx, Ci)
+    _permute_one(0.0, idx, Cr, c.real, idx, Tr)
+    _permute_one(1.0, idx, Cr, -c.imag, idx, Ti)
+    _permute_one(0.0, idx, Ci, c.real, idx, Ti)
+    _permute_one(1.0, idx, Ci, c.imag, idx, Tr)
 
 
 class EinsumDotPlan(EinsumGenericPlan):
@@ -261,8 +323,27 @@ def _permute_one(C_prefactor, C_indices, C, A_prefactor, A_indices, A) -> None:
     stream = None  # the current stream, set by permute()
     dC, finC = _as_device(C, write=True)
     dA, _ = _as_device(A, write=False)
+    if dC.dtype == dA.dtype and dC.dtype in _COMPLEX:
+        # the parts move independently; a complex C_prefactor or A_prefactor mixes them
+        c, a = complex(C_prefactor), complex(A_prefactor)
+        Cr, Ci = _parts(dC)
+        Ar, Ai = _parts(dA)
+        if c.imag != 0.0:
+            _complex_scale(c, dC, Cr, Ci)
+            beta = 1.0
+        else:
+            beta = c.real
+        for target, parts in ((Cr, [(a.real, Ar), (-a.imag, Ai)]), (Ci, [(a.real, Ai), (a.imag, Ar)])):
+            _permute_one(beta, C_indices, target, parts[0][0], A_indices, parts[0][1])
+            if parts[1][0] != 0.0:
+                _permute_one(1.0, C_indices, target, parts[1][0], A_indices, parts[1][1])
+        if finC is not None:
+            torch.cuda.current_stream().synchronize()
+            finC()
+        return
     if dC.dtype != dA.dtype or dC.dtype not in _SUPPORTED:
-        raise TypeError("permute: operands must share float64 or float32")
+        raise TypeError("permute: operands must share float64, float32, complex128 or complex64")
+    C_prefactor, A_prefactor = _real_prefactor(C_prefactor), _real_prefactor(A_prefactor)
     r = dA.dim()
     if dC.dim() != r or len(A_indices) != r:
         raise RankError("permute: rank mismatch")
