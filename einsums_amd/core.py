@@ -215,11 +215,17 @@ class EinsumGenericPlan:
         """Complex operands as real einsums on the real and imaginary parts (strided real
         views, the same kernels): with P = sum A B, Re P = Re A Re B - Im A Im B and
         Im P = Re A Im B + Im A Re B, and C = c C + a P for complex c and a. A complex c
-        first forms c C from a copy of C (identity permutes), then the products accumulate."""
+        first forms c C from a copy of C (identity permutes), then the products accumulate.
+        For a GEMM plan the parts of A and B are first packed into contiguous planes."""
         c, a = complex(C_prefactor), complex(AB_prefactor)
         Cr, Ci = _parts(dC)
         Ar, Ai = _parts(dA)
         Br, Bi = _parts(dB)
+        if self.algorithm == GEMM:
+            # the GEMM stages stride-2 operands through its slow element path: each part is
+            # first copied into a contiguous plane by the permute
+            Ar, Ai = _packed(Ar, self.A_indices), _packed(Ai, self.A_indices)
+            Br, Bi = _packed(Br, self.B_indices), _packed(Bi, self.B_indices)
         if c.imag != 0.0:
             _complex_scale(c, dC, Cr, Ci)
             beta = 1.0
@@ -249,6 +255,13 @@ def _parts(t: torch.Tensor):
     """Real and imaginary parts of a complex tensor as strided real views of its storage."""
     r = torch.view_as_real(t)
     return r[..., 0], r[..., 1]
+
+
+def _packed(x: torch.Tensor, idx: str) -> torch.Tensor:
+    """A contiguous copy of the real view x, made by the identity permute."""
+    out = torch.empty(x.shape, dtype=x.dtype, device=x.device)
+    _permute_one(0.0, idx, out, 1.0, idx, x)
+    return out
 
 
 def _complex_scale(c: complex, dC: torch.Tensor, Cr: torch.Tensor, Ci: torch.Tensor) -> None:
