@@ -7,6 +7,7 @@
 #pragma once
 
 #include <array>
+#include <complex>
 #include <cstdint>
 #include <memory>
 #include <numeric>
@@ -59,6 +60,45 @@ inline ein_stream_t &stream() {
 inline void set_stream(ein_stream_t s) { stream() = s; }
 inline void stream_wait() { check(ein_stream_synchronize(stream()), "stream_wait"); }
 } // namespace gpu
+
+namespace detail {
+// Complex element types run as real operations on the real and imaginary parts: views of
+// the same storage with every stride doubled (in real elements), the imaginary part one
+// real element in.
+template <typename T>
+struct is_complex : std::false_type {};
+template <typename R>
+struct is_complex<std::complex<R>> : std::true_type {};
+template <typename T>
+inline constexpr bool is_complex_v = is_complex<T>::value;
+template <typename T>
+struct real_of {
+    using type = T;
+};
+template <typename R>
+struct real_of<std::complex<R>> {
+    using type = R;
+};
+template <typename T>
+using real_t = typename real_of<T>::type;
+
+inline ein_tensor_desc part_desc(ein_tensor_desc d) {
+    for (int32_t i = 0; i < d.rank; ++i) d.strides[i] *= 2;
+    return d;
+}
+
+// C = beta*C + alpha*A over one real part, element for element (an identity permute).
+template <typename R>
+int part_axpby(ein_tensor_desc const &d, R alpha, R const *a, ein_tensor_desc const &da, R beta, R *c,
+               ein_stream_t s) {
+    int32_t perm[EIN_MAX_RANK];
+    for (int32_t i = 0; i < d.rank; ++i) perm[i] = i;
+    if constexpr (std::is_same_v<R, double>)
+        return ein_permute_f64(d.rank, d.dims, perm, alpha, a, da.strides, beta, c, d.strides, s);
+    else
+        return ein_permute_f32(d.rank, d.dims, perm, alpha, a, da.strides, beta, c, d.strides, s);
+}
+} // namespace detail
 
 struct Range {
     int64_t first, last; // [first, last)
@@ -116,10 +156,13 @@ class DeviceTensorBase {
     void zero() { scale(T(0)); }
     void scale(T alpha) {
         auto d = desc(nullptr);
-        if constexpr (std::is_same_v<T, double>)
+        if constexpr (detail::is_complex_v<T>) {
+            scale_complex(alpha);
+        } else if constexpr (std::is_same_v<T, double>) {
             check(ein_tensor_scale_f64(&d, ptr_, alpha, gpu::stream()), "scale");
-        else
+        } else {
             check(ein_tensor_scale_f32(&d, ptr_, alpha, gpu::stream()), "scale");
+        }
     }
     DeviceTensorBase &operator*=(T alpha) {
         scale(alpha);
@@ -168,6 +211,46 @@ class DeviceTensorBase {
     DeviceTensorView<T, R> operator()(Args... args) const;
 
   protected:
+    // C = alpha*C for complex elements: a real alpha scales both parts; a complex one mixes
+    // them, through a contiguous copy of the two parts.
+    void scale_complex(T alpha) {
+        using Re   = detail::real_t<T>;
+        auto pd    = detail::part_desc(desc(nullptr));
+        Re *re     = reinterpret_cast<Re *>(ptr_);
+        Re *im     = re + 1;
+        if (alpha.imag() == Re(0)) {
+            if constexpr (std::is_same_v<Re, double>) {
+                check(ein_tensor_scale_f64(&pd, re, alpha.real(), gpu::stream()), "scale");
+                check(ein_tensor_scale_f64(&pd, im, alpha.real(), gpu::stream()), "scale");
+            } else {
+                check(ein_tensor_scale_f32(&pd, re, alpha.real(), gpu::stream()), "scale");
+                check(ein_tensor_scale_f32(&pd, im, alpha.real(), gpu::stream()), "scale");
+            }
+            return;
+        }
+        int64_t const n = size();
+        if (n == 0) return;
+        void *tmp = nullptr;
+        check(ein_malloc(&tmp, size_t(n) * sizeof(T)), "scale workspace");
+        std::shared_ptr<void> hold(tmp, [](void *q) { (void)ein_free_async(q, gpu::stream()); });
+        ein_tensor_desc td = desc(nullptr);
+        int64_t st = 1;
+        for (int d = int(R) - 1; d >= 0; --d) {
+            td.strides[d] = st;
+            st *= dims_[size_t(d)];
+        }
+        td       = detail::part_desc(td);
+        Re *tre  = static_cast<Re *>(tmp);
+        Re *tim  = tre + 1;
+        auto run = [&](int status) { check(status, "scale"); };
+        run(detail::part_axpby<Re>(td, Re(1), re, pd, Re(0), tre, gpu::stream()));
+        run(detail::part_axpby<Re>(td, Re(1), im, pd, Re(0), tim, gpu::stream()));
+        run(detail::part_axpby<Re>(pd, alpha.real(), tre, td, Re(0), re, gpu::stream()));
+        run(detail::part_axpby<Re>(pd, -alpha.imag(), tim, td, Re(1), re, gpu::stream()));
+        run(detail::part_axpby<Re>(pd, alpha.real(), tim, td, Re(0), im, gpu::stream()));
+        run(detail::part_axpby<Re>(pd, alpha.imag(), tre, td, Re(1), im, gpu::stream()));
+    }
+
     std::string name_;
     T *ptr_ = nullptr;
     std::array<int64_t, R> dims_{};

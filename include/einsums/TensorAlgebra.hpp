@@ -212,6 +212,55 @@ auto deref_c(T *p) {
     else return p;
 }
 
+// C = c*C + a * sum op(A) op(B) for complex elements (op = conj where ConjA / ConjB), as
+// real einsums on the parts: with A = Ar + i sA Ai and B = Br + i sB Bi (s = -1 for a
+// conjugated operand), Re P = Ar Br - sA sB Ai Bi and Im P = sB Ar Bi + sA Ai Br; the terms
+// accumulate into the parts of C with beta = 1 after the first. A complex c first scales C.
+template <typename R>
+int real_einsum(int32_t alg, R cp, ein_tensor_desc const &cd, R *c, R abp, ein_tensor_desc const &ad, R const *a,
+                ein_tensor_desc const &bd, R const *b, int32_t *used) {
+    if constexpr (std::is_same_v<R, double>)
+        return ein_einsum_execute_f64(alg, cp, &cd, c, abp, &ad, a, &bd, b, used, gpu::stream());
+    else
+        return ein_einsum_execute_f32(alg, cp, &cd, c, abp, &ad, a, &bd, b, used, gpu::stream());
+}
+
+template <bool ConjA, bool ConjB, typename CType, typename T>
+void complex_einsum(int32_t alg, T c, ein_tensor_desc const &cd, CType *C, T a, ein_tensor_desc const &ad, T const *A,
+                    ein_tensor_desc const &bd, T const *B, int32_t *used) {
+    using R            = ::einsums::detail::real_t<T>;
+    R const sA         = ConjA ? R(-1) : R(1);
+    R const sB         = ConjB ? R(-1) : R(1);
+    ein_tensor_desc pc = ::einsums::detail::part_desc(cd), pa = ::einsums::detail::part_desc(ad), pb = ::einsums::detail::part_desc(bd);
+    R *Cr              = reinterpret_cast<R *>(C->data());
+    R const *Ar        = reinterpret_cast<R const *>(A);
+    R const *Br        = reinterpret_cast<R const *>(B);
+    R beta             = c.real();
+    if (c.imag() != R(0)) {
+        C->scale(c);
+        beta = R(1);
+    }
+    struct Term {
+        R pre;
+        R const *x;
+        R const *y;
+    };
+    Term const re[4] = {{a.real(), Ar, Br}, {-a.real() * sA * sB, Ar + 1, Br + 1}, {-a.imag() * sB, Ar, Br + 1},
+                        {-a.imag() * sA, Ar + 1, Br}};
+    Term const im[4] = {{a.real() * sB, Ar, Br + 1}, {a.real() * sA, Ar + 1, Br}, {a.imag(), Ar, Br},
+                        {-a.imag() * sA * sB, Ar + 1, Br + 1}};
+    for (int part = 0; part < 2; ++part) {
+        Term const *terms = part == 0 ? re : im;
+        R *target         = Cr + part;
+        for (int t = 0; t < 4; ++t) {
+            if (t > 0 && terms[t].pre == R(0)) continue;
+            check(real_einsum<R>(alg, t == 0 ? beta : R(1), pc, target, terms[t].pre, pa, terms[t].x, pb, terms[t].y,
+                                 used),
+                  "einsum");
+        }
+    }
+}
+
 } // namespace detail
 
 template <bool ConjA = false, bool ConjB = false, detail::DeviceTensorConcept CType, detail::DeviceTensorConcept AType,
@@ -222,8 +271,9 @@ void einsum(U const C_prefactor, std::tuple<CIndices...> const &C_indices, CType
     using T = typename CType::ValueType;
     static_assert(std::is_same_v<T, typename AType::ValueType> && std::is_same_v<T, typename BType::ValueType>,
                   "mixed element types are not supported on the device path");
-    static_assert(std::is_same_v<T, double> || std::is_same_v<T, float>, "FP64 and FP32 device tensors only");
-    static_assert(!ConjA && !ConjB, "conjugation applies to complex tensors, which this backend does not provide");
+    static_assert(std::is_same_v<::einsums::detail::real_t<T>, double> || std::is_same_v<::einsums::detail::real_t<T>, float>,
+                  "double, float and their complex types only");
+    static_assert(::einsums::detail::is_complex_v<T> || (!ConjA && !ConjB), "conjugation applies to complex tensors");
     static_assert(sizeof...(CIndices) == CType::Rank, "Rank of C does not match Indices given for C.");
     static_assert(sizeof...(AIndices) == AType::Rank, "Rank of A does not match Indices given for A.");
     static_assert(sizeof...(BIndices) == BType::Rank, "Rank of B does not match Indices given for B.");
@@ -238,14 +288,19 @@ void einsum(U const C_prefactor, std::tuple<CIndices...> const &C_indices, CType
     (void)B_indices;
     ein_tensor_desc cd = C->desc(c_ids.data()), ad = A.desc(a_ids.data()), bd = B.desc(b_ids.data());
     int32_t used = -1;
-    int st;
-    if constexpr (std::is_same_v<T, double>)
-        st = ein_einsum_execute_f64(int32_t(alg), double(C_prefactor), &cd, C->data(), double(AB_prefactor), &ad, A.data(),
-                                    &bd, B.data(), &used, gpu::stream());
-    else
-        st = ein_einsum_execute_f32(int32_t(alg), float(C_prefactor), &cd, C->data(), float(AB_prefactor), &ad, A.data(),
-                                    &bd, B.data(), &used, gpu::stream());
-    check(st, "einsum");
+    if constexpr (::einsums::detail::is_complex_v<T>) {
+        detail::complex_einsum<ConjA, ConjB>(int32_t(alg), T(C_prefactor), cd, C, T(AB_prefactor), ad, A.data(), bd,
+                                             B.data(), &used);
+    } else {
+        int st;
+        if constexpr (std::is_same_v<T, double>)
+            st = ein_einsum_execute_f64(int32_t(alg), double(C_prefactor), &cd, C->data(), double(AB_prefactor), &ad,
+                                        A.data(), &bd, B.data(), &used, gpu::stream());
+        else
+            st = ein_einsum_execute_f32(int32_t(alg), float(C_prefactor), &cd, C->data(), float(AB_prefactor), &ad,
+                                        A.data(), &bd, B.data(), &used, gpu::stream());
+        check(st, "einsum");
+    }
     if (algorithm_choice != nullptr) *algorithm_choice = detail::AlgorithmChoice(used);
 }
 
@@ -361,14 +416,44 @@ void permute(U const C_prefactor, std::tuple<CIndices...> const &, CType *C, U c
         sC[i] = C->stride(i);
         if (C->dim(i) != A.dim(size_t(perm[i]))) throw dimension_error("permute: extents do not match");
     }
-    int st;
-    if constexpr (std::is_same_v<T, double>)
-        st = ein_permute_f64(int32_t(R), dA.data(), perm.data(), double(A_prefactor), A.data(), sA.data(),
-                             double(C_prefactor), C->data(), sC.data(), gpu::stream());
-    else
-        st = ein_permute_f32(int32_t(R), dA.data(), perm.data(), float(A_prefactor), A.data(), sA.data(),
-                             float(C_prefactor), C->data(), sC.data(), gpu::stream());
-    check(st, "permute");
+    if constexpr (::einsums::detail::is_complex_v<T>) {
+        // the parts move independently; complex prefactors mix them
+        using Re  = ::einsums::detail::real_t<T>;
+        T const c = T(C_prefactor), a = T(A_prefactor);
+        Re beta   = c.real();
+        if (c.imag() != Re(0)) {
+            C->scale(c);
+            beta = Re(1);
+        }
+        std::array<int64_t, R == 0 ? 1 : R> pA{}, pC{};
+        for (size_t i = 0; i < R; ++i) {
+            pA[i] = 2 * sA[i];
+            pC[i] = 2 * sC[i];
+        }
+        Re const *Ar = reinterpret_cast<Re const *>(A.data());
+        Re *Cr       = reinterpret_cast<Re *>(C->data());
+        auto run     = [&](Re alpha, Re const *x, Re b, Re *y) {
+            int rc;
+            if constexpr (std::is_same_v<Re, double>)
+                rc = ein_permute_f64(int32_t(R), dA.data(), perm.data(), alpha, x, pA.data(), b, y, pC.data(), gpu::stream());
+            else
+                rc = ein_permute_f32(int32_t(R), dA.data(), perm.data(), alpha, x, pA.data(), b, y, pC.data(), gpu::stream());
+            check(rc, "permute");
+        };
+        run(a.real(), Ar, beta, Cr); // Re C = beta Re C + a_r Re A - a_i Im A
+        if (a.imag() != Re(0)) run(-a.imag(), Ar + 1, Re(1), Cr);
+        run(a.real(), Ar + 1, beta, Cr + 1); // Im C = beta Im C + a_r Im A + a_i Re A
+        if (a.imag() != Re(0)) run(a.imag(), Ar, Re(1), Cr + 1);
+    } else {
+        int st;
+        if constexpr (std::is_same_v<T, double>)
+            st = ein_permute_f64(int32_t(R), dA.data(), perm.data(), double(A_prefactor), A.data(), sA.data(),
+                                 double(C_prefactor), C->data(), sC.data(), gpu::stream());
+        else
+            st = ein_permute_f32(int32_t(R), dA.data(), perm.data(), float(A_prefactor), A.data(), sA.data(),
+                                 float(C_prefactor), C->data(), sC.data(), gpu::stream());
+        check(st, "permute");
+    }
 }
 
 // Host tensors: staged through device memory like the host einsum.

@@ -45,6 +45,14 @@ def test_disk_tensor_program_gpu():
 
 
 @pytest.mark.gpu
+def test_complex_program():
+    """Complex einsum (GEMM with conjugated operands, generic, dot), permute and scaling on
+    DeviceTensor<std::complex<T>> and a host Tensor einsum, against long-double loops."""
+    res = _run("test_complex", 120)
+    assert res.returncode == 0 and "PASS test_complex" in res.stdout, res.stdout + res.stderr
+
+
+@pytest.mark.gpu
 def test_device_einsum_program():
     res = _run("test_device_einsum", 120)
     assert res.returncode == 0, res.stdout + res.stderr
