@@ -195,6 +195,8 @@ std::vector<ein_contraction_item> block_items(T C_prefactor, std::array<int32_t,
                                               T AB_prefactor, std::array<int32_t, NA> const &a_ids, AType const &A,
                                               std::array<int32_t, NB> const &b_ids, BType const &B) {
     static_assert(is_block_v<AType> || is_block_v<BType>, "A or B must be a BlockDeviceTensor");
+    static_assert(std::is_same_v<T, double> || std::is_same_v<T, float>,
+                  "block and tiled einsums run on double and float (one grouped launch of real items)");
     auto const &blk = [&]() -> auto const & {
         if constexpr (is_block_v<AType>) return A;
         else return B;
