@@ -250,6 +250,9 @@ int einsum_execute(int alg, T cp, ein_tensor_desc const &C, T *c, T abp, ein_ten
         // C(i, j) += x(i) y(j); i = A's labels, j = B's labels (in C order)
         std::vector<int> cA = positions_of(C, Ai), cB = positions_of(C, Bi);
         if (cA.size() != size_t(A.rank) || cB.size() != size_t(B.rank)) break; // A or B has summed-only labels
+        // C has labels of its own (C is constant along them): the outer product would fill
+        // one slice only, the generic contraction broadcasts over them
+        if (cA.size() + cB.size() != size_t(C.rank)) break;
         if (!order_matches(A, std::vector<int>([&] {
                                std::vector<int> v(A.rank);
                                for (int d = 0; d < A.rank; ++d) v[d] = d;
@@ -371,7 +374,9 @@ int einsum_adjoint(int32_t wrt, T alpha, const ein_tensor_desc *C, const T *dC, 
         Gd.labels[d] = X.labels[d];
     }
     T const *y = wrt == 0 ? b : a;
-    // dX[x] = beta*dX + alpha * sum over labels not in X of dC[c] * Y[y]
+    // dX[x] = beta*dX + alpha * sum over labels not in X of dC[c] * Y[y]; a label of X found
+    // in neither C nor Y (summed in the forward pass) is a label of this product's output
+    // only, and dX is constant along it
     return einsum_execute<T>(EIN_ALG_INDETERMINATE, beta, Gd, g, alpha, *C, dC, Y, y, algorithm_used,
                                   as_stream(stream));
 }

@@ -6,9 +6,12 @@ B (a diagonal), strided operand and output views, prefactors and the element typ
 from a seed fixed by the case number, so a failure names a reproducible case. One case in
 four uses MFMA-sized extents (8..40), the rest 1..7 (extents of 1 included, and in about
 one small case in twelve an extent of 0). Half of the cases run through
-compile_plan/execute (every dispatch route), the rest through the generic contraction
-alone. The oracle computes the expected C in long double from the same values;
-C outside a strided output view must stay untouched. Tolerance: relative Frobenius 1e-13
+compile_plan/execute; those whose every label is a target or a link are checked against
+the oracle's dispatch on views with the same strides (the result, and on dense operands
+with no empty extent the algorithm it picks), the others against the oracle's
+long-double generic loop (labels found in one operand are summed, labels found in C only
+broadcast). The rest run through the generic contraction alone, against that loop. C
+outside a strided output view must stay untouched. Tolerance: relative Frobenius 1e-13
 (FP64), 1e-5 (FP32).
 """
 import zlib
@@ -94,15 +97,35 @@ def test_random_einsum_vs_oracle(core, oracle, case):
     c_base, c_t, c_idx = _operand(rng, Ci, ext, case["c_view"], dtype)
     A, B, C = a_t[a_idx], b_t[b_idx], c_t[c_idx]
 
-    ref = np.array(c_base[c_idx], dtype=np.float64, copy=True, order="C")
-    oracle.einsum_generic(case["cp"], Ci, ref, case["ab"], Ai,
-                          np.array(a_base[a_idx], dtype=np.float64, copy=True, order="C"), Bi,
-                          np.array(b_base[b_idx], dtype=np.float64, copy=True, order="C"), extended=True)
-
-    if case["route"] == "plan":
-        core.compile_plan(Ci, Ai, Bi).execute(case["cp"], C, case["ab"], A, B)
+    # labels in one operand only, or in C only: the BLAS-shaped routes do not take such a
+    # product (the reference's folded views, restated by the oracle's dispatch, would use
+    # one slice along them), so it is checked against the generic loop
+    plain = all(sum(l in s for s in (Ci, Ai, Bi)) >= 2 for l in set(Ci + Ai + Bi))
+    if case["route"] == "plan" and plain:
+        # the oracle's dispatch on views with the same strides: like the reference's, its
+        # choice (and a fallback to the generic loop) depends on them
+        ref_full = c_base.copy()
+        ref_view = ref_full[c_idx + (Ellipsis,)]
+        expect_alg = oracle.einsum(case["cp"], Ci, ref_view, case["ab"], Ai, a_base[a_idx + (Ellipsis,)], Bi,
+                                   b_base[b_idx + (Ellipsis,)])
+        ref = np.array(ref_view, dtype=np.float64)
+        plan = core.compile_plan(Ci, Ai, Bi)
+        plan.execute(case["cp"], C, case["ab"], A, B)
+        # the algorithm is compared on dense operands with no empty extent: the device's
+        # BLAS-shaped routes need one stride per folded group (other views run as the
+        # generic contraction, with the same result), and empty products run as the generic
+        # contraction
+        if case["views"] == ["dense", "dense"] and case["c_view"] == "dense" and 0 not in ext.values():
+            assert core.algorithm_name(plan.last_algorithm) == expect_alg
     else:
-        core.contract(Ci, C, Ai, A, Bi, B, C_prefactor=case["cp"], AB_prefactor=case["ab"])
+        ref = np.array(c_base[c_idx], dtype=np.float64, copy=True, order="C")
+        oracle.einsum_generic(case["cp"], Ci, ref, case["ab"], Ai,
+                              np.array(a_base[a_idx], dtype=np.float64, copy=True, order="C"), Bi,
+                              np.array(b_base[b_idx], dtype=np.float64, copy=True, order="C"), extended=True)
+        if case["route"] == "plan":
+            core.compile_plan(Ci, Ai, Bi).execute(case["cp"], C, case["ab"], A, B)
+        else:
+            core.contract(Ci, C, Ai, A, Bi, B, C_prefactor=case["cp"], AB_prefactor=case["ab"])
     out = host(c_t)
     tol = 1e-13 if dtype == np.float64 else 1e-5
     assert rel_fro(out[c_idx], ref) < tol
