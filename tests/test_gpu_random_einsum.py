@@ -133,3 +133,30 @@ def test_random_einsum_vs_oracle(core, oracle, case):
         outside = np.ones(c_base.shape, dtype=bool)
         outside[c_idx] = False
         assert np.array_equal(out[outside], c_base[outside])
+
+
+BIG = [c for c in CASES if c["seed"] % 4 == 3]
+
+
+@pytest.mark.parametrize("route", ["gather", "ttgt"])
+@pytest.mark.parametrize("case", BIG, ids=_case_id)
+def test_random_einsum_forced_routes(core, oracle, case, route):
+    """The MFMA-sized cases through each generic-contraction route forced (the strided-gather
+    MFMA kernel, and packing by the permute + dense GEMM), against the oracle's loop."""
+    rng = np.random.default_rng(zlib.crc32(f"random-einsum-route|{case['seed']}".encode()))
+    Ci, Ai, Bi, ext, dtype = case["C"], case["A"], case["B"], case["ext"], case["dtype"]
+    a_base, a_t, a_idx = _operand(rng, Ai, ext, case["views"][0], dtype)
+    b_base, b_t, b_idx = _operand(rng, Bi, ext, case["views"][1], dtype)
+    c_base, c_t, c_idx = _operand(rng, Ci, ext, case["c_view"], dtype)
+    ref = np.array(c_base[c_idx], dtype=np.float64, copy=True, order="C")
+    oracle.einsum_generic(case["cp"], Ci, ref, case["ab"], Ai,
+                          np.array(a_base[a_idx], dtype=np.float64, copy=True, order="C"), Bi,
+                          np.array(b_base[b_idx], dtype=np.float64, copy=True, order="C"), extended=True)
+    with core.contraction_route(route):
+        core.contract(Ci, c_t[c_idx], Ai, a_t[a_idx], Bi, b_t[b_idx], C_prefactor=case["cp"], AB_prefactor=case["ab"])
+    out = host(c_t)
+    assert rel_fro(out[c_idx], ref) < (1e-13 if dtype == np.float64 else 1e-5)
+    if case["c_view"] != "dense":
+        outside = np.ones(c_base.shape, dtype=bool)
+        outside[c_idx] = False
+        assert np.array_equal(out[outside], c_base[outside])
