@@ -121,3 +121,24 @@ def test_random_grouped_plan(core, oracle, plan):
         torch.cuda.synchronize()
         for k, (d, e) in enumerate(zip(dev, expect)):
             assert rel_fro(host(d), e) < 1e-13, f"tensor {k}, execution {rep}"
+
+
+@pytest.mark.parametrize("plan", PLANS[:40], ids=lambda p: f"{p['seed']}:{len(p['terms'])}terms:f32")
+def test_random_grouped_plan_fp32(core, oracle, plan):
+    """The first 40 plans in FP32 (one execution), against the oracle's long-double loop on
+    the same FP32 values: relative Frobenius 1e-4. A plan's terms feed one another (up to 24
+    in a row, each a sum of up to ~10^3 products rounded to FP32), so the rounding compounds
+    past the 1e-5 of a single FP32 contraction (measured up to 1.8e-5)."""
+    from einsums_amd.blocks import GroupedEinsum
+
+    rng = np.random.default_rng(zlib.crc32(f"random-grouped-data-f32|{plan['seed']}".encode()))
+    init = [rng.uniform(-1, 1, shape).astype(np.float32) for shape in plan["pool"]]
+    dev = [torch.from_numpy(x.copy()).to("cuda") for x in init]
+    expect = [x.astype(np.float64) for x in init]
+    for ci, c, ai, a, bi, b, cp, ab in plan["terms"]:
+        oracle.einsum_generic(cp, ci, expect[c], ab, ai, expect[a].copy(), bi, expect[b].copy(), extended=True)
+    g = GroupedEinsum([(ci, dev[c], ai, dev[a], bi, dev[b], cp, ab) for ci, c, ai, a, bi, b, cp, ab in plan["terms"]])
+    g.execute()
+    torch.cuda.synchronize()
+    for k, (d, e) in enumerate(zip(dev, expect)):
+        assert rel_fro(host(d), e) < 1e-4, f"tensor {k}"
